@@ -1,0 +1,164 @@
+"""ctypes binding of the CPU oracle (liboracle.so) -- test infrastructure only.
+
+Imported by tests/, __graft_entry__.smoke() (as the checker) and bench.py's
+cpu_baseline leg (as the timed CPU path).  Never imported by tweakseq_amd/.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("mode", "alphabet", "gap_open", "gap_extend", "match", "mismatch",
+                 "traceback", "lanes")]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("score", ctypes.c_int32), ("a_begin", ctypes.c_int32),
+                ("a_end", ctypes.c_int32), ("b_begin", ctypes.c_int32),
+                ("b_end", ctypes.c_int32), ("cigar_off", ctypes.c_int64),
+                ("cigar_len", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(path)
+        P = ctypes.POINTER
+        L.or_encode.argtypes = [P(ctypes.c_int32), ctypes.c_int32, ctypes.c_char_p,
+                                ctypes.c_int32, P(ctypes.c_uint8)]
+        L.or_align.argtypes = [P(Params), P(ctypes.c_int8), ctypes.c_int32,
+                               P(ctypes.c_uint8), ctypes.c_int32, P(ctypes.c_uint8),
+                               ctypes.c_int32, P(Result), ctypes.c_char_p, ctypes.c_int64]
+        L.or_align_batch.argtypes = [P(Params), P(ctypes.c_int8), ctypes.c_int32,
+                                     P(ctypes.c_uint8), P(ctypes.c_int64), P(ctypes.c_int32),
+                                     P(ctypes.c_int64), P(ctypes.c_int32), ctypes.c_int32,
+                                     P(Result), ctypes.c_char_p, P(ctypes.c_int64),
+                                     P(ctypes.c_int64), ctypes.c_int32]
+        L.or_rescore.argtypes = [P(Params), P(ctypes.c_int8), ctypes.c_int32,
+                                 P(ctypes.c_uint8), P(ctypes.c_uint8), P(Result),
+                                 ctypes.c_char_p, P(ctypes.c_int32)]
+        _LIB = L
+    return _LIB
+
+
+def blosum62_fixture():
+    """The reference's BLOSUM62 table + letter map, from tests/golden/blosum62.json."""
+    path = os.path.join(_HERE, "..", "tests", "golden", "blosum62.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def protein_matrix():
+    d = blosum62_fixture()
+    return np.array(d["matrix"], dtype=np.int8), np.array(d["map"], dtype=np.int32)
+
+
+def dna_matrix(match, mismatch):
+    m = np.full((4, 4), mismatch, dtype=np.int8)
+    np.fill_diagonal(m, match)
+    return m
+
+
+def params(mode=0, alphabet=0, gap_open=-10, gap_extend=-1, match=2, mismatch=-3,
+           traceback=1):
+    return Params(mode, alphabet, gap_open, gap_extend, match, mismatch, traceback, 0)
+
+
+def _u8(a):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def encode(s, alphabet=0):
+    _, cmap = protein_matrix()
+    raw = s.encode() if isinstance(s, str) else bytes(s)
+    out = np.zeros(max(len(raw), 1), dtype=np.uint8)
+    rc = lib().or_encode(cmap.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), alphabet,
+                         raw, len(raw), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    if rc:
+        raise ValueError("residue outside the alphabet")
+    return out[:len(raw)]
+
+
+def matrix_for(p):
+    if p.alphabet == 0:
+        return protein_matrix()[0]
+    return dna_matrix(p.match, p.mismatch)
+
+
+def align(p, a, b, smat=None):
+    """a, b: uint8 code arrays.  Returns (score, a_begin, a_end, b_begin, b_end, cigar)."""
+    if smat is None:
+        smat = matrix_for(p)
+    smat = np.ascontiguousarray(smat, dtype=np.int8)
+    a, pa = _u8(a)
+    b, pb = _u8(b)
+    res = Result()
+    cap = 2 * (len(a) + len(b)) + 16
+    buf = ctypes.create_string_buffer(cap)
+    rc = lib().or_align(ctypes.byref(p), smat.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                        smat.shape[0], pa, len(a), pb, len(b), ctypes.byref(res), buf, cap)
+    if rc:
+        raise RuntimeError("or_align failed: %d" % rc)
+    cig = buf.raw[:res.cigar_len].decode() if p.traceback else ""
+    return (res.score, res.a_begin, res.a_end, res.b_begin, res.b_end, cig)
+
+
+def align_batch(p, codes, a_off, la, b_off, lb, nthreads=0, with_cigar=True, smat=None):
+    """Many pairs over OpenMP.  Returns (results structured array, cigars list or None)."""
+    if smat is None:
+        smat = matrix_for(p)
+    smat = np.ascontiguousarray(smat, dtype=np.int8)
+    codes, pc = _u8(codes)
+    a_off = np.ascontiguousarray(a_off, dtype=np.int64)
+    b_off = np.ascontiguousarray(b_off, dtype=np.int64)
+    la = np.ascontiguousarray(la, dtype=np.int32)
+    lb = np.ascontiguousarray(lb, dtype=np.int32)
+    n = len(la)
+    res = (Result * n)()
+    P = ctypes.POINTER
+    if with_cigar and p.traceback:
+        cap = (2 * (la.astype(np.int64) + lb) + 16)
+        off = np.zeros(n, dtype=np.int64)
+        off[1:] = np.cumsum(cap)[:-1]
+        buf = ctypes.create_string_buffer(int(cap.sum()) + 1)
+        cptr, offp, capp = buf, off.ctypes.data_as(P(ctypes.c_int64)), cap.ctypes.data_as(P(ctypes.c_int64))
+    else:
+        buf, cptr, offp, capp = None, None, None, None
+    rc = lib().or_align_batch(ctypes.byref(p), smat.ctypes.data_as(P(ctypes.c_int8)), smat.shape[0],
+                              pc, a_off.ctypes.data_as(P(ctypes.c_int64)), la.ctypes.data_as(P(ctypes.c_int32)),
+                              b_off.ctypes.data_as(P(ctypes.c_int64)), lb.ctypes.data_as(P(ctypes.c_int32)),
+                              n, res, cptr, offp, capp, nthreads)
+    if rc:
+        raise RuntimeError("or_align_batch failed: %d" % rc)
+    cigars = None
+    if buf is not None:
+        raw = buf.raw
+        cigars = [raw[res[k].cigar_off:res[k].cigar_off + res[k].cigar_len].decode() for k in range(n)]
+    return res, cigars
+
+
+def rescore(p, a, b, result_tuple, smat=None):
+    if smat is None:
+        smat = matrix_for(p)
+    smat = np.ascontiguousarray(smat, dtype=np.int8)
+    a, pa = _u8(a)
+    b, pb = _u8(b)
+    score, ab, ae, bb, be, cig = result_tuple
+    r = Result(score, ab, ae, bb, be, 0, len(cig), 0)
+    out = ctypes.c_int32()
+    rc = lib().or_rescore(ctypes.byref(p), smat.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                          smat.shape[0], pa, pb, ctypes.byref(r), cig.encode(), ctypes.byref(out))
+    if rc:
+        raise ValueError("CIGAR does not consume the reported ranges")
+    return out.value

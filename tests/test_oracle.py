@@ -1,0 +1,140 @@
+"""Pins the CPU oracle: reference table fixture, brute force, closed forms,
+symmetry, the linear-gap special case and CIGAR re-scoring."""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+import bruteforce as bf
+
+S_PROT, MAP = po.protein_matrix()
+SP = S_PROT.astype(int).tolist()
+
+
+def rand_codes(rng, n, k=20):
+    return np.array([rng.randrange(k) for _ in range(n)], dtype=np.uint8)
+
+
+def test_fixture_is_blosum62():
+    d = po.blosum62_fixture()
+    M = np.array(d["matrix"])
+    assert d["order"] == "ARNDCQEGHILKMFPSTWYVBZX"
+    assert (M == M.T).all()
+    assert M.min() == -4 and M.max() == 11
+    assert list(np.diag(M)) == [4, 5, 6, 6, 9, 5, 5, 6, 8, 4, 4, 5, 5, 6, 7, 4, 5, 11, 7, 4, 4, 4, -1]
+    # A spot check against the published NCBI table (Henikoff & Henikoff 1992).
+    idx = {c: i for i, c in enumerate(d["order"])}
+    assert M[idx["W"], idx["C"]] == -2 and M[idx["H"], idx["Y"]] == 2 and M[idx["I"], idx["V"]] == 3
+    # Letter map: J, O, U, X -> X row; B, Z rows; every letter maps to its own row.
+    cmap = d["map"]
+    for c in d["order"]:
+        assert cmap[ord(c) - 65] == idx[c]
+    for c in "JOUX":
+        assert cmap[ord(c) - 65] == 22
+
+
+def test_encode():
+    assert list(po.encode("arNdX")) == [0, 1, 2, 3, 22]
+    assert list(po.encode("ACGTacgt", alphabet=1)) == [0, 1, 2, 3] * 2
+    with pytest.raises(ValueError):
+        po.encode("AC-D")
+    with pytest.raises(ValueError):
+        po.encode("ACGN", alphabet=1)
+
+
+@pytest.mark.parametrize("go,ge", [(-10, -1), (-5, -2), (-3, -3), (-11, -1), (-4, 0)])
+def test_global_bruteforce(go, ge):
+    rng = random.Random(1000 + go * 7 + ge)
+    p = po.params(mode=0, gap_open=go, gap_extend=ge)
+    for _ in range(60):
+        m, n = rng.randint(0, 5), rng.randint(0, 5)
+        k = rng.choice([3, 6, 23])  # small alphabets force ties
+        a, b = rand_codes(rng, m, k), rand_codes(rng, n, k)
+        got = po.align(p, a, b)
+        want = bf.global_align(list(a), list(b), SP, go, ge)
+        assert got[:5] == want[:5], (a, b, got, want)
+        if m and n:
+            assert got[5] == want[5], (a, b, got, want)
+
+
+@pytest.mark.parametrize("go,ge", [(-10, -1), (-5, -2), (-3, -3), (-11, -1), (-2, -1)])
+def test_local_bruteforce(go, ge):
+    rng = random.Random(2000 - go * 3 + ge)
+    p = po.params(mode=1, gap_open=go, gap_extend=ge)
+    for _ in range(40):
+        m, n = rng.randint(0, 5), rng.randint(0, 5)
+        k = rng.choice([3, 6, 23])
+        a, b = rand_codes(rng, m, k), rand_codes(rng, n, k)
+        got = po.align(p, a, b)
+        want = bf.local_align(list(a), list(b), SP, go, ge)
+        assert got == want, (a, b, got, want)
+
+
+def test_dna_bruteforce():
+    rng = random.Random(7)
+    p = po.params(mode=0, alphabet=1, gap_open=-5, gap_extend=-2, match=2, mismatch=-3)
+    S = po.dna_matrix(2, -3).astype(int).tolist()
+    for mode in (0, 1):
+        p.mode = mode
+        for _ in range(40):
+            a, b = rand_codes(rng, rng.randint(1, 5), 4), rand_codes(rng, rng.randint(1, 5), 4)
+            got = po.align(p, a, b)
+            fn = bf.global_align if mode == 0 else bf.local_align
+            assert got == fn(list(a), list(b), S, -5, -2)
+
+
+def test_closed_forms():
+    rng = random.Random(3)
+    p = po.params(mode=0)
+    assert po.align(p, [], []) == (0, 0, -1, 0, -1, "")
+    for la in (1, 2, 7, 50):
+        a = rand_codes(rng, la)
+        assert po.align(p, a, [])[0] == -10 + (la - 1) * -1
+        assert po.align(p, a, [])[5] == "%dI" % la
+        assert po.align(p, [], a)[5] == "%dD" % la
+    # Identical X-free sequences: gap-free alignment of the diagonal.
+    for la in (1, 10, 300):
+        a = rand_codes(rng, la)
+        diag = int(sum(SP[c][c] for c in a))
+        g = po.align(p, a, a)
+        assert g[0] == diag and g[5] == "%dM" % la
+        l = po.align(po.params(mode=1), a, a)
+        assert l[0] == diag and l[1:5] == (0, la - 1, 0, la - 1) and l[5] == "%dM" % la
+    # Local of unrelated sequences is never negative; empty when nothing scores.
+    assert po.align(po.params(mode=1), po.encode("WWW"), po.encode("GGG"))[0] == 0
+    assert po.align(po.params(mode=1), po.encode("WWW"), po.encode("GGG"))[1] == -1
+
+
+def test_linear_gap_equals_textbook_nw():
+    rng = random.Random(4)
+    for g in (-1, -4, -8):
+        p = po.params(mode=0, gap_open=g, gap_extend=g)
+        for _ in range(30):
+            a, b = rand_codes(rng, rng.randint(0, 40)), rand_codes(rng, rng.randint(0, 40))
+            assert po.align(p, a, b)[0] == bf.linear_nw(list(a), list(b), SP, g)
+
+
+def test_symmetry_and_rescoring():
+    rng = random.Random(5)
+    for mode in (0, 1):
+        p = po.params(mode=mode, gap_open=-11 if mode else -10)
+        for _ in range(40):
+            a, b = rand_codes(rng, rng.randint(1, 120)), rand_codes(rng, rng.randint(1, 120))
+            r = po.align(p, a, b)
+            assert po.align(p, b, a)[0] == r[0]
+            assert po.rescore(p, a, b, r) == r[0]
+
+
+def test_batch_matches_single():
+    rng = random.Random(6)
+    seqs = [rand_codes(rng, rng.randint(0, 80)) for _ in range(20)]
+    codes = np.concatenate([s for s in seqs] + [np.zeros(1, np.uint8)])
+    offs = np.cumsum([0] + [len(s) for s in seqs])[:-1]
+    pairs = [(i, j) for i in range(20) for j in range(i + 1, 20)]
+    p = po.params(mode=1, gap_open=-11)
+    res, cig = po.align_batch(p, codes, [offs[i] for i, _ in pairs], [len(seqs[i]) for i, _ in pairs],
+                              [offs[j] for _, j in pairs], [len(seqs[j]) for _, j in pairs], nthreads=4)
+    for k, (i, j) in enumerate(pairs):
+        one = po.align(p, seqs[i], seqs[j])
+        assert (res[k].score, res[k].a_begin, res[k].a_end, res[k].b_begin, res[k].b_end, cig[k]) == one
