@@ -1,0 +1,130 @@
+/*
+ * tweakalign -- MI355X (gfx950) pairwise alignment engine, plain C ABI.
+ *
+ * Replaces the external aligner process behind tweakseq's AlignmentTool
+ * boundary.  In the reference an alignment is: export FASTA
+ * (tweakseq/Core/Project.cpp:870-894), AlignmentTool::makeCommand
+ * (tweakseq/Core/AlignmentTool.h:54, Muscle.cpp:49-53), QProcess::start
+ * (tweakseq/UI/SeqEditMainWin.cpp:1660), success test on exit code + output
+ * file (SeqEditMainWin.cpp:836-861), ingest (Project.cpp:908-1032).  The
+ * calls below are what a host binds instead of spawning that process; the
+ * tweakalign CLI (tweakseq_amd/csrc/cli.cpp) wraps them behind the same
+ * "-in FILE -out FILE" contract for an unmodified host.
+ *
+ * Conventions: negative TA_E_* return codes, never abort, no C++ types or
+ * exceptions across the boundary.  Inputs are caller-owned and only read
+ * during the call (synchronous calls) or until ta_poll reports completion.
+ * A ta_ctx is used by one host thread at a time; several may share a GPU.
+ */
+#ifndef TWEAKALIGN_H
+#define TWEAKALIGN_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TA_OK 0
+#define TA_E_INVALID -1       /* bad argument */
+#define TA_E_ALPHABET -2      /* residue outside the alphabet */
+#define TA_E_SPACE -3         /* caller buffer too small; *used = bytes needed */
+#define TA_E_DEVICE -4        /* HIP runtime error */
+#define TA_E_NOMEM -5         /* allocation failed */
+#define TA_E_BUDGET -6        /* request exceeds the traceback-memory budget */
+#define TA_E_CANCELLED -7     /* request was cancelled */
+#define TA_E_TICKET -8        /* unknown ticket */
+#define TA_E_NODEVICE -9      /* no usable GPU */
+#define TA_E_PENDING 1        /* ta_poll: not finished yet (not an error) */
+
+#define TA_GLOBAL 0           /* Needleman-Wunsch (Gotoh affine gaps) */
+#define TA_LOCAL 1            /* Smith-Waterman (Gotoh affine gaps) */
+#define TA_PROTEIN 0          /* BLOSUM62, tweakseq/Core/Annotations/Consensus.cpp:34-69 */
+#define TA_DNA 1              /* A,C,G,T; match / mismatch scores */
+
+/* ta_params.lanes */
+#define TA_LANES_AUTO 0       /* packed int16 where safe, int32 otherwise */
+#define TA_LANES_I16 16       /* force packed int16 (saturating pairs rerun in int32) */
+#define TA_LANES_I32 32       /* force int32 */
+
+/* ta_result.flags */
+#define TA_F_I32 0x1          /* result computed by the int32 kernels */
+#define TA_F_RERUN 0x2        /* int16 pass saturated; rerun in int32 */
+#define TA_F_LONG 0x4         /* long-pair (multi-strip pipeline) path */
+#define TA_F_EMPTY 0x8        /* local: no cell scored above zero */
+
+typedef struct ta_ctx ta_ctx;
+
+typedef struct {
+  int32_t mode;       /* TA_GLOBAL | TA_LOCAL */
+  int32_t alphabet;   /* TA_PROTEIN | TA_DNA */
+  int32_t gap_open;   /* score of a 1-long gap, <= 0 (gap of k = open + (k-1)*extend) */
+  int32_t gap_extend; /* <= 0 */
+  int32_t match, mismatch; /* TA_DNA only */
+  int32_t traceback;  /* 0: score + coordinates; 1: also CIGAR */
+  int32_t lanes;      /* TA_LANES_* */
+} ta_params;
+
+typedef struct {
+  const char *a; int32_t la;   /* query (rows, CIGAR 'I' consumes it), ASCII letters */
+  const char *b; int32_t lb;   /* reference (columns, CIGAR 'D' consumes it) */
+} ta_pair;
+
+typedef struct {
+  int32_t score;
+  int32_t a_begin, a_end;      /* 0-based inclusive; -1 when empty (local) */
+  int32_t b_begin, b_end;
+  int64_t cigar_off;           /* offset of this pair's CIGAR in the caller buffer */
+  int32_t cigar_len;           /* bytes, no terminator */
+  int32_t flags;               /* TA_F_* */
+} ta_result;
+
+const char *ta_strerror(int code);
+
+/* One context per device per host thread. */
+int ta_create(ta_ctx **out, int device);
+void ta_destroy(ta_ctx *ctx);
+
+/* Synchronous batch: encode, upload, align on the GPU, return results and
+ * CIGARs (packed back to back in cigar_buf).  TA_E_SPACE with *used set when
+ * cap is too small (results are still written).  cigar_buf may be NULL when
+ * p->traceback == 0. */
+int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n,
+                   ta_result *out, char *cigar_buf, int64_t cap, int64_t *used);
+
+/* Device-resident batch: inputs encoded and uploaded once, the alignment
+ * run as often as wanted with inputs already in HBM. */
+typedef struct ta_batch ta_batch;
+int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n,
+                    ta_batch **out);
+int ta_batch_run(ta_ctx *ctx, ta_batch *b);         /* enqueue on the context stream */
+int ta_batch_run_plain(ta_ctx *ctx, ta_batch *b, int32_t first, int32_t count);
+                                                     /* one launch per pair (baseline) */
+int ta_batch_wait(ta_ctx *ctx, ta_batch *b);
+int ta_batch_results(ta_ctx *ctx, ta_batch *b, ta_result *out, char *cigar_buf,
+                     int64_t cap, int64_t *used);
+/* Kernel timings of the last ta_batch_run, milliseconds (HIP events on the
+ * context stream): [0] fill kernel(s), [1] int32 rerun, [2] traceback,
+ * [3] whole run.  Returns the number of values written. */
+int ta_batch_timings(ta_batch *b, double *ms, int32_t cap);
+/* Cells (sum of la*lb) and direction-plane bytes of the batch. */
+int ta_batch_info(ta_batch *b, int64_t *cells, int64_t *tb_bytes, int32_t *tasks);
+void ta_batch_destroy(ta_batch *b);
+
+/* Serving: continuous batching.  ta_submit copies the pairs and returns at
+ * once; requests that arrive while others run are admitted as traceback
+ * memory (budget set by ta_serve_config) frees up, and run concurrently. */
+int ta_serve_config(ta_ctx *ctx, int64_t tb_budget_bytes, int32_t max_inflight);
+int ta_submit(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n,
+              uint64_t *ticket);
+int ta_poll(ta_ctx *ctx, uint64_t ticket, ta_result *out, char *cigar_buf, int64_t cap,
+            int64_t *used);                          /* TA_OK done, TA_E_PENDING, <0 error */
+int ta_cancel(ta_ctx *ctx, uint64_t ticket);
+
+/* Substitution table in use for an alphabet (row-major K x K, int8). */
+int ta_matrix(int32_t alphabet, int32_t match, int32_t mismatch, int8_t *out, int32_t *K);
+/* Letter -> code, as the engine encodes input (TA_E_ALPHABET on others). */
+int ta_encode(int32_t alphabet, const char *s, int32_t len, uint8_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,173 @@
+"""GPU vs CPU oracle: scores, coordinates and CIGARs must be identical."""
+import random
+
+import numpy as np
+import pytest
+
+import tweakseq_amd as ta
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+AA = "ARNDCQEGHILKMFPSTWYV"
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = ta.Engine(0)
+    yield e
+    e.close()
+
+
+def rand_prot(rng, n, alphabet=AA):
+    return "".join(rng.choice(alphabet) for _ in range(n))
+
+
+def mutate(rng, s, psub=0.2, pindel=0.05, alphabet=AA):
+    out, i = [], 0
+    while i < len(s):
+        if rng.random() < pindel:
+            k = 1 + int(rng.expovariate(1 / 2.0))
+            if rng.random() < 0.5:
+                out += [rng.choice(alphabet) for _ in range(k)]
+            else:
+                i += k
+                continue
+        c = s[i]
+        if rng.random() < psub:
+            c = rng.choice(alphabet)
+        out.append(c)
+        i += 1
+    return "".join(out)
+
+
+def oracle_results(params, pairs):
+    p = po.params(mode=params.mode, alphabet=params.alphabet, gap_open=params.gap_open,
+                  gap_extend=params.gap_extend, match=params.match, mismatch=params.mismatch,
+                  traceback=1)
+    out = []
+    for a, b in pairs:
+        ea, eb = po.encode(a, params.alphabet), po.encode(b, params.alphabet)
+        out.append(po.align(p, ea, eb))
+    return out
+
+
+def compare(got, want, params, pairs, label):
+    bad = []
+    for k, (g, w) in enumerate(zip(got, want)):
+        gt = (g.score, g.a_begin, g.a_end, g.b_begin, g.b_end)
+        if gt != w[:5] or (params.traceback and g.cigar != w[5]):
+            bad.append((k, len(pairs[k][0]), len(pairs[k][1]), gt, w[:5], g.cigar[:60], w[5][:60], g.flags))
+    assert not bad, "%s: %d/%d mismatches, first: %r" % (label, len(bad), len(got), bad[:3])
+
+
+def check(eng, pairs, label, **kw):
+    params = ta.engine.make_params(**kw)
+    got = eng.align(pairs, params)
+    want = oracle_results(params, pairs)
+    compare(got, want, params, pairs, label)
+    return got
+
+
+EDGE_LENGTHS = [1, 2, 3, 7, 8, 9, 15, 16, 17, 31, 127, 128, 129, 255, 256, 257, 300]
+
+
+@pytest.mark.parametrize("mode", [ta.GLOBAL, ta.LOCAL])
+@pytest.mark.parametrize("lanes", [ta.LANES_AUTO, ta.LANES_I32])
+def test_edge_lengths(eng, mode, lanes):
+    rng = random.Random(11 + mode + lanes)
+    pairs = []
+    for la in EDGE_LENGTHS:
+        for lb in (1, 5, 16, 17, 130, la, la + 3):
+            a = rand_prot(rng, la)
+            b = mutate(rng, a) if rng.random() < 0.7 and lb == la else rand_prot(rng, lb)
+            pairs.append((a, b))
+    check(eng, pairs, "edge", mode=mode, lanes=lanes, gap_open=-11 if mode else -10)
+
+
+@pytest.mark.parametrize("mode", [ta.GLOBAL, ta.LOCAL])
+def test_random_related(eng, mode):
+    rng = random.Random(21 + mode)
+    pairs = []
+    for _ in range(300):
+        a = rand_prot(rng, rng.randint(20, 700))
+        b = mutate(rng, a, 0.25, 0.04)
+        pairs.append((a, b))
+    check(eng, pairs, "related", mode=mode, gap_open=-11 if mode else -10)
+
+
+def test_small_alphabet_ties(eng):
+    # Three letters make ties in every cell kind; tie-breaks must match exactly.
+    rng = random.Random(5)
+    pairs = [(rand_prot(rng, rng.randint(1, 90), "AGW"), rand_prot(rng, rng.randint(1, 90), "AGW"))
+             for _ in range(200)]
+    for mode in (ta.GLOBAL, ta.LOCAL):
+        for go, ge in ((-10, -1), (-3, -3), (-4, 0), (-6, -2)):
+            check(eng, pairs, "ties %d %d %d" % (mode, go, ge), mode=mode, gap_open=go, gap_extend=ge)
+
+
+def test_dna(eng):
+    rng = random.Random(8)
+    pairs = []
+    for _ in range(120):
+        a = rand_prot(rng, rng.randint(1, 900), "ACGT")
+        pairs.append((a, mutate(rng, a, 0.1, 0.02, "ACGT")))
+    for mode in (ta.GLOBAL, ta.LOCAL):
+        check(eng, pairs, "dna", mode=mode, alphabet=ta.DNA, gap_open=-5, gap_extend=-2, match=2,
+              mismatch=-3)
+
+
+def test_empty_and_letters(eng):
+    pairs = [("", ""), ("", "ACD"), ("WWW", ""), ("a", "A"), ("xjou", "XXXX"), ("bz", "DE")]
+    for mode in (ta.GLOBAL, ta.LOCAL):
+        check(eng, pairs, "empty", mode=mode)
+    with pytest.raises(RuntimeError):
+        eng.align([("AC-D", "ACD")], ta.engine.make_params())
+
+
+def test_score_only_global(eng):
+    rng = random.Random(3)
+    seqs = [rand_prot(rng, rng.randint(100, 600)) for _ in range(24)]
+    pairs = [(seqs[i], seqs[j]) for i in range(24) for j in range(i + 1, 24)]
+    params = ta.engine.make_params(mode=ta.GLOBAL, traceback=0)
+    got = eng.align(pairs, params)
+    want = oracle_results(params, pairs)
+    assert [g.score for g in got] == [w[0] for w in want]
+    assert all(g.cigar == "" for g in got)
+
+
+def test_int16_saturation_reruns_in_int32(eng):
+    # 3000 x 'W' against itself scores 33,000 > int16: the packed pass must
+    # flag it and the int32 rerun must give the exact result.
+    rng = random.Random(9)
+    w = "W" * 3000
+    pairs = [(w, w), (w[:2990] + "A" * 5, w), (rand_prot(rng, 400), rand_prot(rng, 380))]
+    got = check(eng, pairs, "saturate", mode=ta.LOCAL, gap_open=-11)
+    assert got[0].score == 33000 and got[0].flags & ta.F_RERUN
+    assert not got[2].flags & ta.F_RERUN
+
+
+def test_batched_equals_plain(eng):
+    rng = random.Random(13)
+    pairs = [(rand_prot(rng, rng.randint(30, 500)), rand_prot(rng, rng.randint(30, 500))) for _ in range(40)]
+    for mode in (ta.GLOBAL, ta.LOCAL):
+        params = ta.engine.make_params(mode=mode)
+        b = eng.batch(pairs, params)
+        b.run(); b.wait()
+        batched = b.results()
+        b.run_plain(); b.wait()
+        plain = b.results()
+        b.close()
+        assert batched == plain
+
+
+def test_workload_config1(eng):
+    w = ta.Workload(1)
+    a, b = w.pair_bytes(0)
+    check(eng, [(a, b)], "config1", mode=w.params.mode, gap_open=w.params.gap_open)
+
+
+def test_workload_config3_prefix(eng):
+    w = ta.Workload(3, size=1500)
+    pairs = [w.pair_bytes(k) for k in range(w.npairs)]
+    check(eng, pairs, "config3", mode=ta.LOCAL, gap_open=-11, gap_extend=-1)

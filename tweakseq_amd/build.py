@@ -1,0 +1,69 @@
+"""Build the native parts of tweakseq_amd in-tree (no JIT cache).
+
+  libtweakalign.so  HIP kernels (gfx950) + host runtime + C ABI
+  tweakalign        CLI honouring the external-aligner "-in FILE -out FILE" contract
+
+Usage: python -m tweakseq_amd.build   (or build() from __graft_entry__)
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libtweakalign.so")
+CLI = os.path.join(HERE, "tweakalign")
+ARCH = os.environ.get("TWEAKALIGN_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+    # packed-FP32 instruction forms stay off (none of this code is FP32)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
+]
+
+
+def run(cmd):
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if out.strip():
+        print(out)
+    if r.returncode:
+        raise RuntimeError("build step failed: " + cmd[0])
+
+
+def build_lib():
+    srcs = [os.path.join(CSRC, f) for f in ("runtime.hip", "serve.hip")]
+    objs = []
+    for s in srcs:
+        o = os.path.join(CSRC, os.path.basename(s) + ".o")
+        run(["hipcc", "-c"] + HIP_FLAGS + [s, "-o", o])
+        objs.append(o)
+    wo = os.path.join(CSRC, "workload.o")
+    run(["g++", "-c", "-O2", "-std=c++17", "-fPIC", "-Wall", os.path.join(CSRC, "workload.cpp"), "-o", wo])
+    objs.append(wo)
+    tmp = LIB + ".tmp"
+    run(["hipcc", "-shared", "--offload-arch=" + ARCH, "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+
+
+def build_cli():
+    src = os.path.join(CSRC, "cli.cpp")
+    if not os.path.exists(src):
+        return
+    run(["g++", "-O2", "-std=c++17", "-Wall", src, "-o", CLI, "-L" + HERE, "-ltweakalign",
+         "-Wl,-rpath,$ORIGIN"])
+
+
+def build():
+    build_lib()
+    build_cli()
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

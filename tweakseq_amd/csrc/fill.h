@@ -1,0 +1,418 @@
+// Gotoh DP fill for short and medium pairs on gfx950.
+//
+// Layout of the work (no counterpart in the reference, which spawns an
+// external aligner; recurrence, boundaries and tie-breaks follow the
+// specification the CPU oracle implements, oracle/gotoh.c):
+//  * a wave64 = 4 DP groups of 16 lanes (one DPP row each); a group owns a
+//    task and sweeps it strip by strip;
+//  * a strip is 16 lanes x R query rows held in registers; lane l works on
+//    column j = t - l at step t, so a group covers an anti-diagonal band and
+//    H/F of a lane's bottom row reach the next lane one step later through
+//    DPP row_shr:1 (lane 0 takes the strip's top boundary instead);
+//  * the bottom row of a strip goes to a ring in global memory and is read
+//    back as the top boundary of the next strip;
+//  * P == 2 packs two pairs ("a couple") into the int16 halves of every
+//    register (saturating v_pk_* arithmetic); P == 1 is one pair in int32;
+//  * the substitution scores of a lane's R rows for every column residue
+//    are staged in a private LDS slot at the start of each strip, so a step
+//    costs one LDS read per half;
+//  * direction bits (4 per cell) leave as sign bits of int differences,
+//    packed per step and stored per 8-step block with dwordx4 stores.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace ta {
+
+typedef short s2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t u32(s2v v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ s2v s2(uint32_t v) { return __builtin_bit_cast(s2v, v); }
+
+template <int P>
+struct Lane;
+
+template <>
+struct Lane<2> {
+  using V = s2v;
+  static __device__ __forceinline__ V add(V a, V b) { return __builtin_elementwise_add_sat(a, b); }
+  static __device__ __forceinline__ V sub(V a, V b) { return __builtin_elementwise_sub_sat(a, b); }
+  static __device__ __forceinline__ V vmax(V a, V b) { return __builtin_elementwise_max(a, b); }
+  static __device__ __forceinline__ V splat(int x) { return V{(short)x, (short)x}; }
+  static __device__ __forceinline__ int bits(V v) { return (int)u32(v); }
+  static __device__ __forceinline__ V unbits(int x) { return s2((uint32_t)x); }
+  static constexpr int NEG = NEG16;
+};
+
+template <>
+struct Lane<1> {
+  using V = int;
+  static __device__ __forceinline__ V add(V a, V b) { return a + b; }
+  static __device__ __forceinline__ V sub(V a, V b) { return a - b; }
+  static __device__ __forceinline__ V vmax(V a, V b) { return a > b ? a : b; }
+  static __device__ __forceinline__ V splat(int x) { return x; }
+  static __device__ __forceinline__ int bits(V v) { return v; }
+  static __device__ __forceinline__ V unbits(int x) { return x; }
+  static constexpr int NEG = NEG32;
+};
+
+struct FillArgs {
+  const Task *tasks;
+  int32_t ntasks;
+  int32_t *ctr;          // work-queue head
+  const int8_t *table;   // Kp x Kp, row/column Kp-1 = pad
+  int32_t Kp;
+  const uint16_t *rowcodes;
+  const uint16_t *colcodes;
+  uint32_t *ring;
+  uint8_t *tb;
+  FillOut *out;
+  int32_t open, ext;
+  int32_t sat_limit;     // local int16: best score above this -> int32 rerun
+};
+
+// DPP row_shr:1 inside each 16-lane row; lane 0 of the row takes `old`.
+__device__ __forceinline__ int shr1(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, 0x111, 0xf, 0xf, false);
+}
+
+// LDS bytes the fill kernel needs for a table of Kp codes.
+__host__ __device__ constexpr int fill_slot_stride(int P, int Kp) { return P * Kp * R + 8; }
+__host__ __device__ constexpr int fill_lds_bytes(int P, int Kp) {
+  return 1024 + 64 * fill_slot_stride(P, Kp);
+}
+
+// Signed byte k of the 8-byte profile row (rows 0..3 in x, 4..7 in y).
+__device__ __forceinline__ int prof1(uint2 p, int k) {
+  uint32_t w = k < 4 ? p.x : p.y;
+  return (int)(int8_t)(w >> (8 * (k & 3)));
+}
+
+// Packed (sext A[k], sext B[k]) from two 8-byte profile rows.  v_perm can
+// sign-extend only odd bytes, so even rows are moved up one byte first.
+__device__ __forceinline__ uint32_t prof2(uint32_t a, uint32_t b, uint32_t as, uint32_t bs, int k) {
+  switch (k & 3) {
+    case 0: return __builtin_amdgcn_perm(bs, as, 0x0A050801u);
+    case 1: return __builtin_amdgcn_perm(b, a, 0x0A050801u);
+    case 2: return __builtin_amdgcn_perm(bs, as, 0x0B070903u);
+    default: return __builtin_amdgcn_perm(b, a, 0x0B070903u);
+  }
+}
+
+template <int P, bool LOCAL, bool TB>
+__global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
+  using L = Lane<P>;
+  using V = typename L::V;
+  extern __shared__ __align__(16) uint8_t lds[];
+
+  const int lane = threadIdx.x;
+  const int l = lane & (G - 1);
+  const int Kp = A.Kp;
+  const int SS = fill_slot_stride(P, Kp);
+
+  int8_t *tab = (int8_t *)lds;
+  for (int x = lane; x < Kp * Kp; x += 64) tab[x] = A.table[x];
+  __syncthreads();
+  int8_t *slot = (int8_t *)lds + 1024 + lane * SS;
+
+  const V vopen = L::splat(A.open);
+  const V vext = L::splat(A.ext);
+  const V vzero = L::splat(0);
+  const V vneg = L::splat(L::NEG);
+
+  // Group state (identical in the 16 lanes of a group).
+  bool alive = true;
+  int task = -1, s = 0, b = 0;
+  int n = 0, S = 0, NB = 0, single = 0;
+  int m0 = 0, m1 = 0, off0 = 0, off1 = 0, pid0 = 0, pid1 = 0;
+  uint64_t row_off = 0, col_off = 0, ring_off = 0, tb_off = 0;
+  int ring_len = 0, flags_or = 0;
+
+  // DP registers.
+  V H[R], E[R], HoL[R];
+  V hd0 = vzero, Hbot = vzero, Fbot = vzero;
+  int ccur = 0;
+  uint32_t key0[R], key1[R];  // local end-cell keys (P == 2: lo / hi half)
+  int rmax[R], rcol[R];       // local end-cell tracking (P == 1)
+  int bestH0 = 0, besti0 = 0, bestj0 = 0, bestH1 = 0, besti1 = 0, bestj1 = 0;
+  uint4 ccodes = {0, 0, 0, 0}, ccodes_n = {0, 0, 0, 0};
+  uint4 rH0 = {0, 0, 0, 0}, rH1 = {0, 0, 0, 0}, rF0 = {0, 0, 0, 0}, rF1 = {0, 0, 0, 0};
+  uint4 nH0 = rH0, nH1 = rH0, nF0 = rH0, nF1 = rH0;
+  int carryH = 0, carryF = 0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    H[k] = vzero; E[k] = vneg; HoL[k] = vzero;
+    key0[k] = 0; key1[k] = 0; rmax[k] = 0; rcol[k] = 0;
+  }
+
+  auto fetch = [&]() {
+    int t = 0;
+    if (l == 0) t = atomicAdd(A.ctr, 1);
+    t = __shfl(t, lane & ~(G - 1), 64);
+    if (t >= A.ntasks) {
+      alive = false;
+      return;
+    }
+    task = t;
+    const Task &T = A.tasks[t];
+    n = T.n; S = T.S; NB = T.NB; single = T.single;
+    m0 = T.m[0]; m1 = T.m[1]; off0 = T.off[0]; off1 = T.off[1];
+    pid0 = T.pair[0]; pid1 = T.pair[1];
+    row_off = T.row_off; col_off = T.col_off; ring_off = T.ring_off; tb_off = T.tb_off;
+    ring_len = T.ring_len;
+    flags_or = T.flags_or;
+    s = 0; b = 0;
+    bestH0 = 0; besti0 = 0; bestj0 = 0; bestH1 = 0; besti1 = 0; bestj1 = 0;
+  };
+
+  // H(i, 0) for 1-based row i.
+  auto left = [&](int i) -> int { return (LOCAL || i == 0) ? 0 : A.open + (i - 1) * A.ext; };
+
+  fetch();
+
+  while (__any(alive)) {
+    if (!alive) continue;
+
+    if (b == 0) {
+      // ---- strip start: profile slot, left boundary, first block inputs.
+      const uint4 rc = *(const uint4 *)(A.rowcodes + row_off + (uint64_t)s * STRIP + l * R);
+      const uint32_t rcw[4] = {rc.x, rc.y, rc.z, rc.w};
+      for (int h = 0; h < P; ++h) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const int code = (rcw[k >> 1] >> (16 * (k & 1) + 8 * h)) & 0xff;
+          const int8_t *trow = tab + code * Kp;
+          int8_t *dst = slot + h * Kp * R + k;
+          for (int c = 0; c < Kp; ++c) dst[c * R] = trow[c];
+        }
+      }
+      const int row0 = s * STRIP + l * R;  // 0-based first row of this lane
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        H[k] = L::splat(left(row0 + k + 1));
+        E[k] = vneg;
+        HoL[k] = L::add(H[k], vopen);
+        key0[k] = 0; key1[k] = 0; rmax[k] = 0; rcol[k] = 0;
+      }
+      hd0 = L::splat(left(row0));
+      ccur = (Kp - 1) | ((Kp - 1) << 8);
+      ccodes_n = *(const uint4 *)(A.colcodes + col_off);
+      if (s > 0) {
+        const uint32_t *rh = A.ring + ring_off + RING_LEAD;
+        const uint32_t *rf = rh + ring_len;
+        nH0 = *(const uint4 *)rh; nH1 = *(const uint4 *)(rh + 4);
+        nF0 = *(const uint4 *)rf; nF1 = *(const uint4 *)(rf + 4);
+      }
+    }
+
+    // ---- one block of 8 steps.
+    ccodes = ccodes_n;
+    rH0 = nH0; rH1 = nH1; rF0 = nF0; rF1 = nF1;
+    if (b + 1 < NB) {  // prefetch the next block's lane-0 inputs
+      ccodes_n = *(const uint4 *)(A.colcodes + col_off + 8 * (b + 1));
+      if (s > 0) {
+        const uint32_t *rh = A.ring + ring_off + RING_LEAD + 8 * (b + 1);
+        const uint32_t *rf = rh + ring_len;
+        nH0 = *(const uint4 *)rh; nH1 = *(const uint4 *)(rh + 4);
+        nF0 = *(const uint4 *)rf; nF1 = *(const uint4 *)(rf + 4);
+      }
+    }
+    const uint32_t ccw[4] = {ccodes.x, ccodes.y, ccodes.z, ccodes.w};
+    const uint32_t rhw[8] = {rH0.x, rH0.y, rH0.z, rH0.w, rH1.x, rH1.y, rH1.z, rH1.w};
+    const uint32_t rfw[8] = {rF0.x, rF0.y, rF0.z, rF0.w, rF1.x, rF1.y, rF1.z, rF1.w};
+    uint32_t tbw[16];
+    int rbH[8], rbF[8];
+
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int t = 8 * b + q;
+      const int j = t - l;
+      const int code0 = (ccw[q >> 1] >> (16 * (q & 1))) & 0xffff;
+      ccur = shr1(code0, ccur);
+      int topH, topF;
+      if (s == 0) {
+        topH = L::bits(L::splat(LOCAL ? 0 : A.open + t * A.ext));
+        topF = L::bits(vneg);
+      } else {
+        topH = (int)rhw[q];
+        topF = (int)rfw[q];
+      }
+      const V Hup = L::unbits(shr1(topH, L::bits(Hbot)));
+      const V Fup = L::unbits(shr1(topF, L::bits(Fbot)));
+
+      if (j >= 0 && j < n) {
+        const int c0 = ccur & 0xff;
+        const uint2 pa = *(const uint2 *)(slot + c0 * R);
+        uint2 pb = pa;
+        if constexpr (P == 2) pb = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
+        const uint32_t pas_x = pa.x << 8, pas_y = pa.y << 8, pbs_x = pb.x << 8, pbs_y = pb.y << 8;
+        const uint32_t cj = 0xffffu - (uint32_t)j;
+
+        V hd = hd0;
+        V HoU = L::add(Hup, vopen);
+        V F = Fup;
+        uint32_t xc = 0, yc = 0, acc = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          V Sk;
+          if constexpr (P == 2) {
+            const bool hiw = k >= 4;
+            Sk = s2(prof2(hiw ? pa.y : pa.x, hiw ? pb.y : pb.x, hiw ? pas_y : pas_x,
+                          hiw ? pbs_y : pbs_x, k));
+          } else {
+            Sk = L::unbits(prof1(pa, k));
+          }
+          const V HdS = L::add(hd, Sk);
+          const V Eext = L::add(E[k], vext);
+          const V En = L::vmax(HoL[k], Eext);
+          const V Fext = L::add(F, vext);
+          const V Fn = L::vmax(HoU, Fext);
+          const V EF = L::vmax(En, Fn);
+          V Hn = L::vmax(HdS, EF);
+          if constexpr (LOCAL) Hn = L::vmax(Hn, vzero);
+          const V Hon = L::add(Hn, vopen);
+          if constexpr (TB) {
+            const uint32_t dE = (uint32_t)L::bits(L::sub(HoL[k], Eext));
+            const uint32_t dF = (uint32_t)L::bits(L::sub(HoU, Fext));
+            const uint32_t dD = (uint32_t)L::bits(L::sub(HdS, EF));
+            const uint32_t dX = (uint32_t)L::bits(L::sub(En, Fn));
+            if constexpr (P == 2) {
+              const uint32_t X = __builtin_amdgcn_perm(dF, dE, 0x07050301u);
+              const uint32_t Y = __builtin_amdgcn_perm(dX, dD, 0x07050301u);
+              const uint32_t mk = 0x01010101u << k;
+              xc = ((X >> (7 - k)) & mk) | (xc & ~mk);
+              yc = ((Y >> (7 - k)) & mk) | (yc & ~mk);
+            } else {
+              acc = __builtin_amdgcn_alignbit(acc, dE, 31);
+              acc = __builtin_amdgcn_alignbit(acc, dF, 31);
+              acc = __builtin_amdgcn_alignbit(acc, dD, 31);
+              acc = __builtin_amdgcn_alignbit(acc, dX, 31);
+            }
+          }
+          if constexpr (LOCAL) {
+            if constexpr (P == 2) {
+              const uint32_t hu = u32(Hn);
+              key0[k] = max(key0[k], (hu << 16) | cj);
+              key1[k] = max(key1[k], (hu & 0xffff0000u) | cj);
+            } else {
+              const int hv = L::bits(Hn);
+              if (hv > rmax[k]) { rmax[k] = hv; rcol[k] = j; }
+            }
+          }
+          hd = H[k];
+          H[k] = Hn;
+          E[k] = En;
+          HoL[k] = Hon;
+          F = Fn;
+          HoU = Hon;
+        }
+        Hbot = H[R - 1];
+        Fbot = F;
+        hd0 = Hup;
+        if constexpr (TB) {
+          if constexpr (P == 2) { tbw[2 * q] = xc; tbw[2 * q + 1] = yc; }
+          else tbw[q] = acc;
+        }
+      }
+      rbH[q] = L::bits(Hbot);
+      rbF[q] = L::bits(Fbot);
+    }
+
+    // ---- block outputs: direction planes (all lanes), ring (lane 15).
+    if constexpr (TB) {
+      if constexpr (P == 2) {
+        uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * NB + b) * G + l) * 64));
+        dst[0] = make_uint4(tbw[0], tbw[1], tbw[2], tbw[3]);
+        dst[1] = make_uint4(tbw[4], tbw[5], tbw[6], tbw[7]);
+        dst[2] = make_uint4(tbw[8], tbw[9], tbw[10], tbw[11]);
+        dst[3] = make_uint4(tbw[12], tbw[13], tbw[14], tbw[15]);
+      } else {
+        uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * NB + b) * G + l) * 32));
+        dst[0] = make_uint4(tbw[0], tbw[1], tbw[2], tbw[3]);
+        dst[1] = make_uint4(tbw[4], tbw[5], tbw[6], tbw[7]);
+      }
+    }
+    if (l == G - 1 && s + 1 < S) {
+      // Lane 15 handles column t - 15 at step t; ring index = column + 16,
+      // so block b carries indices 8b .. 8b+7: last block's q = 7 then q 0..6.
+      uint32_t *rh = A.ring + ring_off + 8 * b;
+      uint32_t *rf = rh + ring_len;
+      *(uint4 *)rh = make_uint4(carryH, rbH[0], rbH[1], rbH[2]);
+      *(uint4 *)(rh + 4) = make_uint4(rbH[3], rbH[4], rbH[5], rbH[6]);
+      *(uint4 *)rf = make_uint4(carryF, rbF[0], rbF[1], rbF[2]);
+      *(uint4 *)(rf + 4) = make_uint4(rbF[3], rbF[4], rbF[5], rbF[6]);
+    }
+    carryH = rbH[7];
+    carryF = rbF[7];
+
+    ++b;
+    if (b < NB) continue;
+
+    // ---- strip end.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int row0 = s * STRIP + l * R;
+    if constexpr (LOCAL) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int i = row0 + k;  // 0-based row
+        if constexpr (P == 2) {
+          const int h0 = (int)(key0[k] >> 16), h1 = (int)(key1[k] >> 16);
+          if (i < m0 && h0 > bestH0) { bestH0 = h0; besti0 = i; bestj0 = 0xffff - (int)(key0[k] & 0xffff); }
+          if (i < m1 && h1 > bestH1) { bestH1 = h1; besti1 = i; bestj1 = 0xffff - (int)(key1[k] & 0xffff); }
+        } else {
+          if (i < m0 && rmax[k] > bestH0) { bestH0 = rmax[k]; besti0 = i; bestj0 = rcol[k]; }
+        }
+      }
+    } else {
+      // Global: H(m, n) sits in the lane owning row m at the last column.
+#pragma unroll
+      for (int h = 0; h < P; ++h) {
+        const int mh = h ? m1 : m0;
+        const int r = mh - 1 - row0;
+        if (mh > 0 && r >= 0 && r < R && !(h == 1 && single)) {
+          int v = 0;
+#pragma unroll
+          for (int k = 0; k < R; ++k)
+            if (k == r) v = L::bits(H[k]);
+          int score = P == 2 ? (int)(int16_t)(h ? ((uint32_t)v >> 16) : ((uint32_t)v & 0xffff)) : v;
+          FillOut o;
+          o.score = score; o.end_i = mh; o.end_j = n; o.flags = (P == 1 ? 1 : 0) | flags_or;
+          A.out[h ? pid1 : pid0] = o;
+        }
+      }
+    }
+    ++s;
+    b = 0;
+    carryH = 0;
+    carryF = 0;
+    if (s < S) continue;
+
+    // ---- task end (local): reduce the group's best cells per half.
+    if constexpr (LOCAL) {
+#pragma unroll
+      for (int h = 0; h < P; ++h) {
+        int bh = h ? bestH1 : bestH0, bi = h ? besti1 : besti0, bj = h ? bestj1 : bestj0;
+#pragma unroll
+        for (int w = 8; w >= 1; w >>= 1) {
+          const int oh = __shfl_xor(bh, w, 64), oi = __shfl_xor(bi, w, 64), oj = __shfl_xor(bj, w, 64);
+          const bool take = oh > bh || (oh == bh && (oi < bi || (oi == bi && oj < bj)));
+          if (take) { bh = oh; bi = oi; bj = oj; }
+        }
+        if (l == 0 && !(h == 1 && single)) {
+          const int off = h ? off1 : off0;
+          FillOut o;
+          o.score = bh;
+          o.end_i = bh > 0 ? bi + 1 : 0;
+          o.end_j = bh > 0 ? bj - off + 1 : 0;
+          o.flags = (P == 1 ? 1 : 0) | (P == 2 && bh > A.sat_limit ? 2 : 0) | (bh == 0 ? 8 : 0) | flags_or;
+          A.out[h ? pid1 : pid0] = o;
+        }
+      }
+    }
+    fetch();
+  }
+}
+
+}  // namespace ta

@@ -1,0 +1,745 @@
+// Host runtime + C ABI of the alignment engine (include/tweakalign.h).
+//
+// A batch is encoded once on the host, split into DP tasks (packed int16
+// couples where the score range allows, int32 singles otherwise), sorted
+// longest-first and uploaded; a run is: persistent fill kernels pulling tasks
+// off a device counter, an optional int32 rerun of pairs whose int16 pass
+// saturated, and the traceback kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/tweakalign.h"
+#include "common.h"
+#include "fill.h"
+#include "runtime_internal.h"
+#include "traceback.h"
+
+using namespace ta;
+
+// ---------------------------------------------------------------------------
+// Alphabets.  The protein table is BLOSUM62 in the row order
+// A R N D C Q E G H I L K M F P S T W Y V B Z X and the letter map of
+// tweakseq/Core/Annotations/Consensus.cpp:34-69 (same numbers, by necessity).
+namespace {
+
+const char kProtOrder[] = "ARNDCQEGHILKMFPSTWYVBZX";
+// Lower triangle, row by row (row r has r + 1 entries).
+const int8_t kBlosumLower[] = {
+    4,
+    -1, 5,
+    -2, 0, 6,
+    -2, -2, 1, 6,
+    0, -3, -3, -3, 9,
+    -1, 1, 0, 0, -3, 5,
+    -1, 0, 0, 2, -4, 2, 5,
+    0, -2, 0, -1, -3, -2, -2, 6,
+    -2, 0, 1, -1, -3, 0, 0, -2, 8,
+    -1, -3, -3, -3, -1, -3, -3, -4, -3, 4,
+    -1, -2, -3, -4, -1, -2, -3, -4, -3, 2, 4,
+    -1, 2, 0, -1, -3, 1, 1, -2, -1, -3, -2, 5,
+    -1, -1, -2, -3, -1, 0, -2, -3, -2, 1, 2, -1, 5,
+    -2, -3, -3, -3, -2, -3, -3, -3, -1, 0, 0, -3, 0, 6,
+    -1, -2, -2, -1, -3, -1, -1, -2, -2, -3, -3, -1, -2, -4, 7,
+    1, -1, 1, 0, -1, 0, 0, 0, -1, -2, -2, 0, -1, -2, -1, 4,
+    0, -1, 0, -1, -1, -1, -1, -2, -2, -1, -1, -1, -1, -2, -1, 1, 5,
+    -3, -3, -4, -4, -2, -2, -3, -2, -2, -3, -2, -3, -1, 1, -4, -3, -2, 11,
+    -2, -2, -2, -3, -2, -1, -2, -3, 2, -1, -1, -2, -1, 3, -3, -2, -2, 2, 7,
+    0, -3, -3, -3, -1, -2, -2, -3, -3, 3, 1, -2, 1, -1, -2, -2, 0, -3, -1, 4,
+    -2, -1, 3, 4, -3, 0, 1, -1, 0, -3, -4, 0, -3, -3, -2, 0, -1, -4, -3, -3, 4,
+    -1, 0, 0, 1, -3, 3, 4, -2, 0, -3, -3, 1, -1, -3, -1, 0, -1, -3, -2, -2, 1, 4,
+    0, -1, -1, -1, -2, -1, -1, -1, -1, -1, -1, -1, -1, -1, -2, 0, 0, -2, -1, -1, -1, -1, -1,
+};
+constexpr int kProtK = 23;
+
+struct Alphabet {
+  int K = 0;
+  int8_t S[32 * 32];  // K x K
+  int8_t map[256];    // byte -> code, -1 outside the alphabet
+  int maxS = 0;
+};
+
+Alphabet make_alphabet(int alphabet, int match, int mismatch) {
+  Alphabet A;
+  memset(A.map, -1, sizeof(A.map));
+  if (alphabet == TA_PROTEIN) {
+    A.K = kProtK;
+    int idx = 0;
+    for (int r = 0; r < kProtK; ++r)
+      for (int c = 0; c <= r; ++c, ++idx) {
+        A.S[r * kProtK + c] = kBlosumLower[idx];
+        A.S[c * kProtK + r] = kBlosumLower[idx];
+      }
+    for (int r = 0; r < kProtK; ++r) {
+      A.map[(unsigned char)kProtOrder[r]] = (int8_t)r;
+      A.map[(unsigned char)(kProtOrder[r] + 32)] = (int8_t)r;
+    }
+    for (char c : {'J', 'O', 'U'}) {  // no row of their own: scored as X
+      A.map[(unsigned char)c] = 22;
+      A.map[(unsigned char)(c + 32)] = 22;
+    }
+  } else {
+    A.K = 4;
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) A.S[r * 4 + c] = (int8_t)(r == c ? match : mismatch);
+    const char *acgt = "ACGT";
+    for (int r = 0; r < 4; ++r) {
+      A.map[(unsigned char)acgt[r]] = (int8_t)r;
+      A.map[(unsigned char)(acgt[r] + 32)] = (int8_t)r;
+    }
+  }
+  A.maxS = -128;
+  for (int x = 0; x < A.K * A.K; ++x) A.maxS = std::max<int>(A.maxS, A.S[x]);
+  return A;
+}
+
+// (K+1) x (K+1) table with a pad row / column (index K) of PAD_SCORE.
+std::vector<int8_t> padded_table(const Alphabet &A) {
+  const int Kp = A.K + 1;
+  std::vector<int8_t> t((size_t)Kp * Kp, PAD_SCORE);
+  for (int r = 0; r < A.K; ++r)
+    for (int c = 0; c < A.K; ++c) t[(size_t)r * Kp + c] = A.S[r * A.K + c];
+  return t;
+}
+
+#define HIPCHK(x)                                                   \
+  do {                                                              \
+    hipError_t e_ = (x);                                            \
+    if (e_ != hipSuccess) {                                         \
+      fprintf(stderr, "tweakalign: %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      return TA_E_DEVICE;                                           \
+    }                                                               \
+  } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return TA_OK;
+  if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return TA_E_NOMEM;
+  }
+  return TA_OK;
+}
+
+template <typename T>
+int upload(T **dst, const std::vector<T> &src, hipStream_t st) {
+  int rc = dalloc(dst, src.size());
+  if (rc) return rc;
+  if (!src.empty()) HIPCHK(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  return TA_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct ta_ctx {
+  int device = 0;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+  Serve *serve = nullptr;
+};
+
+const char *ta_strerror(int code) {
+  switch (code) {
+    case TA_OK: return "ok";
+    case TA_E_INVALID: return "invalid argument";
+    case TA_E_ALPHABET: return "residue outside the alphabet";
+    case TA_E_SPACE: return "output buffer too small";
+    case TA_E_DEVICE: return "HIP runtime error";
+    case TA_E_NOMEM: return "out of memory";
+    case TA_E_BUDGET: return "request exceeds the traceback-memory budget";
+    case TA_E_CANCELLED: return "request cancelled";
+    case TA_E_TICKET: return "unknown ticket";
+    case TA_E_NODEVICE: return "no usable GPU";
+    case TA_E_PENDING: return "pending";
+    default: return "unknown error";
+  }
+}
+
+int ta_matrix(int32_t alphabet, int32_t match, int32_t mismatch, int8_t *out, int32_t *K) {
+  if (alphabet != TA_PROTEIN && alphabet != TA_DNA) return TA_E_INVALID;
+  Alphabet A = make_alphabet(alphabet, match, mismatch);
+  if (K) *K = A.K;
+  if (out) memcpy(out, A.S, (size_t)A.K * A.K);
+  return TA_OK;
+}
+
+int ta_encode(int32_t alphabet, const char *s, int32_t len, uint8_t *out) {
+  if (alphabet != TA_PROTEIN && alphabet != TA_DNA) return TA_E_INVALID;
+  if (len < 0 || (len > 0 && (!s || !out))) return TA_E_INVALID;
+  Alphabet A = make_alphabet(alphabet, 1, -1);
+  for (int32_t i = 0; i < len; ++i) {
+    int c = A.map[(unsigned char)s[i]];
+    if (c < 0) return TA_E_ALPHABET;
+    out[i] = (uint8_t)c;
+  }
+  return TA_OK;
+}
+
+int ta_create(ta_ctx **out, int device) {
+  if (!out) return TA_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TA_E_NODEVICE;
+  if (device < 0 || device >= ndev) return TA_E_INVALID;
+  HIPCHK(hipSetDevice(device));
+  ta_ctx *c = new ta_ctx();
+  c->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    c->cus = cus;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return TA_E_DEVICE;
+  }
+  *out = c;
+  return TA_OK;
+}
+
+void ta_destroy(ta_ctx *c) {
+  if (!c) return;
+  serve_shutdown(c);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// ---------------------------------------------------------------------------
+// Batch construction.
+namespace {
+
+struct HostPair {
+  uint64_t a_off, b_off;
+  int32_t la, lb;
+  int path;  // 2 packed int16, 1 int32, 0 no DP (empty side)
+};
+
+struct Plan {
+  std::vector<Task> tasks2, tasks1;
+  std::vector<uint16_t> rowcodes, colcodes;
+  std::vector<PairTB> ptb;
+  uint64_t ring_elems = 0, tb_bytes = 0;
+};
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Append one task (pair lo / hi; hi == lo for a single) to the plan.
+void add_task(Plan &pl, std::vector<Task> &dst, int P, bool trace, const std::vector<HostPair> &hp,
+              const std::vector<uint8_t> &codes, int lo, int hi, bool single, int Kp) {
+  const HostPair &A = hp[lo], &B = hp[hi];
+  Task T{};
+  T.pair[0] = lo; T.pair[1] = hi;
+  T.m[0] = A.la; T.m[1] = B.la;
+  T.n = std::max(A.lb, B.lb);
+  T.off[0] = T.n - A.lb; T.off[1] = T.n - B.lb;
+  T.S = ceil_div(std::max(A.la, B.la), STRIP);
+  T.NB = ceil_div(T.n + 16, 8);
+  T.single = single ? 1 : 0;
+  const uint16_t pad = (uint16_t)(Kp - 1);
+  // Row codes: S * STRIP entries, pad rows past each pair's m.
+  T.row_off = pl.rowcodes.size();
+  for (int r = 0; r < T.S * STRIP; ++r) {
+    uint16_t ca = r < A.la ? codes[A.a_off + r] : pad;
+    uint16_t cb = r < B.la ? codes[B.a_off + r] : pad;
+    pl.rowcodes.push_back((uint16_t)(ca | (cb << 8)));
+  }
+  // Column codes: leading pad columns for the shorter pair, pad slack after n.
+  T.col_off = pl.colcodes.size();
+  const int ncol = 8 * T.NB + 8;
+  for (int j = 0; j < ncol; ++j) {
+    int ja = j - T.off[0], jb = j - T.off[1];
+    uint16_t ca = (ja >= 0 && ja < A.lb) ? codes[A.b_off + ja] : pad;
+    uint16_t cb = (jb >= 0 && jb < B.lb) ? codes[B.b_off + jb] : pad;
+    pl.colcodes.push_back((uint16_t)(ca | (cb << 8)));
+  }
+  while (pl.colcodes.size() % 8) pl.colcodes.push_back(pad | (pad << 8));
+  T.ring_len = T.S > 1 ? 8 * T.NB + 32 : 0;
+  T.ring_off = pl.ring_elems;
+  pl.ring_elems += 2ull * T.ring_len;
+  T.tb_off = trace ? pl.tb_bytes : 0;
+  if (trace) pl.tb_bytes += (uint64_t)T.S * tb_strip_bytes(P, T.NB);
+  for (int h = 0; h < (single ? 1 : 2); ++h) {
+    PairTB &d = pl.ptb[h ? hi : lo];
+    d.tb_off = T.tb_off; d.P = P; d.half = h; d.NB = T.NB; d.off = T.off[h];
+  }
+  dst.push_back(T);
+}
+
+double task_cost(const Task &T) { return (double)T.S * T.NB; }
+
+}  // namespace
+
+struct ta_batch {
+  ta_params p{};
+  int32_t n = 0;
+  Alphabet alph;
+  int Kp = 0;
+  bool trace = false;       // direction planes are stored (local, or CIGAR asked)
+  bool may_saturate = false;
+  int64_t cells = 0;
+  std::vector<HostPair> hp;
+  std::vector<uint8_t> codes;
+  std::vector<FillOut> fo_init;
+  FillOut *d_fo_init = nullptr;
+  std::vector<PairDev> pairs_h;
+  uint64_t cig_bytes = 0;
+  Plan plan;
+  // device
+  uint8_t *d_codes = nullptr;
+  int8_t *d_table = nullptr;
+  Task *d_tasks2 = nullptr, *d_tasks1 = nullptr;
+  uint16_t *d_rowcodes = nullptr, *d_colcodes = nullptr;
+  uint32_t *d_ring = nullptr;
+  uint8_t *d_tb = nullptr;
+  PairTB *d_ptb = nullptr;
+  PairDev *d_pairs = nullptr;
+  FillOut *d_fo = nullptr;
+  ResultDev *d_res = nullptr;
+  char *d_cig = nullptr;
+  int32_t *d_ctr = nullptr;
+  // int32 rerun of saturated pairs (allocated on demand)
+  Task *d_tasks_r = nullptr;
+  uint16_t *d_rowcodes_r = nullptr, *d_colcodes_r = nullptr;
+  uint32_t *d_ring_r = nullptr;
+  uint8_t *d_tb_r = nullptr;
+  int32_t n_rerun = 0;
+  // timing
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double ms[4] = {0, 0, 0, 0};
+  // plain (one launch per pair) resources, built on first use
+  bool plain_ready = false;
+  std::vector<Task> plain_tasks;
+  Task *d_plain_tasks = nullptr;
+  uint16_t *d_prow = nullptr, *d_pcol = nullptr;
+  uint32_t *d_pring = nullptr;
+  uint8_t *d_ptbuf = nullptr;
+  PairTB *d_pptb = nullptr;
+};
+
+namespace {
+
+bool global_i16_safe(const ta_params &p, int maxS, int la, int lb) {
+  const long lo = 3L * -p.gap_open + (long)(la + lb + 2) * -p.gap_extend + 128;
+  const long hi = (long)std::min(la, lb) * maxS + maxS;
+  return lo < 16000 && hi < 32000;
+}
+
+void free_rerun(ta_batch *B) {
+  dfree(B->d_tasks_r); dfree(B->d_rowcodes_r); dfree(B->d_colcodes_r);
+  dfree(B->d_ring_r); dfree(B->d_tb_r);
+  B->n_rerun = 0;
+}
+
+}  // namespace
+
+void ta_batch_destroy(ta_batch *B) {
+  if (!B) return;
+  dfree(B->d_codes); dfree(B->d_table); dfree(B->d_tasks2); dfree(B->d_tasks1);
+  dfree(B->d_rowcodes); dfree(B->d_colcodes); dfree(B->d_ring); dfree(B->d_tb);
+  dfree(B->d_ptb); dfree(B->d_pairs); dfree(B->d_fo); dfree(B->d_res); dfree(B->d_cig);
+  dfree(B->d_ctr); dfree(B->d_fo_init);
+  free_rerun(B);
+  dfree(B->d_plain_tasks); dfree(B->d_prow); dfree(B->d_pcol); dfree(B->d_pring);
+  dfree(B->d_ptbuf); dfree(B->d_pptb);
+  for (auto &e : B->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete B;
+}
+
+int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, ta_batch **out) {
+  if (!ctx || !p || !out || n < 0 || (n > 0 && !pairs)) return TA_E_INVALID;
+  *out = nullptr;
+  if (p->mode != TA_GLOBAL && p->mode != TA_LOCAL) return TA_E_INVALID;
+  if (p->alphabet != TA_PROTEIN && p->alphabet != TA_DNA) return TA_E_INVALID;
+  if (p->gap_open > 0 || p->gap_extend > 0 || p->gap_open < -1000 || p->gap_extend < -1000)
+    return TA_E_INVALID;
+  if (p->alphabet == TA_DNA && (p->match < -100 || p->match > 100 || p->mismatch < -100 || p->mismatch > 100))
+    return TA_E_INVALID;
+  if (p->lanes != TA_LANES_AUTO && p->lanes != TA_LANES_I16 && p->lanes != TA_LANES_I32) return TA_E_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+
+  ta_batch *B = new ta_batch();
+  B->p = *p;
+  B->n = n;
+  B->alph = make_alphabet(p->alphabet, p->match, p->mismatch);
+  B->Kp = B->alph.K + 1;
+  const bool local = p->mode == TA_LOCAL;
+  B->trace = local || p->traceback;
+
+  // Encode (sequences shared between pairs are encoded once).
+  std::unordered_map<const char *, std::pair<uint64_t, int32_t>> seen;
+  auto enc = [&](const char *s, int32_t len, uint64_t *off) -> int {
+    auto it = seen.find(s);
+    if (it != seen.end() && it->second.second == len) { *off = it->second.first; return TA_OK; }
+    *off = B->codes.size();
+    for (int32_t i = 0; i < len; ++i) {
+      int c = B->alph.map[(unsigned char)s[i]];
+      if (c < 0) return TA_E_ALPHABET;
+      B->codes.push_back((uint8_t)c);
+    }
+    seen[s] = {*off, len};
+    return TA_OK;
+  };
+  B->hp.resize(n);
+  B->fo_init.assign(n, FillOut{0, 0, 0, 0});
+  B->pairs_h.resize(n);
+  for (int32_t k = 0; k < n; ++k) {
+    const ta_pair &q = pairs[k];
+    if (q.la < 0 || q.lb < 0 || q.la > 1000000 || q.lb > 1000000 || (q.la && !q.a) || (q.lb && !q.b)) {
+      delete B;
+      return TA_E_INVALID;
+    }
+    HostPair &h = B->hp[k];
+    int rc = enc(q.a, q.la, &h.a_off);
+    if (!rc) rc = enc(q.b, q.lb, &h.b_off);
+    if (rc) { delete B; return rc; }
+    h.la = q.la; h.lb = q.lb;
+    B->cells += (int64_t)q.la * q.lb;
+    if (q.la == 0 || q.lb == 0) {
+      h.path = 0;
+      FillOut &f = B->fo_init[k];
+      if (local) { f.flags = 8; }
+      else {
+        const int len = q.la + q.lb;
+        f.score = len ? p->gap_open + (len - 1) * p->gap_extend : 0;
+        f.end_i = q.la; f.end_j = q.lb;
+      }
+    } else if (p->lanes == TA_LANES_I32) {
+      h.path = 1;
+    } else if (local) {
+      h.path = 2;
+      if ((long)std::min(q.la, q.lb) * B->alph.maxS > 32767 - 2L * B->alph.maxS) B->may_saturate = true;
+    } else {
+      h.path = global_i16_safe(*p, B->alph.maxS, q.la, q.lb) ? 2 : 1;
+    }
+    PairDev &d = B->pairs_h[k];
+    d.a_off = h.a_off; d.b_off = h.b_off; d.la = q.la; d.lb = q.lb;
+    d.cig_cap = p->traceback ? 2 * (q.la + q.lb) + 8 : 0;
+    d.cig_off = B->cig_bytes;
+    B->cig_bytes += (uint64_t)d.cig_cap;
+  }
+  while (B->codes.size() % 16) B->codes.push_back(0);
+
+  // Couples: local pairs pad to the longer column count; global couples
+  // need equal column counts (so H(m, n) of both sits in the last column).
+  Plan &pl = B->plan;
+  pl.ptb.assign(n, PairTB{0, 0, 0, 0, 0});
+  std::vector<int> i16, i32;
+  for (int32_t k = 0; k < n; ++k) {
+    if (B->hp[k].path == 2) i16.push_back(k);
+    else if (B->hp[k].path == 1) i32.push_back(k);
+  }
+  auto bigger = [&](int x, int y) {
+    const HostPair &a = B->hp[x], &b = B->hp[y];
+    const int sa = ceil_div(a.la, STRIP), sb = ceil_div(b.la, STRIP);
+    if (sa != sb) return sa > sb;
+    if (a.lb != b.lb) return a.lb > b.lb;
+    return x < y;
+  };
+  std::vector<std::pair<int, int>> couples;
+  if (local) {
+    std::sort(i16.begin(), i16.end(), bigger);
+    for (size_t t = 0; t < i16.size(); t += 2)
+      couples.push_back({i16[t], t + 1 < i16.size() ? i16[t + 1] : -1});
+  } else {
+    std::map<int, std::vector<int>> byn;
+    for (int k : i16) byn[B->hp[k].lb].push_back(k);
+    for (auto &kv : byn) {
+      auto &v = kv.second;
+      std::sort(v.begin(), v.end(), bigger);
+      for (size_t t = 0; t < v.size(); t += 2) couples.push_back({v[t], t + 1 < v.size() ? v[t + 1] : -1});
+    }
+  }
+  for (auto &c : couples) {
+    const bool single = c.second < 0;
+    add_task(pl, pl.tasks2, 2, B->trace, B->hp, B->codes, c.first, single ? c.first : c.second, single, B->Kp);
+  }
+  for (int k : i32) add_task(pl, pl.tasks1, 1, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+  auto by_cost = [](const Task &x, const Task &y) { return task_cost(x) > task_cost(y); };
+  std::stable_sort(pl.tasks2.begin(), pl.tasks2.end(), by_cost);
+  std::stable_sort(pl.tasks1.begin(), pl.tasks1.end(), by_cost);
+
+  // Upload.
+  hipStream_t st = ctx->stream;
+  std::vector<int8_t> table = padded_table(B->alph);
+  int rc = TA_OK;
+  rc = rc ? rc : upload(&B->d_codes, B->codes, st);
+  rc = rc ? rc : upload(&B->d_table, table, st);
+  rc = rc ? rc : upload(&B->d_tasks2, pl.tasks2, st);
+  rc = rc ? rc : upload(&B->d_tasks1, pl.tasks1, st);
+  rc = rc ? rc : upload(&B->d_rowcodes, pl.rowcodes, st);
+  rc = rc ? rc : upload(&B->d_colcodes, pl.colcodes, st);
+  rc = rc ? rc : upload(&B->d_ptb, pl.ptb, st);
+  rc = rc ? rc : upload(&B->d_pairs, B->pairs_h, st);
+  rc = rc ? rc : upload(&B->d_fo_init, B->fo_init, st);
+  rc = rc ? rc : dalloc(&B->d_ring, pl.ring_elems);
+  rc = rc ? rc : dalloc(&B->d_tb, pl.tb_bytes);
+  rc = rc ? rc : dalloc(&B->d_fo, (size_t)std::max(n, 1));
+  rc = rc ? rc : dalloc(&B->d_res, (size_t)std::max(n, 1));
+  rc = rc ? rc : dalloc(&B->d_cig, B->cig_bytes);
+  rc = rc ? rc : dalloc(&B->d_ctr, 4);
+  for (auto &e : B->ev)
+    if (!rc && hipEventCreate(&e) != hipSuccess) rc = TA_E_DEVICE;
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = TA_E_DEVICE;
+  if (rc) {
+    ta_batch_destroy(B);
+    return rc;
+  }
+  *out = B;
+  return TA_OK;
+}
+
+int ta_batch_info(ta_batch *B, int64_t *cells, int64_t *tb_bytes, int32_t *tasks) {
+  if (!B) return TA_E_INVALID;
+  if (cells) *cells = B->cells;
+  if (tb_bytes) *tb_bytes = (int64_t)B->plan.tb_bytes;
+  if (tasks) *tasks = (int32_t)(B->plan.tasks2.size() + B->plan.tasks1.size());
+  return TA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers.
+namespace {
+
+template <int P>
+int launch_fill(ta_ctx *ctx, ta_batch *B, const Task *d_tasks, int ntasks, uint16_t *rows, uint16_t *cols,
+                uint32_t *ring, uint8_t *tb, int32_t *ctr, int max_groups) {
+  if (ntasks == 0) return TA_OK;
+  FillArgs a;
+  a.tasks = d_tasks; a.ntasks = ntasks; a.ctr = ctr;
+  a.table = B->d_table; a.Kp = B->Kp;
+  a.rowcodes = rows; a.colcodes = cols; a.ring = ring; a.tb = tb; a.out = B->d_fo;
+  a.open = B->p.gap_open; a.ext = B->p.gap_extend;
+  a.sat_limit = 32767 - 2 * B->alph.maxS;
+  const int lds = fill_lds_bytes(P, B->Kp);
+  const int per_cu = std::max(1, std::min(8, (160 * 1024) / lds));
+  int groups = (ntasks + 3) / 4;
+  if (max_groups > 0) groups = std::min(groups, max_groups);
+  groups = std::max(1, std::min(groups, ctx->cus * per_cu));
+  const bool local = B->p.mode == TA_LOCAL;
+  void (*k)(FillArgs) = nullptr;
+  if (local) k = B->trace ? fill_kernel<P, true, true> : fill_kernel<P, true, false>;
+  else k = B->trace ? fill_kernel<P, false, true> : fill_kernel<P, false, false>;
+  hipLaunchKernelGGL(k, dim3(groups), dim3(64), lds, ctx->stream, a);
+  HIPCHK(hipGetLastError());
+  return TA_OK;
+}
+
+int launch_traceback(ta_ctx *ctx, ta_batch *B, const PairDev *pairs, const PairTB *ptb, const uint8_t *tb,
+                     int first, int count) {
+  if (count <= 0) return TA_OK;
+  TbArgs a;
+  a.pairs = pairs + first; a.npairs = count; a.fo = B->d_fo + first; a.ptb = ptb + first;
+  a.codes = B->d_codes; a.table = B->d_table; a.Kp = B->Kp; a.tb = tb; a.cig = B->d_cig;
+  a.res = B->d_res + first;
+  a.local = B->p.mode == TA_LOCAL; a.open = B->p.gap_open; a.ext = B->p.gap_extend;
+  a.do_cigar = B->p.traceback;
+  hipLaunchKernelGGL(traceback_kernel, dim3((count + 63) / 64), dim3(64), 0, ctx->stream, a);
+  HIPCHK(hipGetLastError());
+  return TA_OK;
+}
+
+// Pairs whose int16 pass saturated go through the int32 kernel.
+int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
+  std::vector<FillOut> fo(B->n);
+  HIPCHK(hipMemcpyAsync(fo.data(), B->d_fo, sizeof(FillOut) * B->n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::vector<int> bad;
+  for (int k = 0; k < B->n; ++k)
+    if (fo[k].flags & TA_F_RERUN) bad.push_back(k);
+  free_rerun(B);
+  if (bad.empty()) return TA_OK;
+  Plan pl;
+  pl.ptb = B->plan.ptb;
+  for (int k : bad) {
+    add_task(pl, pl.tasks1, 1, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+    pl.tasks1.back().flags_or = TA_F_RERUN;
+  }
+  hipStream_t st = ctx->stream;
+  int rc = TA_OK;
+  rc = rc ? rc : upload(&B->d_tasks_r, pl.tasks1, st);
+  rc = rc ? rc : upload(&B->d_rowcodes_r, pl.rowcodes, st);
+  rc = rc ? rc : upload(&B->d_colcodes_r, pl.colcodes, st);
+  rc = rc ? rc : dalloc(&B->d_ring_r, pl.ring_elems);
+  rc = rc ? rc : dalloc(&B->d_tb_r, pl.tb_bytes);
+  if (rc) return rc;
+  B->n_rerun = (int)bad.size();
+  // Rerun pairs read their planes from the rerun arena: point their PairTB there
+  // by offsetting against the main arena base.
+  std::vector<PairTB> ptb(B->plan.ptb);
+  for (int k : bad) {
+    ptb[k] = pl.ptb[k];
+    ptb[k].tb_off += (uint64_t)(B->d_tb_r - B->d_tb);
+  }
+  HIPCHK(hipMemcpyAsync(B->d_ptb, ptb.data(), sizeof(PairTB) * B->n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(B->d_ctr + 2, 0, sizeof(int32_t), st));
+  return launch_fill<1>(ctx, B, B->d_tasks_r, (int)pl.tasks1.size(), B->d_rowcodes_r, B->d_colcodes_r,
+                        B->d_ring_r, B->d_tb_r, B->d_ctr + 2, 0);
+}
+
+}  // namespace
+
+int ta_batch_run(ta_ctx *ctx, ta_batch *B) {
+  if (!ctx || !B) return TA_E_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const Plan &pl = B->plan;
+  if (B->n > 0)
+    HIPCHK(hipMemcpyAsync(B->d_fo, B->d_fo_init, sizeof(FillOut) * B->n, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemsetAsync(B->d_ctr, 0, 4 * sizeof(int32_t), st));
+  HIPCHK(hipEventRecord(B->ev[0], st));
+  int rc = launch_fill<2>(ctx, B, B->d_tasks2, (int)pl.tasks2.size(), B->d_rowcodes, B->d_colcodes, B->d_ring,
+                          B->d_tb, B->d_ctr, 0);
+  if (!rc)
+    rc = launch_fill<1>(ctx, B, B->d_tasks1, (int)pl.tasks1.size(), B->d_rowcodes, B->d_colcodes, B->d_ring,
+                        B->d_tb, B->d_ctr + 1, 0);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(B->ev[1], st));
+  if (B->may_saturate) {
+    rc = rerun_saturated(ctx, B);
+    if (rc) return rc;
+  } else if (B->n_rerun) {
+    free_rerun(B);
+  }
+  HIPCHK(hipEventRecord(B->ev[2], st));
+  rc = launch_traceback(ctx, B, B->d_pairs, B->d_ptb, B->d_tb, 0, B->n);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(B->ev[3], st));
+  return TA_OK;
+}
+
+int ta_batch_wait(ta_ctx *ctx, ta_batch *B) {
+  if (!ctx || !B) return TA_E_INVALID;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  float t = 0;
+  if (hipEventElapsedTime(&t, B->ev[0], B->ev[1]) == hipSuccess) B->ms[0] = t;
+  if (hipEventElapsedTime(&t, B->ev[1], B->ev[2]) == hipSuccess) B->ms[1] = t;
+  if (hipEventElapsedTime(&t, B->ev[2], B->ev[3]) == hipSuccess) B->ms[2] = t;
+  if (hipEventElapsedTime(&t, B->ev[0], B->ev[3]) == hipSuccess) B->ms[3] = t;
+  return TA_OK;
+}
+
+int ta_batch_timings(ta_batch *B, double *ms, int32_t cap) {
+  if (!B || !ms) return TA_E_INVALID;
+  int k = 0;
+  for (; k < cap && k < 4; ++k) ms[k] = B->ms[k];
+  return k;
+}
+
+int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, int64_t cap, int64_t *used) {
+  if (!ctx || !B || (B->n && !out)) return TA_E_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<ResultDev> res(B->n);
+  std::vector<char> cig(B->cig_bytes);
+  if (B->n) HIPCHK(hipMemcpyAsync(res.data(), B->d_res, sizeof(ResultDev) * B->n, hipMemcpyDeviceToHost, ctx->stream));
+  if (B->cig_bytes && B->p.traceback)
+    HIPCHK(hipMemcpyAsync(cig.data(), B->d_cig, B->cig_bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int64_t pos = 0;
+  int rc = TA_OK;
+  for (int32_t k = 0; k < B->n; ++k) {
+    const ResultDev &r = res[k];
+    ta_result &o = out[k];
+    o.score = r.score; o.a_begin = r.a_begin; o.a_end = r.a_end; o.b_begin = r.b_begin; o.b_end = r.b_end;
+    o.flags = r.flags;
+    o.cigar_len = B->p.traceback ? r.cigar_len : 0;
+    o.cigar_off = pos;
+    if (o.cigar_len) {
+      if (cigar_buf && pos + o.cigar_len <= cap) memcpy(cigar_buf + pos, cig.data() + r.cigar_off, o.cigar_len);
+      else rc = TA_E_SPACE;
+    }
+    pos += o.cigar_len;
+  }
+  if (used) *used = pos;
+  return rc;
+}
+
+int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, ta_result *out,
+                   char *cigar_buf, int64_t cap, int64_t *used) {
+  ta_batch *B = nullptr;
+  int rc = ta_batch_create(ctx, p, pairs, n, &B);
+  if (rc) return rc;
+  rc = ta_batch_run(ctx, B);
+  if (!rc) rc = ta_batch_wait(ctx, B);
+  if (!rc) rc = ta_batch_results(ctx, B, out, cigar_buf, cap, used);
+  ta_batch_destroy(B);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Plain form: one fill launch (one group, one task) and one traceback launch
+// per pair, serialised on the stream -- the baseline batching is measured
+// against.
+int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
+  if (!ctx || !B || first < 0 || count < 0 || first + count > B->n) return TA_E_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (!B->plain_ready) {
+    Plan pl;
+    pl.ptb.assign(B->n, PairTB{0, 0, 0, 0, 0});
+    uint64_t max_tb = 0, max_ring = 0;
+    std::vector<uint64_t> tb_at(B->n, 0), ring_at(B->n, 0);
+    for (int k = 0; k < B->n; ++k) {
+      const HostPair &h = B->hp[k];
+      const int P = h.path == 1 ? 1 : 2;
+      const uint64_t tb0 = pl.tb_bytes, r0 = pl.ring_elems;
+      if (h.path == 0) {
+        pl.tasks2.push_back(Task{});
+        continue;
+      }
+      add_task(pl, pl.tasks2, P, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+      // Every plain launch reuses the same scratch: rebase to 0.
+      Task &T = pl.tasks2.back();
+      max_tb = std::max(max_tb, pl.tb_bytes - tb0);
+      max_ring = std::max(max_ring, pl.ring_elems - r0);
+      T.tb_off = 0; T.ring_off = 0;
+      pl.ptb[k].tb_off = 0;
+      pl.tb_bytes = tb0; pl.ring_elems = r0;
+    }
+    int rc = TA_OK;
+    rc = rc ? rc : upload(&B->d_plain_tasks, pl.tasks2, st);
+    rc = rc ? rc : upload(&B->d_prow, pl.rowcodes, st);
+    rc = rc ? rc : upload(&B->d_pcol, pl.colcodes, st);
+    rc = rc ? rc : upload(&B->d_pptb, pl.ptb, st);
+    rc = rc ? rc : dalloc(&B->d_pring, max_ring);
+    rc = rc ? rc : dalloc(&B->d_ptbuf, max_tb);
+    if (rc) return rc;
+    B->plain_tasks = pl.tasks2;
+    B->plain_ready = true;
+  }
+  HIPCHK(hipEventRecord(B->ev[0], st));
+  if (B->n > 0)
+    HIPCHK(hipMemcpyAsync(B->d_fo, B->d_fo_init, sizeof(FillOut) * B->n, hipMemcpyDeviceToDevice, st));
+  for (int k = first; k < first + count; ++k) {
+    const HostPair &h = B->hp[k];
+    if (h.path) {
+      HIPCHK(hipMemsetAsync(B->d_ctr, 0, sizeof(int32_t), st));
+      int rc = h.path == 1
+                   ? launch_fill<1>(ctx, B, B->d_plain_tasks + k, 1, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf,
+                                    B->d_ctr, 1)
+                   : launch_fill<2>(ctx, B, B->d_plain_tasks + k, 1, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf,
+                                    B->d_ctr, 1);
+      if (rc) return rc;
+    }
+    int rc = launch_traceback(ctx, B, B->d_pairs, B->d_pptb, B->d_ptbuf, k, 1);
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(B->ev[1], st));
+  HIPCHK(hipEventRecord(B->ev[2], st));
+  HIPCHK(hipEventRecord(B->ev[3], st));
+  return TA_OK;
+}
