@@ -4,15 +4,16 @@
 
 namespace ta {
 
-// One DP group = one DPP row of 16 lanes.  A wave64 runs 4 groups; each group
-// owns one task (a pair, or a couple of pairs packed into int16 halves).
+// A DP group is 16 lanes (one DPP row; a wave64 runs 4 groups) or the whole
+// 64-lane wave (long tasks).  A group owns one task (a pair, or a couple of
+// pairs packed into int16 halves) and sweeps it strip by strip.
 constexpr int G = 16;
-// Rows per lane held in registers (a strip is G * R rows of the query).
+constexpr int GWAVE = 64;
+// Rows per lane held in registers (a strip is GG * R rows of the query).
 constexpr int R = 8;
 constexpr int STRIP = G * R;
-// Ring index of column j is j + RING_LEAD (keeps lane 15's block stores and
+// Ring index of column j is j + GG (keeps the last lane's block stores and
 // lane 0's block loads in bounds and lane 0's loads 16-byte aligned).
-constexpr int RING_LEAD = 16;
 
 constexpr int16_t NEG16 = -16384;          // -inf of the int16 lanes
 constexpr int32_t NEG32 = INT32_MIN / 4;   // -inf of the int32 lanes
@@ -24,8 +25,8 @@ struct Task {
   int32_t m[2];        // query rows of each half
   int32_t off[2];      // leading pad columns of each half (local couples)
   int32_t n;           // columns of the task
-  int32_t S;           // strips = ceil(max m / STRIP)
-  int32_t NB;          // 8-step blocks per strip = ceil((n + G - 1) / 8)
+  int32_t S;           // strips = ceil(max m / (GG * R))
+  int32_t NB;          // 8-step blocks per strip = ceil((n + GG) / 8)
   int32_t single;      // hi half duplicates lo (results of hi discarded)
   uint64_t row_off;    // rowcodes element offset (S * STRIP uint16)
   uint64_t col_off;    // colcodes element offset (8 * NB + 8 uint16)
@@ -49,6 +50,8 @@ struct PairTB {
   int32_t half;   // 0 lo, 1 hi
   int32_t NB;
   int32_t off;    // leading pad columns
+  int32_t GG;     // lanes per group of the task (16 or 64)
+  int32_t pad_;
 };
 
 // Pair as the runtime keeps it on the device.
@@ -68,8 +71,8 @@ struct ResultDev {
 };
 
 // Bytes of direction planes per strip of a task.
-inline uint64_t tb_strip_bytes(int P, int NB) {
-  return (uint64_t)NB * G * 8 * (P == 2 ? 8 : 4);
+inline uint64_t tb_strip_bytes(int P, int NB, int GG) {
+  return (uint64_t)NB * GG * 8 * (P == 2 ? 8 : 4);
 }
 
 }  // namespace ta

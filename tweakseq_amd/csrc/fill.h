@@ -17,7 +17,9 @@
 //    are staged in a private LDS slot at the start of each strip, so a step
 //    costs one LDS read per half;
 //  * direction bits (4 per cell) leave as sign bits of int differences,
-//    packed per step and stored per 8-step block with dwordx4 stores.
+//    packed per step and stored per 8-step block with dwordx4 stores, lane
+//    major ([strip][lane][block]) so a traceback reads a lane's consecutive
+//    blocks in one go.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -58,9 +60,9 @@ struct Lane<1> {
 };
 
 struct FillArgs {
-  const Task *tasks;
-  int32_t ntasks;
-  int32_t *ctr;          // work-queue head
+  const Task *tasks[2];  // [0] whole-wave tasks (GG = 64), [1] 16-lane tasks
+  int32_t ntasks[2];
+  int32_t *ctr;          // work-queue heads ctr[0], ctr[1]
   const int8_t *table;   // Kp x Kp, row/column Kp-1 = pad
   int32_t Kp;
   const uint16_t *rowcodes;
@@ -72,9 +74,11 @@ struct FillArgs {
   int32_t sat_limit;     // local int16: best score above this -> int32 rerun
 };
 
-// DPP row_shr:1 inside each 16-lane row; lane 0 of the row takes `old`.
+// Shift one lane up inside a group: DPP row_shr:1 (16-lane rows) or
+// wave_shr:1 (whole wave); the group's lane 0 takes `old`.
+template <int GG>
 __device__ __forceinline__ int shr1(int old, int v) {
-  return __builtin_amdgcn_update_dpp(old, v, 0x111, 0xf, 0xf, false);
+  return __builtin_amdgcn_update_dpp(old, v, GG == 16 ? 0x111 : 0x138, 0xf, 0xf, false);
 }
 
 // LDS bytes the fill kernel needs for a table of Kp codes.
@@ -100,21 +104,60 @@ __device__ __forceinline__ uint32_t prof2(uint32_t a, uint32_t b, uint32_t as, u
   }
 }
 
-template <int P, bool LOCAL, bool TB>
-__global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
+// A lane's profile slot for its R = 8 rows: slot[h][c][k] = S(row code k of
+// half h, column code c), built by transposing the rows' table lines with
+// v_perm (rows are read as whole 32-byte lines; c runs over 4-byte groups
+// with compile-time byte positions so nothing is indexed dynamically).
+template <int P>
+__device__ __forceinline__ void build_slot(const int8_t *tab, int8_t *slot, int Kp, uint4 rc) {
+  const uint32_t rcw[4] = {rc.x, rc.y, rc.z, rc.w};
+#pragma unroll
+  for (int h = 0; h < P; ++h) {
+    uint4 lo[R], hi[R];  // table line of each row: bytes 0..15, 16..31
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int code = (rcw[k >> 1] >> (16 * (k & 1) + 8 * h)) & 0xff;
+      lo[k] = *(const uint4 *)(tab + 32 * code);
+      hi[k] = *(const uint4 *)(tab + 32 * code + 16);
+    }
+    int8_t *dst = slot + h * Kp * R;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      if (4 * d >= Kp) break;
+      uint32_t w[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const uint4 &q = d < 4 ? lo[k] : hi[k];
+        const int e = d & 3;
+        w[k] = e == 0 ? q.x : (e == 1 ? q.y : (e == 2 ? q.z : q.w));
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int c = 4 * d + b;
+        if (c >= Kp) break;
+        // byte b of rows (0,1), (2,3), ... gathered into one dword per 4 rows
+        const uint32_t sel = (uint32_t)b | ((uint32_t)(b + 4) << 8);
+        const uint32_t x01 = __builtin_amdgcn_perm(w[1], w[0], sel);
+        const uint32_t x23 = __builtin_amdgcn_perm(w[3], w[2], sel);
+        const uint32_t x45 = __builtin_amdgcn_perm(w[5], w[4], sel);
+        const uint32_t x67 = __builtin_amdgcn_perm(w[7], w[6], sel);
+        const uint32_t a = __builtin_amdgcn_perm(x23, x01, 0x05040100u);
+        const uint32_t bb = __builtin_amdgcn_perm(x67, x45, 0x05040100u);
+        *(uint2 *)(dst + c * R) = make_uint2(a, bb);
+      }
+    }
+  }
+}
+
+template <int GG, int P, bool LOCAL, bool TB>
+__device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks, int ntasks, int32_t *ctr,
+                                           const int8_t *tab, int8_t *slot) {
   using L = Lane<P>;
   using V = typename L::V;
-  extern __shared__ __align__(16) uint8_t lds[];
-
+  constexpr int ST = GG * R;  // rows per strip
   const int lane = threadIdx.x;
-  const int l = lane & (G - 1);
+  const int l = lane & (GG - 1);
   const int Kp = A.Kp;
-  const int SS = fill_slot_stride(P, Kp);
-
-  int8_t *tab = (int8_t *)lds;
-  for (int x = lane; x < Kp * Kp; x += 64) tab[x] = A.table[x];
-  __syncthreads();
-  int8_t *slot = (int8_t *)lds + 1024 + lane * SS;
 
   const V vopen = L::splat(A.open);
   const V vext = L::splat(A.ext);
@@ -132,11 +175,13 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
   // DP registers.
   V H[R], E[R], HoL[R];
   V hd0 = vzero, Hbot = vzero, Fbot = vzero;
-  int ccur = 0;
+  int ccur = 0;             // residue codes of the current step (lo | hi << 8)
+  uint2 pa_n = {0, 0}, pb_n = {0, 0};  // profile rows of the current step, read one step early
   uint32_t key0[R], key1[R];  // local end-cell keys (P == 2: lo / hi half)
   int rmax[R], rcol[R];       // local end-cell tracking (P == 1)
   int bestH0 = 0, besti0 = 0, bestj0 = 0, bestH1 = 0, besti1 = 0, bestj1 = 0;
   uint4 ccodes = {0, 0, 0, 0}, ccodes_n = {0, 0, 0, 0};
+  uint4 rc_next = {0, 0, 0, 0};  // row codes of the next strip (prefetched)
   uint4 rH0 = {0, 0, 0, 0}, rH1 = {0, 0, 0, 0}, rF0 = {0, 0, 0, 0}, rF1 = {0, 0, 0, 0};
   uint4 nH0 = rH0, nH1 = rH0, nF0 = rH0, nF1 = rH0;
   int carryH = 0, carryF = 0;
@@ -148,14 +193,14 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
 
   auto fetch = [&]() {
     int t = 0;
-    if (l == 0) t = atomicAdd(A.ctr, 1);
-    t = __shfl(t, lane & ~(G - 1), 64);
-    if (t >= A.ntasks) {
+    if (l == 0) t = atomicAdd(ctr, 1);
+    t = __shfl(t, lane & ~(GG - 1), 64);
+    if (t >= ntasks) {
       alive = false;
       return;
     }
     task = t;
-    const Task &T = A.tasks[t];
+    const Task &T = tasks[t];
     n = T.n; S = T.S; NB = T.NB; single = T.single;
     m0 = T.m[0]; m1 = T.m[1]; off0 = T.off[0]; off1 = T.off[1];
     pid0 = T.pair[0]; pid1 = T.pair[1];
@@ -176,18 +221,10 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
 
     if (b == 0) {
       // ---- strip start: profile slot, left boundary, first block inputs.
-      const uint4 rc = *(const uint4 *)(A.rowcodes + row_off + (uint64_t)s * STRIP + l * R);
-      const uint32_t rcw[4] = {rc.x, rc.y, rc.z, rc.w};
-      for (int h = 0; h < P; ++h) {
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-          const int code = (rcw[k >> 1] >> (16 * (k & 1) + 8 * h)) & 0xff;
-          const int8_t *trow = tab + code * Kp;
-          int8_t *dst = slot + h * Kp * R + k;
-          for (int c = 0; c < Kp; ++c) dst[c * R] = trow[c];
-        }
-      }
-      const int row0 = s * STRIP + l * R;  // 0-based first row of this lane
+      const uint4 rc = s == 0 ? *(const uint4 *)(A.rowcodes + row_off + l * R) : rc_next;
+      if (s + 1 < S) rc_next = *(const uint4 *)(A.rowcodes + row_off + (uint64_t)(s + 1) * ST + l * R);
+      build_slot<P>(tab, slot, Kp, rc);
+      const int row0 = s * ST + l * R;  // 0-based first row of this lane
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         H[k] = L::splat(left(row0 + k + 1));
@@ -196,10 +233,15 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
         key0[k] = 0; key1[k] = 0; rmax[k] = 0; rcol[k] = 0;
       }
       hd0 = L::splat(left(row0));
-      ccur = (Kp - 1) | ((Kp - 1) << 8);
       ccodes_n = *(const uint4 *)(A.colcodes + col_off);
+      // Residue of step 0: lane 0 takes column 0, the others a pad code
+      // (they are idle until their first column arrives).
+      ccur = shr1<GG>((int)(ccodes_n.x & 0xffff), (Kp - 1) | ((Kp - 1) << 8));
+      pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
+      pb_n = pa_n;
+      if constexpr (P == 2) pb_n = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
       if (s > 0) {
-        const uint32_t *rh = A.ring + ring_off + RING_LEAD;
+        const uint32_t *rh = A.ring + ring_off + GG;
         const uint32_t *rf = rh + ring_len;
         nH0 = *(const uint4 *)rh; nH1 = *(const uint4 *)(rh + 4);
         nF0 = *(const uint4 *)rf; nF1 = *(const uint4 *)(rf + 4);
@@ -212,13 +254,14 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
     if (b + 1 < NB) {  // prefetch the next block's lane-0 inputs
       ccodes_n = *(const uint4 *)(A.colcodes + col_off + 8 * (b + 1));
       if (s > 0) {
-        const uint32_t *rh = A.ring + ring_off + RING_LEAD + 8 * (b + 1);
+        const uint32_t *rh = A.ring + ring_off + GG + 8 * (b + 1);
         const uint32_t *rf = rh + ring_len;
         nH0 = *(const uint4 *)rh; nH1 = *(const uint4 *)(rh + 4);
         nF0 = *(const uint4 *)rf; nF1 = *(const uint4 *)(rf + 4);
       }
     }
     const uint32_t ccw[4] = {ccodes.x, ccodes.y, ccodes.z, ccodes.w};
+    const uint32_t next_first = ccodes_n.x & 0xffff;  // lane 0's residue at step 8b + 8
     const uint32_t rhw[8] = {rH0.x, rH0.y, rH0.z, rH0.w, rH1.x, rH1.y, rH1.z, rH1.w};
     const uint32_t rfw[8] = {rF0.x, rF0.y, rF0.z, rF0.w, rF1.x, rF1.y, rF1.z, rF1.w};
     uint32_t tbw[16];
@@ -228,8 +271,17 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
     for (int q = 0; q < 8; ++q) {
       const int t = 8 * b + q;
       const int j = t - l;
-      const int code0 = (ccw[q >> 1] >> (16 * (q & 1))) & 0xffff;
-      ccur = shr1(code0, ccur);
+      // This step's residues and profile rows were fetched last step; start
+      // the LDS reads of the next step now (lane l's next residue is lane
+      // l-1's current one).
+      const uint2 pa = pa_n;
+      const uint2 pb = pb_n;
+      const int cthis = ccur;
+      const int code1 = q < 7 ? (int)((ccw[(q + 1) >> 1] >> (16 * ((q + 1) & 1))) & 0xffff) : (int)next_first;
+      ccur = shr1<GG>(code1, cthis);
+      pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
+      if constexpr (P == 2) pb_n = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
+      else pb_n = pa_n;
       int topH, topF;
       if (s == 0) {
         topH = L::bits(L::splat(LOCAL ? 0 : A.open + t * A.ext));
@@ -238,14 +290,10 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
         topH = (int)rhw[q];
         topF = (int)rfw[q];
       }
-      const V Hup = L::unbits(shr1(topH, L::bits(Hbot)));
-      const V Fup = L::unbits(shr1(topF, L::bits(Fbot)));
+      const V Hup = L::unbits(shr1<GG>(topH, L::bits(Hbot)));
+      const V Fup = L::unbits(shr1<GG>(topF, L::bits(Fbot)));
 
       if (j >= 0 && j < n) {
-        const int c0 = ccur & 0xff;
-        const uint2 pa = *(const uint2 *)(slot + c0 * R);
-        uint2 pb = pa;
-        if constexpr (P == 2) pb = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
         const uint32_t pas_x = pa.x << 8, pas_y = pa.y << 8, pbs_x = pb.x << 8, pbs_y = pb.y << 8;
         const uint32_t cj = 0xffffu - (uint32_t)j;
 
@@ -322,20 +370,21 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
     // ---- block outputs: direction planes (all lanes), ring (lane 15).
     if constexpr (TB) {
       if constexpr (P == 2) {
-        uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * NB + b) * G + l) * 64));
+        uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * GG + l) * NB + b) * 64));
         dst[0] = make_uint4(tbw[0], tbw[1], tbw[2], tbw[3]);
         dst[1] = make_uint4(tbw[4], tbw[5], tbw[6], tbw[7]);
         dst[2] = make_uint4(tbw[8], tbw[9], tbw[10], tbw[11]);
         dst[3] = make_uint4(tbw[12], tbw[13], tbw[14], tbw[15]);
       } else {
-        uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * NB + b) * G + l) * 32));
+        uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * GG + l) * NB + b) * 32));
         dst[0] = make_uint4(tbw[0], tbw[1], tbw[2], tbw[3]);
         dst[1] = make_uint4(tbw[4], tbw[5], tbw[6], tbw[7]);
       }
     }
-    if (l == G - 1 && s + 1 < S) {
-      // Lane 15 handles column t - 15 at step t; ring index = column + 16,
-      // so block b carries indices 8b .. 8b+7: last block's q = 7 then q 0..6.
+    if (l == GG - 1 && s + 1 < S) {
+      // The last lane handles column t - (GG-1) at step t; ring index =
+      // column + GG, so block b carries indices 8b .. 8b+7: the previous
+      // block's q = 7 then q 0..6.
       uint32_t *rh = A.ring + ring_off + 8 * b;
       uint32_t *rf = rh + ring_len;
       *(uint4 *)rh = make_uint4(carryH, rbH[0], rbH[1], rbH[2]);
@@ -352,7 +401,7 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
     // ---- strip end.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const int row0 = s * STRIP + l * R;
+    const int row0 = s * ST + l * R;
     if constexpr (LOCAL) {
 #pragma unroll
       for (int k = 0; k < R; ++k) {
@@ -395,7 +444,7 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
       for (int h = 0; h < P; ++h) {
         int bh = h ? bestH1 : bestH0, bi = h ? besti1 : besti0, bj = h ? bestj1 : bestj0;
 #pragma unroll
-        for (int w = 8; w >= 1; w >>= 1) {
+        for (int w = GG / 2; w >= 1; w >>= 1) {
           const int oh = __shfl_xor(bh, w, 64), oi = __shfl_xor(bi, w, 64), oj = __shfl_xor(bj, w, 64);
           const bool take = oh > bh || (oh == bh && (oi < bi || (oi == bi && oj < bj)));
           if (take) { bh = oh; bi = oi; bj = oj; }
@@ -413,6 +462,23 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
     }
     fetch();
   }
+}
+
+template <int P, bool LOCAL, bool TB>
+__global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int lane = threadIdx.x;
+  // Substitution table in LDS with 32-byte rows (row r at tab + 32 r).
+  int8_t *tab = (int8_t *)lds;
+  for (int x = lane; x < 32 * 32; x += 64) {
+    const int r = x >> 5, c = x & 31;
+    tab[x] = (r < A.Kp && c < A.Kp) ? A.table[r * A.Kp + c] : (int8_t)0;
+  }
+  __syncthreads();
+  int8_t *slot = (int8_t *)lds + 1024 + lane * fill_slot_stride(P, A.Kp);
+  // Long tasks first, one per wave; then 16-lane groups take the rest.
+  if (A.ntasks[0]) fill_phase<GWAVE, P, LOCAL, TB>(A, A.tasks[0], A.ntasks[0], A.ctr, tab, slot);
+  if (A.ntasks[1]) fill_phase<G, P, LOCAL, TB>(A, A.tasks[1], A.ntasks[1], A.ctr + 1, tab, slot);
 }
 
 }  // namespace ta

@@ -229,7 +229,7 @@ struct HostPair {
 };
 
 struct Plan {
-  std::vector<Task> tasks2, tasks1;
+  std::vector<Task> tasks2[2], tasks1[2];  // [0] whole-wave groups, [1] 16-lane groups
   std::vector<uint16_t> rowcodes, colcodes;
   std::vector<PairTB> ptb;
   uint64_t ring_elems = 0, tb_bytes = 0;
@@ -237,22 +237,24 @@ struct Plan {
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
-// Append one task (pair lo / hi; hi == lo for a single) to the plan.
-void add_task(Plan &pl, std::vector<Task> &dst, int P, bool trace, const std::vector<HostPair> &hp,
+// Append one task (pair lo / hi; hi == lo for a single) to the plan, swept
+// by groups of GG lanes.
+void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const std::vector<HostPair> &hp,
               const std::vector<uint8_t> &codes, int lo, int hi, bool single, int Kp) {
   const HostPair &A = hp[lo], &B = hp[hi];
+  const int ST = GG * R;
   Task T{};
   T.pair[0] = lo; T.pair[1] = hi;
   T.m[0] = A.la; T.m[1] = B.la;
   T.n = std::max(A.lb, B.lb);
   T.off[0] = T.n - A.lb; T.off[1] = T.n - B.lb;
-  T.S = ceil_div(std::max(A.la, B.la), STRIP);
-  T.NB = ceil_div(T.n + 16, 8);
+  T.S = ceil_div(std::max(A.la, B.la), ST);
+  T.NB = ceil_div(T.n + GG, 8);
   T.single = single ? 1 : 0;
   const uint16_t pad = (uint16_t)(Kp - 1);
-  // Row codes: S * STRIP entries, pad rows past each pair's m.
+  // Row codes: S * ST entries, pad rows past each pair's m.
   T.row_off = pl.rowcodes.size();
-  for (int r = 0; r < T.S * STRIP; ++r) {
+  for (int r = 0; r < T.S * ST; ++r) {
     uint16_t ca = r < A.la ? codes[A.a_off + r] : pad;
     uint16_t cb = r < B.la ? codes[B.a_off + r] : pad;
     pl.rowcodes.push_back((uint16_t)(ca | (cb << 8)));
@@ -267,19 +269,31 @@ void add_task(Plan &pl, std::vector<Task> &dst, int P, bool trace, const std::ve
     pl.colcodes.push_back((uint16_t)(ca | (cb << 8)));
   }
   while (pl.colcodes.size() % 8) pl.colcodes.push_back(pad | (pad << 8));
-  T.ring_len = T.S > 1 ? 8 * T.NB + 32 : 0;
+  T.ring_len = T.S > 1 ? 8 * T.NB + 2 * GG + 16 : 0;
   T.ring_off = pl.ring_elems;
   pl.ring_elems += 2ull * T.ring_len;
   T.tb_off = trace ? pl.tb_bytes : 0;
-  if (trace) pl.tb_bytes += (uint64_t)T.S * tb_strip_bytes(P, T.NB);
+  if (trace) pl.tb_bytes += (uint64_t)T.S * tb_strip_bytes(P, T.NB, GG);
   for (int h = 0; h < (single ? 1 : 2); ++h) {
     PairTB &d = pl.ptb[h ? hi : lo];
-    d.tb_off = T.tb_off; d.P = P; d.half = h; d.NB = T.NB; d.off = T.off[h];
+    d.tb_off = T.tb_off; d.P = P; d.half = h; d.NB = T.NB; d.off = T.off[h]; d.GG = GG;
   }
   dst.push_back(T);
 }
 
+// Steps a 16-lane group needs for a task of these shapes, in 8-step blocks.
+double g16_cost(int mmax, int n) { return (double)ceil_div(mmax, G * R) * ceil_div(n + G, 8); }
+
 double task_cost(const Task &T) { return (double)T.S * T.NB; }
+
+// Whole-wave groups for tasks long enough to set the makespan of a 16-lane
+// sweep: more than 0.75x the mean work of a 16-lane slot (a list-scheduling
+// simulation of the 65,536-pair log-normal batch gives 99% slot efficiency
+// there, 68% at 1.5x), and tall enough (>= 384 rows) not to waste most of a
+// 512-row strip.
+int pick_gg(int mmax, int n, double mean_slot) {
+  return (mmax >= 3 * G * R && g16_cost(mmax, n) > 0.75 * mean_slot) ? GWAVE : G;
+}
 
 }  // namespace
 
@@ -295,13 +309,14 @@ struct ta_batch {
   std::vector<uint8_t> codes;
   std::vector<FillOut> fo_init;
   FillOut *d_fo_init = nullptr;
+  int32_t *d_order = nullptr;  // traceback walker -> pair, longest first
   std::vector<PairDev> pairs_h;
   uint64_t cig_bytes = 0;
   Plan plan;
   // device
   uint8_t *d_codes = nullptr;
   int8_t *d_table = nullptr;
-  Task *d_tasks2 = nullptr, *d_tasks1 = nullptr;
+  Task *d_tasks2[2] = {nullptr, nullptr}, *d_tasks1[2] = {nullptr, nullptr};
   uint16_t *d_rowcodes = nullptr, *d_colcodes = nullptr;
   uint32_t *d_ring = nullptr;
   uint8_t *d_tb = nullptr;
@@ -348,10 +363,11 @@ void free_rerun(ta_batch *B) {
 
 void ta_batch_destroy(ta_batch *B) {
   if (!B) return;
-  dfree(B->d_codes); dfree(B->d_table); dfree(B->d_tasks2); dfree(B->d_tasks1);
+  dfree(B->d_codes); dfree(B->d_table);
+  for (int g = 0; g < 2; ++g) { dfree(B->d_tasks2[g]); dfree(B->d_tasks1[g]); }
   dfree(B->d_rowcodes); dfree(B->d_colcodes); dfree(B->d_ring); dfree(B->d_tb);
   dfree(B->d_ptb); dfree(B->d_pairs); dfree(B->d_fo); dfree(B->d_res); dfree(B->d_cig);
-  dfree(B->d_ctr); dfree(B->d_fo_init);
+  dfree(B->d_ctr); dfree(B->d_fo_init); dfree(B->d_order);
   free_rerun(B);
   dfree(B->d_plain_tasks); dfree(B->d_prow); dfree(B->d_pcol); dfree(B->d_pring);
   dfree(B->d_ptbuf); dfree(B->d_pptb);
@@ -385,6 +401,7 @@ int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   auto enc = [&](const char *s, int32_t len, uint64_t *off) -> int {
     auto it = seen.find(s);
     if (it != seen.end() && it->second.second == len) { *off = it->second.first; return TA_OK; }
+    while (B->codes.size() % 16) B->codes.push_back(0);  // 16-byte aligned starts (walker windows)
     *off = B->codes.size();
     for (int32_t i = 0; i < len; ++i) {
       int c = B->alph.map[(unsigned char)s[i]];
@@ -428,16 +445,17 @@ int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     }
     PairDev &d = B->pairs_h[k];
     d.a_off = h.a_off; d.b_off = h.b_off; d.la = q.la; d.lb = q.lb;
-    d.cig_cap = p->traceback ? 2 * (q.la + q.lb) + 8 : 0;
+    d.cig_cap = p->traceback ? 4 * (q.la + q.lb + 1) : 0;  // one dword per run at most
     d.cig_off = B->cig_bytes;
     B->cig_bytes += (uint64_t)d.cig_cap;
   }
   while (B->codes.size() % 16) B->codes.push_back(0);
+  B->codes.insert(B->codes.end(), 32, 0);  // the walker reads 32-byte windows
 
   // Couples: local pairs pad to the longer column count; global couples
   // need equal column counts (so H(m, n) of both sits in the last column).
   Plan &pl = B->plan;
-  pl.ptb.assign(n, PairTB{0, 0, 0, 0, 0});
+  pl.ptb.assign(n, PairTB{0, 0, 0, 0, 0, 0, 0});
   std::vector<int> i16, i32;
   for (int32_t k = 0; k < n; ++k) {
     if (B->hp[k].path == 2) i16.push_back(k);
@@ -464,14 +482,34 @@ int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       for (size_t t = 0; t < v.size(); t += 2) couples.push_back({v[t], t + 1 < v.size() ? v[t + 1] : -1});
     }
   }
+  // Mean 16-lane-slot work decides which tasks get whole-wave groups.
+  double work = 0;
+  auto shape = [&](int lo, int hi) {
+    return std::make_pair(std::max(B->hp[lo].la, B->hp[hi].la), std::max(B->hp[lo].lb, B->hp[hi].lb));
+  };
+  for (auto &c : couples) {
+    auto sh = shape(c.first, c.second < 0 ? c.first : c.second);
+    work += g16_cost(sh.first, sh.second);
+  }
+  for (int k : i32) work += g16_cost(B->hp[k].la, B->hp[k].lb);
+  const double slots = (double)ctx->cus * std::max(1, (160 * 1024) / fill_lds_bytes(2, B->Kp)) * 4;
+  const double mean_slot = work / slots;
   for (auto &c : couples) {
     const bool single = c.second < 0;
-    add_task(pl, pl.tasks2, 2, B->trace, B->hp, B->codes, c.first, single ? c.first : c.second, single, B->Kp);
+    const int hi = single ? c.first : c.second;
+    auto sh = shape(c.first, hi);
+    const int GG = pick_gg(sh.first, sh.second, mean_slot);
+    add_task(pl, pl.tasks2[GG == GWAVE ? 0 : 1], 2, GG, B->trace, B->hp, B->codes, c.first, hi, single, B->Kp);
   }
-  for (int k : i32) add_task(pl, pl.tasks1, 1, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+  for (int k : i32) {
+    const int GG = pick_gg(B->hp[k].la, B->hp[k].lb, mean_slot);
+    add_task(pl, pl.tasks1[GG == GWAVE ? 0 : 1], 1, GG, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+  }
   auto by_cost = [](const Task &x, const Task &y) { return task_cost(x) > task_cost(y); };
-  std::stable_sort(pl.tasks2.begin(), pl.tasks2.end(), by_cost);
-  std::stable_sort(pl.tasks1.begin(), pl.tasks1.end(), by_cost);
+  for (int g = 0; g < 2; ++g) {
+    std::stable_sort(pl.tasks2[g].begin(), pl.tasks2[g].end(), by_cost);
+    std::stable_sort(pl.tasks1[g].begin(), pl.tasks1[g].end(), by_cost);
+  }
 
   // Upload.
   hipStream_t st = ctx->stream;
@@ -479,19 +517,29 @@ int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   int rc = TA_OK;
   rc = rc ? rc : upload(&B->d_codes, B->codes, st);
   rc = rc ? rc : upload(&B->d_table, table, st);
-  rc = rc ? rc : upload(&B->d_tasks2, pl.tasks2, st);
-  rc = rc ? rc : upload(&B->d_tasks1, pl.tasks1, st);
+  for (int g = 0; g < 2; ++g) {
+    rc = rc ? rc : upload(&B->d_tasks2[g], pl.tasks2[g], st);
+    rc = rc ? rc : upload(&B->d_tasks1[g], pl.tasks1[g], st);
+  }
   rc = rc ? rc : upload(&B->d_rowcodes, pl.rowcodes, st);
   rc = rc ? rc : upload(&B->d_colcodes, pl.colcodes, st);
   rc = rc ? rc : upload(&B->d_ptb, pl.ptb, st);
   rc = rc ? rc : upload(&B->d_pairs, B->pairs_h, st);
   rc = rc ? rc : upload(&B->d_fo_init, B->fo_init, st);
+  {
+    std::vector<int32_t> order(n);
+    for (int32_t k = 0; k < n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+      return B->hp[x].la + B->hp[x].lb > B->hp[y].la + B->hp[y].lb;
+    });
+    rc = rc ? rc : upload(&B->d_order, order, st);
+  }
   rc = rc ? rc : dalloc(&B->d_ring, pl.ring_elems);
   rc = rc ? rc : dalloc(&B->d_tb, pl.tb_bytes);
   rc = rc ? rc : dalloc(&B->d_fo, (size_t)std::max(n, 1));
   rc = rc ? rc : dalloc(&B->d_res, (size_t)std::max(n, 1));
   rc = rc ? rc : dalloc(&B->d_cig, B->cig_bytes);
-  rc = rc ? rc : dalloc(&B->d_ctr, 4);
+  rc = rc ? rc : dalloc(&B->d_ctr, 8);
   for (auto &e : B->ev)
     if (!rc && hipEventCreate(&e) != hipSuccess) rc = TA_E_DEVICE;
   if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = TA_E_DEVICE;
@@ -507,7 +555,9 @@ int ta_batch_info(ta_batch *B, int64_t *cells, int64_t *tb_bytes, int32_t *tasks
   if (!B) return TA_E_INVALID;
   if (cells) *cells = B->cells;
   if (tb_bytes) *tb_bytes = (int64_t)B->plan.tb_bytes;
-  if (tasks) *tasks = (int32_t)(B->plan.tasks2.size() + B->plan.tasks1.size());
+  if (tasks)
+    *tasks = (int32_t)(B->plan.tasks2[0].size() + B->plan.tasks2[1].size() + B->plan.tasks1[0].size() +
+                       B->plan.tasks1[1].size());
   return TA_OK;
 }
 
@@ -516,24 +566,27 @@ int ta_batch_info(ta_batch *B, int64_t *cells, int64_t *tb_bytes, int32_t *tasks
 namespace {
 
 template <int P>
-int launch_fill(ta_ctx *ctx, ta_batch *B, const Task *d_tasks, int ntasks, uint16_t *rows, uint16_t *cols,
-                uint32_t *ring, uint8_t *tb, int32_t *ctr, int max_groups) {
-  if (ntasks == 0) return TA_OK;
+int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntasks[2], uint16_t *rows,
+                uint16_t *cols, uint32_t *ring, uint8_t *tb, int32_t *ctr, int max_groups) {
+  if (ntasks[0] + ntasks[1] == 0) return TA_OK;
   FillArgs a;
-  a.tasks = d_tasks; a.ntasks = ntasks; a.ctr = ctr;
+  a.tasks[0] = d_tasks[0]; a.tasks[1] = d_tasks[1];
+  a.ntasks[0] = ntasks[0]; a.ntasks[1] = ntasks[1];
+  a.ctr = ctr;
   a.table = B->d_table; a.Kp = B->Kp;
   a.rowcodes = rows; a.colcodes = cols; a.ring = ring; a.tb = tb; a.out = B->d_fo;
   a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.sat_limit = 32767 - 2 * B->alph.maxS;
   const int lds = fill_lds_bytes(P, B->Kp);
   const int per_cu = std::max(1, std::min(8, (160 * 1024) / lds));
-  int groups = (ntasks + 3) / 4;
+  int groups = ntasks[0] + (ntasks[1] + 3) / 4;
   if (max_groups > 0) groups = std::min(groups, max_groups);
   groups = std::max(1, std::min(groups, ctx->cus * per_cu));
   const bool local = B->p.mode == TA_LOCAL;
   void (*k)(FillArgs) = nullptr;
   if (local) k = B->trace ? fill_kernel<P, true, true> : fill_kernel<P, true, false>;
   else k = B->trace ? fill_kernel<P, false, true> : fill_kernel<P, false, false>;
+  (void)hipGetLastError();  // drop an earlier, already-handled error
   hipLaunchKernelGGL(k, dim3(groups), dim3(64), lds, ctx->stream, a);
   HIPCHK(hipGetLastError());
   return TA_OK;
@@ -543,12 +596,18 @@ int launch_traceback(ta_ctx *ctx, ta_batch *B, const PairDev *pairs, const PairT
                      int first, int count) {
   if (count <= 0) return TA_OK;
   TbArgs a;
-  a.pairs = pairs + first; a.npairs = count; a.fo = B->d_fo + first; a.ptb = ptb + first;
+  const bool whole = first == 0 && count == B->n;
+  a.pairs = whole ? pairs : pairs + first; a.npairs = count;
+  a.order = whole ? B->d_order : nullptr;
+  a.fo = whole ? B->d_fo : B->d_fo + first;
+  a.ptb = whole ? ptb : ptb + first;
   a.codes = B->d_codes; a.table = B->d_table; a.Kp = B->Kp; a.tb = tb; a.cig = B->d_cig;
-  a.res = B->d_res + first;
+  a.res = whole ? B->d_res : B->d_res + first;
   a.local = B->p.mode == TA_LOCAL; a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.do_cigar = B->p.traceback;
-  hipLaunchKernelGGL(traceback_kernel, dim3((count + 63) / 64), dim3(64), 0, ctx->stream, a);
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(traceback_kernel, dim3((count + TB_WALKERS - 1) / TB_WALKERS), dim3(TB_WALKERS), 0,
+                     ctx->stream, a);
   HIPCHK(hipGetLastError());
   return TA_OK;
 }
@@ -566,12 +625,15 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   Plan pl;
   pl.ptb = B->plan.ptb;
   for (int k : bad) {
-    add_task(pl, pl.tasks1, 1, B->trace, B->hp, B->codes, k, k, true, B->Kp);
-    pl.tasks1.back().flags_or = TA_F_RERUN;
+    const int GG = pick_gg(B->hp[k].la, B->hp[k].lb, 0.0);
+    add_task(pl, pl.tasks1[GG == GWAVE ? 0 : 1], 1, GG, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+    pl.tasks1[GG == GWAVE ? 0 : 1].back().flags_or = TA_F_RERUN;
   }
+  std::vector<Task> all(pl.tasks1[0]);
+  all.insert(all.end(), pl.tasks1[1].begin(), pl.tasks1[1].end());
   hipStream_t st = ctx->stream;
   int rc = TA_OK;
-  rc = rc ? rc : upload(&B->d_tasks_r, pl.tasks1, st);
+  rc = rc ? rc : upload(&B->d_tasks_r, all, st);
   rc = rc ? rc : upload(&B->d_rowcodes_r, pl.rowcodes, st);
   rc = rc ? rc : upload(&B->d_colcodes_r, pl.colcodes, st);
   rc = rc ? rc : dalloc(&B->d_ring_r, pl.ring_elems);
@@ -586,9 +648,10 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
     ptb[k].tb_off += (uint64_t)(B->d_tb_r - B->d_tb);
   }
   HIPCHK(hipMemcpyAsync(B->d_ptb, ptb.data(), sizeof(PairTB) * B->n, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(B->d_ctr + 2, 0, sizeof(int32_t), st));
-  return launch_fill<1>(ctx, B, B->d_tasks_r, (int)pl.tasks1.size(), B->d_rowcodes_r, B->d_colcodes_r,
-                        B->d_ring_r, B->d_tb_r, B->d_ctr + 2, 0);
+  HIPCHK(hipMemsetAsync(B->d_ctr + 4, 0, 2 * sizeof(int32_t), st));
+  Task *const tk[2] = {B->d_tasks_r, B->d_tasks_r + pl.tasks1[0].size()};
+  const int nt[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size()};
+  return launch_fill<1>(ctx, B, tk, nt, B->d_rowcodes_r, B->d_colcodes_r, B->d_ring_r, B->d_tb_r, B->d_ctr + 4, 0);
 }
 
 }  // namespace
@@ -602,11 +665,11 @@ int ta_batch_run(ta_ctx *ctx, ta_batch *B) {
     HIPCHK(hipMemcpyAsync(B->d_fo, B->d_fo_init, sizeof(FillOut) * B->n, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemsetAsync(B->d_ctr, 0, 4 * sizeof(int32_t), st));
   HIPCHK(hipEventRecord(B->ev[0], st));
-  int rc = launch_fill<2>(ctx, B, B->d_tasks2, (int)pl.tasks2.size(), B->d_rowcodes, B->d_colcodes, B->d_ring,
-                          B->d_tb, B->d_ctr, 0);
-  if (!rc)
-    rc = launch_fill<1>(ctx, B, B->d_tasks1, (int)pl.tasks1.size(), B->d_rowcodes, B->d_colcodes, B->d_ring,
-                        B->d_tb, B->d_ctr + 1, 0);
+  const int n2[2] = {(int)pl.tasks2[0].size(), (int)pl.tasks2[1].size()};
+  const int n1[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size()};
+  int rc = launch_fill<2>(ctx, B, B->d_tasks2, n2, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb, B->d_ctr, 0);
+  if (!rc) rc = launch_fill<1>(ctx, B, B->d_tasks1, n1, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb,
+                               B->d_ctr + 2, 0);
   if (rc) return rc;
   HIPCHK(hipEventRecord(B->ev[1], st));
   if (B->may_saturate) {
@@ -640,14 +703,34 @@ int ta_batch_timings(ta_batch *B, double *ms, int32_t cap) {
   return k;
 }
 
+// Binary runs ((len << 2) | op, op 0 M / 1 D / 2 I) -> CIGAR text.
+namespace {
+int64_t format_runs(const uint32_t *runs, int n, char *out) {
+  static const char ops[4] = {'M', 'D', 'I', '?'};
+  int64_t pos = 0;
+  char tmp[12];
+  for (int k = 0; k < n; ++k) {
+    uint32_t len = runs[k] >> 2;
+    int d = 0;
+    do { tmp[d++] = (char)('0' + len % 10); len /= 10; } while (len);
+    if (out)
+      while (d) out[pos++] = tmp[--d];
+    else
+      pos += d;
+    if (out) out[pos] = ops[runs[k] & 3];
+    ++pos;
+  }
+  return pos;
+}
+}  // namespace
+
 int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, int64_t cap, int64_t *used) {
   if (!ctx || !B || (B->n && !out)) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
   std::vector<ResultDev> res(B->n);
-  std::vector<char> cig(B->cig_bytes);
+  std::vector<char> cig(B->p.traceback ? B->cig_bytes : 0);
   if (B->n) HIPCHK(hipMemcpyAsync(res.data(), B->d_res, sizeof(ResultDev) * B->n, hipMemcpyDeviceToHost, ctx->stream));
-  if (B->cig_bytes && B->p.traceback)
-    HIPCHK(hipMemcpyAsync(cig.data(), B->d_cig, B->cig_bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (!cig.empty()) HIPCHK(hipMemcpyAsync(cig.data(), B->d_cig, B->cig_bytes, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   int64_t pos = 0;
   int rc = TA_OK;
@@ -656,11 +739,15 @@ int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, 
     ta_result &o = out[k];
     o.score = r.score; o.a_begin = r.a_begin; o.a_end = r.a_end; o.b_begin = r.b_begin; o.b_end = r.b_end;
     o.flags = r.flags;
-    o.cigar_len = B->p.traceback ? r.cigar_len : 0;
     o.cigar_off = pos;
-    if (o.cigar_len) {
-      if (cigar_buf && pos + o.cigar_len <= cap) memcpy(cigar_buf + pos, cig.data() + r.cigar_off, o.cigar_len);
+    o.cigar_len = 0;
+    if (B->p.traceback && r.cigar_len > 0) {
+      const uint32_t *runs = (const uint32_t *)(cig.data() + r.cigar_off);
+      const int nr = r.cigar_len / 4;
+      const int64_t need = format_runs(runs, nr, nullptr);
+      if (cigar_buf && pos + need <= cap) format_runs(runs, nr, cigar_buf + pos);
       else rc = TA_E_SPACE;
+      o.cigar_len = (int32_t)need;
     }
     pos += o.cigar_len;
   }
@@ -690,20 +777,21 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
   hipStream_t st = ctx->stream;
   if (!B->plain_ready) {
     Plan pl;
-    pl.ptb.assign(B->n, PairTB{0, 0, 0, 0, 0});
+    pl.ptb.assign(B->n, PairTB{0, 0, 0, 0, 0, 0, 0});
     uint64_t max_tb = 0, max_ring = 0;
-    std::vector<uint64_t> tb_at(B->n, 0), ring_at(B->n, 0);
+    std::vector<Task> tasks;
     for (int k = 0; k < B->n; ++k) {
       const HostPair &h = B->hp[k];
       const int P = h.path == 1 ? 1 : 2;
       const uint64_t tb0 = pl.tb_bytes, r0 = pl.ring_elems;
       if (h.path == 0) {
-        pl.tasks2.push_back(Task{});
+        tasks.push_back(Task{});
         continue;
       }
-      add_task(pl, pl.tasks2, P, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+      const int GG = pick_gg(h.la, h.lb, 0.0);
+      add_task(pl, tasks, P, GG, B->trace, B->hp, B->codes, k, k, true, B->Kp);
       // Every plain launch reuses the same scratch: rebase to 0.
-      Task &T = pl.tasks2.back();
+      Task &T = tasks.back();
       max_tb = std::max(max_tb, pl.tb_bytes - tb0);
       max_ring = std::max(max_ring, pl.ring_elems - r0);
       T.tb_off = 0; T.ring_off = 0;
@@ -711,14 +799,14 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
       pl.tb_bytes = tb0; pl.ring_elems = r0;
     }
     int rc = TA_OK;
-    rc = rc ? rc : upload(&B->d_plain_tasks, pl.tasks2, st);
+    rc = rc ? rc : upload(&B->d_plain_tasks, tasks, st);
     rc = rc ? rc : upload(&B->d_prow, pl.rowcodes, st);
     rc = rc ? rc : upload(&B->d_pcol, pl.colcodes, st);
     rc = rc ? rc : upload(&B->d_pptb, pl.ptb, st);
     rc = rc ? rc : dalloc(&B->d_pring, max_ring);
     rc = rc ? rc : dalloc(&B->d_ptbuf, max_tb);
     if (rc) return rc;
-    B->plain_tasks = pl.tasks2;
+    B->plain_tasks = tasks;
     B->plain_ready = true;
   }
   HIPCHK(hipEventRecord(B->ev[0], st));
@@ -727,12 +815,12 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
   for (int k = first; k < first + count; ++k) {
     const HostPair &h = B->hp[k];
     if (h.path) {
-      HIPCHK(hipMemsetAsync(B->d_ctr, 0, sizeof(int32_t), st));
-      int rc = h.path == 1
-                   ? launch_fill<1>(ctx, B, B->d_plain_tasks + k, 1, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf,
-                                    B->d_ctr, 1)
-                   : launch_fill<2>(ctx, B, B->d_plain_tasks + k, 1, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf,
-                                    B->d_ctr, 1);
+      HIPCHK(hipMemsetAsync(B->d_ctr + 6, 0, 2 * sizeof(int32_t), st));
+      const bool wave = B->plain_tasks[k].S && pick_gg(h.la, h.lb, 0.0) == GWAVE;
+      Task *const tk[2] = {B->d_plain_tasks + k, B->d_plain_tasks + k};
+      const int nt[2] = {wave ? 1 : 0, wave ? 0 : 1};
+      int rc = h.path == 1 ? launch_fill<1>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1)
+                           : launch_fill<2>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1);
       if (rc) return rc;
     }
     int rc = launch_traceback(ctx, B, B->d_pairs, B->d_pptb, B->d_ptbuf, k, 1);

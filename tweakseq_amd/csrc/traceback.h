@@ -7,6 +7,16 @@
 // row 0 / column 0; local pairs stop on entering a cell whose H is 0, which
 // the walker knows by carrying the score backwards (H(i-1,j-1) = H - S,
 // gap open / extend undone likewise), so no stop code is stored.
+//
+// Memory: the fill stores direction planes lane-major ([strip][lane][block]
+// of 8 steps).  While the walk stays in one lane (8 rows) it needs at most
+// two consecutive blocks, so each lane visit is one 128-byte window, kept in
+// a padded per-thread LDS slice (dynamic indexing stays out of scratch) and
+// usually prefetched while the previous lane is walked.  Residue codes come
+// through 16-byte register windows read backwards.  One iteration emits one
+// CIGAR op without branching on the state; runs leave as binary dwords.  The
+// walk is latency-bound, so a wave carries few walkers (more waves per SIMD
+// overlap their loads); walkers run longest-path first.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -17,10 +27,11 @@ namespace ta {
 struct TbArgs {
   const PairDev *pairs;
   int32_t npairs;
+  const int32_t *order;  // walker -> pair (longest walks first); may be null
   const FillOut *fo;
   const PairTB *ptb;
-  const uint8_t *codes;
-  const int8_t *table;  // Kp x Kp
+  const uint8_t *codes;  // sequences start 16-byte aligned
+  const int8_t *table;   // Kp x Kp
   int32_t Kp;
   const uint8_t *tb;
   char *cig;
@@ -29,74 +40,192 @@ struct TbArgs {
   int32_t do_cigar;
 };
 
-// 4 direction bits of cell (i, j) (1-based) of a pair: bit0 E-extend,
-// bit1 F-extend, bit2 not-diag, bit3 F-rather-than-E.
-struct TbReader {
-  const uint8_t *tb;
-  PairTB d;
-  uint64_t cached = ~0ull;
-  uint4 blk[4];
-
-  __device__ __forceinline__ int get(int i, int j) {
-    const int r0 = i - 1;
-    const int s = r0 / STRIP, w = r0 - s * STRIP, l = w / R, k = w - l * R;
-    const int t = j - 1 + d.off + l, bi = t >> 3, q = t & 7;
-    const int esz = d.P == 2 ? 64 : 32;
-    const uint64_t base = d.tb_off + (((uint64_t)s * d.NB + bi) * G + l) * esz;
-    if (base != cached) {
-      const uint4 *src = (const uint4 *)(tb + base);
-      blk[0] = src[0];
-      blk[1] = src[1];
-      if (d.P == 2) { blk[2] = src[2]; blk[3] = src[3]; }
-      cached = base;
-    }
-    const uint32_t *w32 = (const uint32_t *)blk;
-    if (d.P == 2) {
-      const uint32_t xc = w32[2 * q], yc = w32[2 * q + 1];
-      const int sh = 8 * d.half + k;
-      return (int)(((xc >> sh) & 1) | (((xc >> (16 + sh)) & 1) << 1) | (((yc >> sh) & 1) << 2) |
-                   (((yc >> (16 + sh)) & 1) << 3));
-    }
-    const uint32_t acc = w32[q];
-    const int p = 31 - 4 * k;  // pushes e, f, d, x of row k land at p, p-1, p-2, p-3
-    return (int)(((acc >> p) & 1) | (((acc >> (p - 1)) & 1) << 1) | (((acc >> (p - 2)) & 1) << 2) |
-                 (((acc >> (p - 3)) & 1) << 3));
-  }
-};
-
-struct CigarWriter {
-  char *base;   // start of the arena
-  char *lo;     // start of this pair's region (writes never go below it)
-  char *wp;     // next byte to write is wp[-1]
-  bool overflow = false;
-  char op = 0;
+// Run-length CIGAR written backwards from the end of the pair's region as
+// one dword per run, (length << 2) | op with op 0 M, 1 D, 2 I (the host turns
+// runs into text when it copies results out).
+struct RunWriter {
+  uint32_t *lo = nullptr;  // start of the pair's region
+  uint32_t *wp = nullptr;  // runs live in [wp, end)
+  int op = -1;
   int len = 0;
-  bool on = true;  // false: walk for coordinates only
+  bool on = true;
+  bool overflow = false;
 
-  __device__ __forceinline__ void flush() {
+  __device__ __forceinline__ void store() {
     if (!len || !on) return;
-    int digits = 1;
-    for (int v = len; v >= 10; v /= 10) ++digits;
-    if (wp - lo < digits + 1) {
-      overflow = true;
-      len = 0;
-      return;
-    }
-    *--wp = op;
-    int v = len;
-    do { *--wp = (char)('0' + v % 10); v /= 10; } while (v);
-    len = 0;
+    if (wp <= lo) { overflow = true; return; }
+    *--wp = ((uint32_t)len << 2) | (uint32_t)op;
   }
-  __device__ __forceinline__ void emit(char o, int k = 1) {
-    if (k <= 0) return;
-    if (o != op) { flush(); op = o; }
+  __device__ __forceinline__ void emit(int o, int k) {
+    if (o != op) { store(); op = o; len = 0; }
     len += k;
   }
+  __device__ __forceinline__ void finish() { store(); len = 0; }
 };
 
-__global__ __launch_bounds__(64) void traceback_kernel(TbArgs A) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A.npairs) return;
+// Byte o (0..31) of a 32-byte window held as two uint4.
+__device__ __forceinline__ int win32_byte(uint4 lo, uint4 hi, int o) {
+  const bool up = (o & 16) != 0, mid = (o & 8) != 0;
+  const uint32_t w0 = mid ? (up ? hi.z : lo.z) : (up ? hi.x : lo.x);
+  const uint32_t w1 = mid ? (up ? hi.w : lo.w) : (up ? hi.y : lo.y);
+  const uint64_t v = ((uint64_t)w1 << 32) | w0;
+  return (int)((v >> ((o & 7) * 8)) & 0xff);
+}
+
+constexpr int WSTRIDE = 68;     // dwords per walker LDS slice: two 32-dword windows + pad (16-byte aligned)
+constexpr int TB_WALKERS = 16;  // walkers (threads) per workgroup = per wave
+constexpr int EPOCH = 8;        // steps per epoch (a walk moves at most 8 rows / columns)
+
+// One walk.  Epochs: load everything the next EPOCH steps can touch -- the
+// current lane's two-block window, the window of the lane above, and 32-byte
+// windows of both sequences -- wait once, then step with nothing in flight.
+template <int P>
+__device__ __forceinline__ void walk(const TbArgs &A, int p, const PairDev &pd, const FillOut &f, ResultDev &r,
+                                     bool local, const int8_t *stab, uint32_t *slice) {
+  const PairTB d = A.ptb[p];
+  RunWriter cw;
+  cw.on = A.do_cigar != 0;
+  if (cw.on) {
+    cw.lo = (uint32_t *)(A.cig + pd.cig_off);
+    cw.wp = (uint32_t *)(A.cig + pd.cig_off + pd.cig_cap);
+  }
+  const uint8_t *pa = A.codes + pd.a_off;
+  const uint8_t *pb = A.codes + pd.b_off;
+  const int Kp = A.Kp;
+  const int GG = d.GG > 0 ? d.GG : G;
+  const int stsh = (GG == 64 ? 6 : 4) + 3;  // log2(GG * R)
+  constexpr bool p2 = P == 2;
+  constexpr int esz = p2 ? 64 : 32;
+  constexpr int NU = p2 ? 8 : 4;  // uint4 per two-block window
+  const int hsh = 8 * d.half;
+  const int NB = d.NB;
+  const uint8_t *tbase = A.tb + d.tb_off;
+  const int open = A.open, ext = A.ext;
+
+  int i = local ? f.end_i : pd.la;
+  int j = local ? f.end_j : pd.lb;
+  int state = 0;  // 0 H, 1 E, 2 F
+  int h = f.score;
+  int s = 0, l = 0, k = 0;  // strip, lane, row-in-lane of row i
+  if (i > 0) {
+    const int r0 = i - 1;
+    s = r0 >> stsh;
+    l = (r0 >> 3) & (GG - 1);
+    k = r0 & 7;
+  }
+  auto first_block = [&](int t) {
+    int ff = (t >> 3) - 1;
+    ff = ff > NB - 2 ? NB - 2 : ff;
+    return ff < 0 ? 0 : ff;
+  };
+
+  bool done = false;
+  int guard = 2 * (pd.la + pd.lb) + 2;
+  while (!done && guard > 0) {
+    if (!local) {
+      if (i == 0) { if (j) cw.emit(1, j); break; }
+      if (j == 0) { cw.emit(2, i); break; }
+    } else if (i <= 0 || j <= 0) {
+      r.flags |= 0x100;
+      break;
+    }
+    // ---- epoch loads
+    const int t0 = j - 1 + d.off + l;
+    const int fA = first_block(t0);
+    int sB = s, lB = l - 1, tB = t0 - 1;
+    if (lB < 0) { lB = GG - 1; sB = s - 1; tB += GG; }
+    const bool hasB = sB >= 0;
+    const int fB = hasB ? first_block(tB) : 0;
+    const int laneA = s * GG + l, laneB = sB * GG + lB;
+    {
+      const uint4 *src = (const uint4 *)(tbase + (((uint64_t)s * GG + l) * NB + fA) * esz);
+      uint4 wa[NU];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) wa[u] = src[u];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) *(uint4 *)(slice + 4 * u) = wa[u];
+    }
+    if (hasB) {
+      const uint4 *src = (const uint4 *)(tbase + (((uint64_t)sB * GG + lB) * NB + fB) * esz);
+      uint4 wb[NU];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) wb[u] = src[u];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) *(uint4 *)(slice + 32 + 4 * u) = wb[u];
+    }
+    uint4 ra0 = {0, 0, 0, 0}, ra1 = ra0, rb0 = ra0, rb1 = ra0;
+    const int abase = i - 1 >= 0 ? ((i - 1 - EPOCH) < 0 ? 0 : (i - 1 - EPOCH)) & ~15 : 0;
+    const int bbase = j - 1 >= 0 ? ((j - 1 - EPOCH) < 0 ? 0 : (j - 1 - EPOCH)) & ~15 : 0;
+    if (local) {
+      ra0 = *(const uint4 *)(pa + abase);
+      ra1 = *(const uint4 *)(pa + abase + 16);
+      rb0 = *(const uint4 *)(pb + bbase);
+      rb1 = *(const uint4 *)(pb + bbase + 16);
+    }
+    // ---- up to EPOCH steps inside the two windows
+    for (int e = 0; e < EPOCH && guard > 0; ++e, --guard) {
+      if (!local && (i == 0 || j == 0)) break;  // boundary handled by the epoch head
+      const int t = j - 1 + d.off + l;
+      const int lane_now = s * GG + l;
+      int wsel, wi;
+      if (lane_now == laneA && t >= 8 * fA && t < 8 * fA + 16) { wsel = 0; wi = t - 8 * fA; }
+      else if (hasB && lane_now == laneB && t >= 8 * fB && t < 8 * fB + 16) { wsel = 32; wi = t - 8 * fB; }
+      else break;  // left both windows: next epoch
+      int code;
+      if constexpr (p2) {
+        const uint2 v = *(const uint2 *)(slice + wsel + (wi >> 3) * 16 + (wi & 7) * 2);
+        const int sh = hsh + k;
+        code = (int)(((v.x >> sh) & 1) | (((v.x >> (16 + sh)) & 1) << 1) | (((v.y >> sh) & 1) << 2) |
+                     (((v.y >> (16 + sh)) & 1) << 3));
+      } else {
+        const uint32_t acc = slice[wsel + (wi >> 3) * 8 + (wi & 7)];
+        const int pbit = 31 - 4 * k;  // pushes e, f, d, x of row k land at pbit .. pbit-3
+        code = (int)(((acc >> pbit) & 1) | (((acc >> (pbit - 1)) & 1) << 1) | (((acc >> (pbit - 2)) & 1) << 2) |
+                     (((acc >> (pbit - 3)) & 1) << 3));
+      }
+      // Op of this step: M (0), D (1) or I (2); from state H a gap source is
+      // taken at once with the same cell's extend bit.
+      const int g = state ? state : ((code & 4) ? ((code & 8) ? 2 : 1) : 0);
+      const int xb = g == 1 ? (code & 1) : (g == 2 ? ((code >> 1) & 1) : 0);
+      if (local) {
+        const int sc = g == 0 ? (int)stab[win32_byte(ra0, ra1, i - 1 - abase) * Kp + win32_byte(rb0, rb1, j - 1 - bbase)]
+                              : (xb ? ext : open);
+        h -= sc;
+      }
+      cw.emit(g, 1);
+      state = xb ? g : 0;
+      if (g != 2) --j;
+      if (g != 1) {
+        --i;
+        if (k > 0) --k;
+        else {
+          k = 7;
+          if (l > 0) --l;
+          else { l = GG - 1; --s; }
+        }
+        if (local && g == 0 && (h == 0 || i == 0 || j == 0)) { done = true; break; }
+      }
+    }
+  }
+  cw.finish();
+  if (guard <= 0 || cw.overflow) r.flags |= 0x100;
+  if (local) { r.a_begin = i; r.b_begin = j; }
+  if (cw.on) {
+    r.cigar_off = (int64_t)((char *)cw.wp - A.cig);
+    r.cigar_len = (int32_t)(A.cig + pd.cig_off + pd.cig_cap - (char *)cw.wp);  // bytes of runs
+  }
+  A.res[p] = r;
+}
+
+__global__ __launch_bounds__(TB_WALKERS) void traceback_kernel(TbArgs A) {
+  __shared__ uint32_t wlds[TB_WALKERS * WSTRIDE];
+  __shared__ int8_t stab[1024];
+  for (int x = threadIdx.x; x < A.Kp * A.Kp; x += TB_WALKERS) stab[x] = A.table[x];
+  __syncthreads();
+
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.npairs) return;
+  const int p = A.order ? A.order[w] : w;
   const PairDev pd = A.pairs[p];
   const FillOut f = A.fo[p];
   ResultDev r;
@@ -104,74 +233,23 @@ __global__ __launch_bounds__(64) void traceback_kernel(TbArgs A) {
   r.flags = f.flags;
   r.cigar_off = (int64_t)(pd.cig_off + pd.cig_cap);
   r.cigar_len = 0;
-  const bool empty = A.local && f.score <= 0;
-  if (A.local) {
+  const bool local = A.local != 0;
+  const bool empty = local && f.score <= 0;
+  if (local) {
     r.a_begin = r.a_end = r.b_begin = r.b_end = -1;
     if (!empty) { r.a_end = f.end_i - 1; r.b_end = f.end_j - 1; }
   } else {
     r.a_begin = 0; r.a_end = pd.la - 1; r.b_begin = 0; r.b_end = pd.lb - 1;
   }
-  if (empty || (!A.local && !A.do_cigar)) {
+  if (empty || (!local && !A.do_cigar)) {
     if (empty) r.flags |= 8;
     A.res[p] = r;
     return;
   }
 
-  TbReader rd;
-  rd.tb = A.tb;
-  rd.d = A.ptb[p];
-  CigarWriter cw;
-  cw.base = A.cig;
-  cw.on = A.do_cigar != 0;
-  cw.wp = cw.on ? A.cig + pd.cig_off + pd.cig_cap : nullptr;
-  cw.lo = cw.on ? A.cig + pd.cig_off : nullptr;
-  const uint8_t *a = A.codes + pd.a_off;
-  const uint8_t *b = A.codes + pd.b_off;
-
-  int i = A.local ? f.end_i : pd.la;
-  int j = A.local ? f.end_j : pd.lb;
-  int state = 0;  // 0 H, 1 E, 2 F
-  int h = f.score;
-  // Every iteration moves or changes state, so 2 * (la + lb) + 2 bound it;
-  // a walk that leaves the matrix is reported instead of read out of range.
-  int guard = 2 * (pd.la + pd.lb) + 2;
-  for (; guard > 0; --guard) {
-    if (!A.local) {
-      if (i == 0) { cw.emit('D', j); break; }
-      if (j == 0) { cw.emit('I', i); break; }
-    } else if (i <= 0 || j <= 0) {
-      r.flags |= 0x100;
-      break;
-    }
-    const int code = rd.get(i, j);
-    if (state == 0) {
-      if (!(code & 4)) {
-        cw.emit('M');
-        h -= A.table[a[i - 1] * A.Kp + b[j - 1]];
-        --i;
-        --j;
-        if (A.local && (h == 0 || i == 0 || j == 0)) break;
-      } else {
-        state = (code & 8) ? 2 : 1;
-      }
-    } else if (state == 1) {
-      cw.emit('D');
-      if (code & 1) { h -= A.ext; } else { h -= A.open; state = 0; }
-      --j;
-    } else {
-      cw.emit('I');
-      if (code & 2) { h -= A.ext; } else { h -= A.open; state = 0; }
-      --i;
-    }
-  }
-  cw.flush();
-  if (guard == 0 || cw.overflow) r.flags |= 0x100;
-  if (A.local) { r.a_begin = i; r.b_begin = j; }
-  if (cw.on) {
-    r.cigar_off = (int64_t)(cw.wp - A.cig);
-    r.cigar_len = (int32_t)(A.cig + pd.cig_off + pd.cig_cap - cw.wp);
-  }
-  A.res[p] = r;
+  uint32_t *slice = wlds + threadIdx.x * WSTRIDE;
+  if (A.ptb[p].P == 2) walk<2>(A, p, pd, f, r, local, stab, slice);
+  else walk<1>(A, p, pd, f, r, local, stab, slice);
 }
 
 }  // namespace ta
