@@ -1,0 +1,109 @@
+"""Serving: continuous batching returns, for every request, exactly what the
+same request gives when aligned alone; cancellation and admission limits."""
+import random
+import time
+
+import pytest
+
+import tweakseq_amd as ta
+
+pytestmark = pytest.mark.gpu
+
+AA = "ARNDCQEGHILKMFPSTWYV"
+
+
+def rand_prot(rng, n):
+    return "".join(rng.choice(AA) for _ in range(n))
+
+
+def make_requests(seed, nreq, maxlen=700):
+    rng = random.Random(seed)
+    reqs = []
+    for _ in range(nreq):
+        k = rng.randint(1, 12)
+        mode = rng.choice([ta.GLOBAL, ta.LOCAL])
+        pairs = []
+        for _ in range(k):
+            a = rand_prot(rng, rng.randint(1, maxlen))
+            b = a[: len(a) // 2] + rand_prot(rng, rng.randint(0, maxlen // 2)) if rng.random() < 0.5 else \
+                rand_prot(rng, rng.randint(1, maxlen))
+            pairs.append((a, b))
+        reqs.append((ta.engine.make_params(mode=mode, gap_open=-11 if mode else -10), pairs))
+    return reqs
+
+
+def wait_all(eng, tickets, timeout=120):
+    out = {}
+    t0 = time.time()
+    while len(out) < len(tickets):
+        for key, (t, n) in tickets.items():
+            if key in out:
+                continue
+            r = eng.poll(t, n)
+            if r is not None:
+                out[key] = r
+        assert time.time() - t0 < timeout, "serving did not finish"
+        time.sleep(0.001)
+    return out
+
+
+def test_results_identical_alone_and_in_the_mix():
+    eng = ta.Engine(0)
+    try:
+        eng.serve_config(64 << 30, 4)
+        reqs = make_requests(1, 60)
+        tickets = {q: eng.submit(pairs, params) for q, (params, pairs) in enumerate(reqs)}
+        mixed = wait_all(eng, tickets)
+        for q, (params, pairs) in enumerate(reqs):
+            alone = eng.align(pairs, params)
+            assert mixed[q] == alone, "request %d differs in the mix" % q
+    finally:
+        eng.close()
+
+
+def test_small_budget_serialises_but_completes():
+    eng = ta.Engine(0)
+    try:
+        reqs = make_requests(2, 12, maxlen=400)
+        # Budget of about one request at a time: admission must still drain the queue.
+        eng.serve_config(6 << 20, 1)
+        tickets = {q: eng.submit(pairs, params) for q, (params, pairs) in enumerate(reqs)}
+        done = wait_all(eng, tickets)
+        for q, (params, pairs) in enumerate(reqs):
+            assert done[q] == eng.align(pairs, params)
+    finally:
+        eng.close()
+
+
+def test_budget_rejects_oversized_request():
+    eng = ta.Engine(0)
+    try:
+        eng.serve_config(1 << 20, 2)
+        big = [(rand_prot(random.Random(3), 3000), rand_prot(random.Random(4), 3000))]
+        with pytest.raises(RuntimeError, match="budget"):
+            eng.submit(big, ta.engine.make_params(mode=ta.LOCAL))
+    finally:
+        eng.close()
+
+
+def test_cancel():
+    eng = ta.Engine(0)
+    try:
+        eng.serve_config(64 << 30, 1)
+        reqs = make_requests(5, 20, maxlen=900)
+        tickets = [eng.submit(pairs, params) for params, pairs in reqs]
+        # Cancel the later half; the rest must still complete exactly.
+        for t, _ in tickets[10:]:
+            assert eng.cancel(t) in (0, -8)
+        for q, (t, n) in enumerate(tickets[:10]):
+            r = None
+            t0 = time.time()
+            while r is None:
+                r = eng.poll(t, n)
+                assert time.time() - t0 < 60
+            assert r == eng.align(reqs[q][1], reqs[q][0])
+        for t, n in tickets[10:]:
+            with pytest.raises(RuntimeError):
+                eng.poll(t, n)
+    finally:
+        eng.close()

@@ -97,3 +97,13 @@ def test_full_workload_digests():
     for cfg, size in ((1, 0), (2, 0), (3, 0)):
         key = "config%d" % cfg
         assert workload_digest(ta.Workload(cfg, size=size)) == want[key], key
+
+
+def test_no_gpu_fails_loudly():
+    """Without a visible GPU the engine refuses to start (no CPU fallback)."""
+    import subprocess, sys
+    code = ("import tweakseq_amd as ta\n"
+            "try:\n    ta.Engine(0)\nexcept RuntimeError as e:\n    print('refused', e)\n")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT,
+                         env={**os.environ, "HIP_VISIBLE_DEVICES": "-1", "ROCR_VISIBLE_DEVICES": "-1"})
+    assert "refused" in out.stdout

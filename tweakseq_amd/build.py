@@ -51,17 +51,20 @@ def build_lib():
         os.remove(o)
 
 
-def build_cli():
-    src = os.path.join(CSRC, "cli.cpp")
-    if not os.path.exists(src):
-        return
-    run(["g++", "-O2", "-std=c++17", "-Wall", src, "-o", CLI, "-L" + HERE, "-ltweakalign",
-         "-Wl,-rpath,$ORIGIN"])
+def build_tools():
+    """Executables linked against libtweakalign.so (rpath = package dir)."""
+    tools = [("cli.cpp", CLI), ("serve_bench.cpp", os.path.join(HERE, "tweakalign-serve-bench"))]
+    for src, out in tools:
+        src = os.path.join(CSRC, src)
+        if not os.path.exists(src):
+            continue
+        run(["g++", "-O2", "-std=c++17", "-Wall", "-pthread", src, "-o", out, "-L" + HERE, "-ltweakalign",
+             "-Wl,-rpath,$ORIGIN"])
 
 
 def build():
     build_lib()
-    build_cli()
+    build_tools()
 
 
 if __name__ == "__main__":

@@ -118,15 +118,72 @@ std::vector<int8_t> padded_table(const Alphabet &A) {
     }                                                               \
   } while (0)
 
+// Device memory cache: blocks up to 256 MB are kept per power-of-two size
+// class and reused (the serving loop builds many small batches; hipMalloc and
+// hipFree per buffer would dominate them).  Larger blocks go straight to HIP.
+struct MemCache {
+  std::mutex mu;
+  std::map<std::pair<int, size_t>, std::vector<void *>> free;  // (device, class) -> blocks
+  std::unordered_map<void *, std::pair<int, size_t>> live;
+};
+MemCache &memcache() {
+  static MemCache *c = new MemCache();  // lives until the process ends
+  return *c;
+}
+constexpr size_t kCacheMax = 256ull << 20;
+
+size_t size_class(size_t n) {
+  size_t c = 256;
+  while (c < n) c <<= 1;
+  return c;
+}
+
+void *cache_alloc(size_t bytes) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  MemCache &mc = memcache();
+  const size_t cls = bytes <= kCacheMax ? size_class(bytes) : bytes;
+  if (cls <= kCacheMax) {
+    std::lock_guard<std::mutex> g(mc.mu);
+    auto &v = mc.free[{dev, cls}];
+    if (!v.empty()) {
+      void *p = v.back();
+      v.pop_back();
+      mc.live[p] = {dev, cls};
+      return p;
+    }
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, cls) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> g(mc.mu);
+  mc.live[p] = {dev, cls};
+  return p;
+}
+
+void cache_free(void *p) {
+  if (!p) return;
+  MemCache &mc = memcache();
+  std::pair<int, size_t> key{0, 0};
+  {
+    std::lock_guard<std::mutex> g(mc.mu);
+    auto it = mc.live.find(p);
+    if (it == mc.live.end()) return;
+    key = it->second;
+    mc.live.erase(it);
+    if (key.second <= kCacheMax) {
+      mc.free[key].push_back(p);
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
 template <typename T>
 int dalloc(T **p, size_t n) {
   *p = nullptr;
   if (n == 0) return TA_OK;
-  if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
-    *p = nullptr;
-    return TA_E_NOMEM;
-  }
-  return TA_OK;
+  *p = (T *)cache_alloc(n * sizeof(T));
+  return *p ? TA_OK : TA_E_NOMEM;
 }
 
 template <typename T>
@@ -139,20 +196,13 @@ int upload(T **dst, const std::vector<T> &src, hipStream_t st) {
 
 template <typename T>
 void dfree(T *&p) {
-  if (p) (void)hipFree((void *)p);
+  cache_free((void *)p);
   p = nullptr;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-struct ta_ctx {
-  int device = 0;
-  int cus = 256;
-  hipStream_t stream = nullptr;
-  Serve *serve = nullptr;
-};
-
 const char *ta_strerror(int code) {
   switch (code) {
     case TA_OK: return "ok";
@@ -298,6 +348,7 @@ int pick_gg(int mmax, int n, double mean_slot) {
 }  // namespace
 
 struct ta_batch {
+  hipStream_t stream = nullptr;  // the context stream, or a serving stream
   ta_params p{};
   int32_t n = 0;
   Alphabet alph;
@@ -377,6 +428,11 @@ void ta_batch_destroy(ta_batch *B) {
 }
 
 int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, ta_batch **out) {
+  return batch_create_on(ctx, p, pairs, n, ctx ? ctx->stream : nullptr, out);
+}
+
+int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, hipStream_t stream,
+                    ta_batch **out) {
   if (!ctx || !p || !out || n < 0 || (n > 0 && !pairs)) return TA_E_INVALID;
   *out = nullptr;
   if (p->mode != TA_GLOBAL && p->mode != TA_LOCAL) return TA_E_INVALID;
@@ -389,6 +445,7 @@ int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   HIPCHK(hipSetDevice(ctx->device));
 
   ta_batch *B = new ta_batch();
+  B->stream = stream;
   B->p = *p;
   B->n = n;
   B->alph = make_alphabet(p->alphabet, p->match, p->mismatch);
@@ -512,7 +569,7 @@ int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   }
 
   // Upload.
-  hipStream_t st = ctx->stream;
+  hipStream_t st = B->stream;
   std::vector<int8_t> table = padded_table(B->alph);
   int rc = TA_OK;
   rc = rc ? rc : upload(&B->d_codes, B->codes, st);
@@ -551,6 +608,15 @@ int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   return TA_OK;
 }
 
+int64_t batch_tb_estimate(const ta_params *p, const ta_pair *pairs, int32_t n) {
+  if (!p || (p->mode != TA_LOCAL && !p->traceback)) return 0;
+  int64_t t = 0;
+  for (int32_t k = 0; k < n; ++k)
+    if (pairs[k].la > 0 && pairs[k].lb > 0)
+      t += (int64_t)ceil_div(pairs[k].la, STRIP) * ceil_div(pairs[k].lb + G, 8) * 512;
+  return t;
+}
+
 int ta_batch_info(ta_batch *B, int64_t *cells, int64_t *tb_bytes, int32_t *tasks) {
   if (!B) return TA_E_INVALID;
   if (cells) *cells = B->cells;
@@ -587,7 +653,7 @@ int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntas
   if (local) k = B->trace ? fill_kernel<P, true, true> : fill_kernel<P, true, false>;
   else k = B->trace ? fill_kernel<P, false, true> : fill_kernel<P, false, false>;
   (void)hipGetLastError();  // drop an earlier, already-handled error
-  hipLaunchKernelGGL(k, dim3(groups), dim3(64), lds, ctx->stream, a);
+  hipLaunchKernelGGL(k, dim3(groups), dim3(64), lds, B->stream, a);
   HIPCHK(hipGetLastError());
   return TA_OK;
 }
@@ -607,7 +673,7 @@ int launch_traceback(ta_ctx *ctx, ta_batch *B, const PairDev *pairs, const PairT
   a.do_cigar = B->p.traceback;
   (void)hipGetLastError();
   hipLaunchKernelGGL(traceback_kernel, dim3((count + TB_WALKERS - 1) / TB_WALKERS), dim3(TB_WALKERS), 0,
-                     ctx->stream, a);
+                     B->stream, a);
   HIPCHK(hipGetLastError());
   return TA_OK;
 }
@@ -615,8 +681,8 @@ int launch_traceback(ta_ctx *ctx, ta_batch *B, const PairDev *pairs, const PairT
 // Pairs whose int16 pass saturated go through the int32 kernel.
 int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   std::vector<FillOut> fo(B->n);
-  HIPCHK(hipMemcpyAsync(fo.data(), B->d_fo, sizeof(FillOut) * B->n, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpyAsync(fo.data(), B->d_fo, sizeof(FillOut) * B->n, hipMemcpyDeviceToHost, B->stream));
+  HIPCHK(hipStreamSynchronize(B->stream));
   std::vector<int> bad;
   for (int k = 0; k < B->n; ++k)
     if (fo[k].flags & TA_F_RERUN) bad.push_back(k);
@@ -631,7 +697,7 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   }
   std::vector<Task> all(pl.tasks1[0]);
   all.insert(all.end(), pl.tasks1[1].begin(), pl.tasks1[1].end());
-  hipStream_t st = ctx->stream;
+  hipStream_t st = B->stream;
   int rc = TA_OK;
   rc = rc ? rc : upload(&B->d_tasks_r, all, st);
   rc = rc ? rc : upload(&B->d_rowcodes_r, pl.rowcodes, st);
@@ -659,7 +725,7 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
 int ta_batch_run(ta_ctx *ctx, ta_batch *B) {
   if (!ctx || !B) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
-  hipStream_t st = ctx->stream;
+  hipStream_t st = B->stream;
   const Plan &pl = B->plan;
   if (B->n > 0)
     HIPCHK(hipMemcpyAsync(B->d_fo, B->d_fo_init, sizeof(FillOut) * B->n, hipMemcpyDeviceToDevice, st));
@@ -687,7 +753,7 @@ int ta_batch_run(ta_ctx *ctx, ta_batch *B) {
 
 int ta_batch_wait(ta_ctx *ctx, ta_batch *B) {
   if (!ctx || !B) return TA_E_INVALID;
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipStreamSynchronize(B->stream));
   float t = 0;
   if (hipEventElapsedTime(&t, B->ev[0], B->ev[1]) == hipSuccess) B->ms[0] = t;
   if (hipEventElapsedTime(&t, B->ev[1], B->ev[2]) == hipSuccess) B->ms[1] = t;
@@ -729,9 +795,9 @@ int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, 
   HIPCHK(hipSetDevice(ctx->device));
   std::vector<ResultDev> res(B->n);
   std::vector<char> cig(B->p.traceback ? B->cig_bytes : 0);
-  if (B->n) HIPCHK(hipMemcpyAsync(res.data(), B->d_res, sizeof(ResultDev) * B->n, hipMemcpyDeviceToHost, ctx->stream));
-  if (!cig.empty()) HIPCHK(hipMemcpyAsync(cig.data(), B->d_cig, B->cig_bytes, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (B->n) HIPCHK(hipMemcpyAsync(res.data(), B->d_res, sizeof(ResultDev) * B->n, hipMemcpyDeviceToHost, B->stream));
+  if (!cig.empty()) HIPCHK(hipMemcpyAsync(cig.data(), B->d_cig, B->cig_bytes, hipMemcpyDeviceToHost, B->stream));
+  HIPCHK(hipStreamSynchronize(B->stream));
   int64_t pos = 0;
   int rc = TA_OK;
   for (int32_t k = 0; k < B->n; ++k) {
@@ -774,7 +840,7 @@ int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_
 int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
   if (!ctx || !B || first < 0 || count < 0 || first + count > B->n) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
-  hipStream_t st = ctx->stream;
+  hipStream_t st = B->stream;
   if (!B->plain_ready) {
     Plan pl;
     pl.ptb.assign(B->n, PairTB{0, 0, 0, 0, 0, 0, 0});

@@ -1,6 +1,24 @@
 // Internal interfaces between the batch runtime and the serving loop.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include "../../include/tweakalign.h"
 
 struct Serve;
+
+// ta_ctx internals shared with the serving loop.
+struct ta_ctx {
+  int device = 0;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+  Serve *serve = nullptr;
+};
+
+// ta_batch_create on a given stream (the serving loop gives each in-flight
+// batch its own).
+int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, hipStream_t stream,
+                    ta_batch **out);
+// Direction-plane bytes a batch of these pairs would allocate (admission).
+int64_t batch_tb_estimate(const ta_params *p, const ta_pair *pairs, int32_t n);
+// Stops the serving thread (called by ta_destroy).
 void serve_shutdown(ta_ctx *ctx);

@@ -229,7 +229,9 @@ class Engine:
         rc = lib().ta_poll(self.ctx, ticket, res, None, 0, ctypes.byref(used))
         if rc == E_PENDING:
             return None
-        if rc < 0 and rc != -3:
+        if rc == 0:  # delivered (no CIGAR text)
+            return _collect(res, ctypes.create_string_buffer(1), n, traceback)
+        if rc != -3:
             _check(rc, "ta_poll")
         cap = used.value + 1
         buf = ctypes.create_string_buffer(cap)
