@@ -1,0 +1,279 @@
+// tweakalign -- command-line drop-in for tweakseq's external aligner slot.
+//
+// tweakseq runs an aligner as a child process (tweakseq/UI/SeqEditMainWin.cpp
+// :1616-1663) with an argument list built by AlignmentTool::makeCommand, and
+// accepts the result when the process exits 0 and the output file exists
+// (:836-861).  This program honours all three argument styles the reference
+// builds, so its path can be set in any of the three tool slots:
+//   MUSCLE   -in FIN -out FOUT                       (Muscle.cpp:49-53)
+//   ClustalO --force -v --outfmt=fa --output-order=tree-order -i FIN -o FOUT
+//                                                    (ClustalO.cpp:48-52)
+//   MAFFT    --auto --thread -1 FIN   (alignment on stdout; MAFFT.cpp:49-53, :98)
+// and the version probes (-version / --version, Muscle.cpp:100-113,
+// MAFFT.cpp:103-116, ClustalO.cpp:100-111: the second token is the version).
+//
+// Input/output are FASTA as tweakseq reads and writes it (FASTAFile.cpp:71-187):
+// records start at '>' or ';', the header line is kept verbatim, sequence lines
+// are concatenated, output is wrapped at 80 columns.  Gaps ('-', '.') left by
+// an earlier alignment are removed first (Sequence::filter keeps them,
+// Sequence.cpp:57-69).  Two sequences: Gotoh global alignment on the GPU.
+// More: center-star multiple alignment -- all-vs-all global scores on the GPU
+// pick the sequence with the highest score sum, every other sequence is
+// aligned to it, and the pairwise alignments are merged on the center.
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/tweakalign.h"
+
+namespace {
+
+struct Record {
+  std::string header;  // verbatim, including '>' or ';'
+  std::string seq;
+};
+
+std::string trim(const std::string &s) {
+  size_t a = 0, b = s.size();
+  while (a < b && isspace((unsigned char)s[a])) ++a;
+  while (b > a && isspace((unsigned char)s[b - 1])) --b;
+  return s.substr(a, b - a);
+}
+
+// FASTAFile::read rules (FASTAFile.cpp:71-147).
+bool read_fasta(std::istream &in, std::vector<Record> &out) {
+  enum { SEEK, COMMENT, SEQ } st = SEEK;
+  std::string line;
+  while (std::getline(in, line)) {
+    const std::string s = trim(line);
+    if (s.empty()) continue;
+    const char c = s[0];
+    if (st == SEEK) {
+      if (c == ';' || c == '>') { out.push_back({s, ""}); st = COMMENT; }
+    } else if (st == COMMENT) {
+      if (c == ';') continue;
+      out.back().seq = s;
+      st = SEQ;
+    } else {
+      if (c == ';' || c == '>') { out.push_back({s, ""}); st = COMMENT; }
+      else out.back().seq += s;
+    }
+  }
+  return true;
+}
+
+// FASTAFile::write rules (FASTAFile.cpp:149-171).
+void write_fasta(std::ostream &o, const std::vector<Record> &recs) {
+  for (const Record &r : recs) {
+    o << r.header << '\n';
+    for (size_t p = 0; p < r.seq.size(); p += 80) o << r.seq.substr(p, 80) << '\n';
+  }
+}
+
+std::string dealign(const std::string &s) {
+  std::string r;
+  for (char c : s)
+    if (c != '-' && c != '.') r.push_back(c);
+  return r;
+}
+
+bool looks_dna(const std::vector<Record> &recs) {
+  size_t n = 0;
+  for (const Record &r : recs)
+    for (char c : r.seq) {
+      const char u = (char)toupper((unsigned char)c);
+      if (u != 'A' && u != 'C' && u != 'G' && u != 'T') return false;
+      ++n;
+    }
+  return n > 0;
+}
+
+// CIGAR text -> ops string (M / I / D per column).
+std::string expand(const char *cig, int len) {
+  std::string ops;
+  int v = 0;
+  for (int p = 0; p < len; ++p) {
+    if (isdigit((unsigned char)cig[p])) { v = v * 10 + (cig[p] - '0'); continue; }
+    ops.append((size_t)v, cig[p]);
+    v = 0;
+  }
+  return ops;
+}
+
+// Gapped rows of a global alignment of a (query) and b (reference).
+void render(const std::string &a, const std::string &b, const std::string &ops, std::string &ra,
+            std::string &rb) {
+  size_t i = 0, j = 0;
+  ra.clear();
+  rb.clear();
+  for (char o : ops) {
+    if (o == 'M') { ra += a[i++]; rb += b[j++]; }
+    else if (o == 'I') { ra += a[i++]; rb += '-'; }
+    else { ra += '-'; rb += b[j++]; }
+  }
+}
+
+int fail(const std::string &msg) {
+  std::cerr << "tweakalign: " << msg << std::endl;
+  return 1;
+}
+
+struct Aligner {
+  ta_ctx *ctx = nullptr;
+  ta_params p{};
+  ~Aligner() { if (ctx) ta_destroy(ctx); }
+
+  int run(const std::vector<std::pair<const std::string *, const std::string *>> &pairs, std::vector<ta_result> &res,
+          std::vector<char> &cig) {
+    std::vector<ta_pair> tp;
+    for (auto &q : pairs) tp.push_back({q.first->data(), (int32_t)q.first->size(), q.second->data(), (int32_t)q.second->size()});
+    res.resize(tp.size());
+    int64_t used = 0;
+    int rc = ta_align_batch(ctx, &p, tp.data(), (int32_t)tp.size(), res.data(), nullptr, 0, &used);
+    if (rc != TA_OK && rc != TA_E_SPACE) return rc;
+    if (!p.traceback) return TA_OK;
+    cig.assign((size_t)used + 1, 0);
+    return ta_align_batch(ctx, &p, tp.data(), (int32_t)tp.size(), res.data(), cig.data(), used + 1, &used);
+  }
+};
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::string fin, fout;
+  bool to_stdout = false, force_dna = false, force_protein = false, fasta_only = false;
+  int gap_open = 0, gap_ext = 0;
+  for (int k = 1; k < argc; ++k) {
+    const std::string a = argv[k];
+    auto next = [&](const char *what) -> std::string {
+      if (k + 1 >= argc) { fail(std::string("missing value after ") + what); exit(1); }
+      return argv[++k];
+    };
+    if (a == "-version" || a == "--version") {
+      std::cout << "tweakalign 1.0.0 (Gotoh on MI355X)" << std::endl;
+      return 0;
+    } else if (a == "-in" || a == "-i") fin = next(a.c_str());
+    else if (a == "-out" || a == "-o") fout = next(a.c_str());
+    else if (a == "-dna") force_dna = true;
+    else if (a == "-protein") force_protein = true;
+    else if (a == "-gapopen") gap_open = atoi(next("-gapopen").c_str());
+    else if (a == "-gapextend") gap_ext = atoi(next("-gapextend").c_str());
+    else if (a == "--fasta-roundtrip") fasta_only = true;  // parse and rewrite only (no GPU)
+    else if (a == "--thread") next("--thread");             // MAFFT style, ignored
+    else if (a == "--auto" || a == "--force" || a == "-v" || a.rfind("--outfmt", 0) == 0 ||
+             a.rfind("--output-order", 0) == 0) {
+    }  // ClustalO / MAFFT flags with no meaning here
+    else if (!a.empty() && a[0] != '-' && fin.empty()) { fin = a; to_stdout = true; }  // MAFFT: input only
+    else return fail("unknown argument " + a);
+  }
+  if (fin.empty()) return fail("no input file (-in FILE)");
+  if (fout.empty()) to_stdout = true;
+
+  std::ifstream in(fin);
+  if (!in) return fail("cannot open " + fin);
+  std::vector<Record> recs;
+  read_fasta(in, recs);
+  if (recs.empty()) return fail("no sequences in " + fin);
+  for (Record &r : recs) r.seq = dealign(r.seq);
+
+  std::vector<Record> outrecs = recs;
+  if (!fasta_only && recs.size() >= 2) {
+    Aligner al;
+    const bool dna = force_dna || (!force_protein && looks_dna(recs));
+    al.p.mode = TA_GLOBAL;
+    al.p.alphabet = dna ? TA_DNA : TA_PROTEIN;
+    al.p.gap_open = gap_open ? gap_open : (dna ? -5 : -10);
+    al.p.gap_extend = gap_ext ? gap_ext : (dna ? -2 : -1);
+    al.p.match = 2;
+    al.p.mismatch = -3;
+    al.p.lanes = TA_LANES_AUTO;
+    int rc = ta_create(&al.ctx, 0);
+    if (rc) return fail(std::string("GPU: ") + ta_strerror(rc));
+    const size_t N = recs.size();
+    size_t center = 0;
+    std::vector<ta_result> res;
+    std::vector<char> cig;
+    if (N > 2) {
+      // All-vs-all global scores (score only) choose the center.
+      std::vector<std::pair<const std::string *, const std::string *>> pairs;
+      for (size_t x = 0; x < N; ++x)
+        for (size_t y = x + 1; y < N; ++y) pairs.push_back({&recs[x].seq, &recs[y].seq});
+      al.p.traceback = 0;
+      rc = al.run(pairs, res, cig);
+      if (rc) return fail(std::string("alignment failed: ") + ta_strerror(rc));
+      std::vector<long long> sum(N, 0);
+      size_t q = 0;
+      for (size_t x = 0; x < N; ++x)
+        for (size_t y = x + 1; y < N; ++y, ++q) { sum[x] += res[q].score; sum[y] += res[q].score; }
+      center = (size_t)(std::max_element(sum.begin(), sum.end()) - sum.begin());
+    }
+    // Every other sequence against the center (center = query rows).
+    std::vector<std::pair<const std::string *, const std::string *>> pairs;
+    std::vector<size_t> who;
+    for (size_t x = 0; x < N; ++x)
+      if (x != center) { pairs.push_back({&recs[center].seq, &recs[x].seq}); who.push_back(x); }
+    al.p.traceback = 1;
+    rc = al.run(pairs, res, cig);
+    if (rc) return fail(std::string("alignment failed: ") + ta_strerror(rc));
+    const std::string &c = recs[center].seq;
+    std::vector<std::string> ops(pairs.size());
+    for (size_t q = 0; q < pairs.size(); ++q) ops[q] = expand(cig.data() + res[q].cigar_off, res[q].cigar_len);
+    if (N == 2) {
+      std::string ra, rb;
+      render(c, recs[who[0]].seq, ops[0], ra, rb);
+      outrecs[center].seq = ra;
+      outrecs[who[0]].seq = rb;
+    } else {
+      // Insertion slots between center residues: the widest 'D' run there.
+      std::vector<size_t> slot(c.size() + 1, 0);
+      for (const std::string &o : ops) {
+        size_t i = 0, run = 0;
+        for (char x : o) {
+          if (x == 'D') { ++run; continue; }
+          slot[i] = std::max(slot[i], run);
+          run = 0;
+          ++i;  // M and I both consume a center residue
+        }
+        slot[i] = std::max(slot[i], run);
+      }
+      std::string cr;
+      for (size_t i = 0; i <= c.size(); ++i) {
+        cr.append(slot[i], '-');
+        if (i < c.size()) cr += c[i];
+      }
+      outrecs[center].seq = cr;
+      for (size_t q = 0; q < ops.size(); ++q) {
+        const std::string &s = recs[who[q]].seq;
+        std::string row;
+        size_t i = 0, j = 0, run = 0;
+        auto close = [&](size_t at) { row.append(slot[at] - run, '-'); run = 0; };
+        for (char x : ops[q]) {
+          if (x == 'D') { row += s[j++]; ++run; continue; }
+          close(i);
+          row += x == 'M' ? s[j++] : '-';
+          ++i;
+        }
+        close(i);
+        outrecs[who[q]].seq = row;
+      }
+    }
+  }
+
+  if (to_stdout) {
+    write_fasta(std::cout, outrecs);
+    std::cout.flush();
+    return std::cout ? 0 : 1;
+  }
+  std::ofstream out(fout);
+  if (!out) return fail("cannot write " + fout);
+  write_fasta(out, outrecs);
+  out.close();
+  return out ? 0 : 1;
+}
