@@ -41,9 +41,12 @@ extern "C" {
 #define TA_DNA 1              /* A,C,G,T; match / mismatch scores */
 
 /* ta_params.lanes */
-#define TA_LANES_AUTO 0       /* packed int16 where safe, int32 otherwise */
+#define TA_LANES_AUTO 0       /* packed int16 where safe, int32 otherwise; global pairs
+                                 of >= 2^25 cells take the long-pair path */
 #define TA_LANES_I16 16       /* force packed int16 (saturating pairs rerun in int32) */
 #define TA_LANES_I32 32       /* force int32 */
+#define TA_LANES_LONG 64      /* global only: force the long-pair path (int32 strips
+                                 pipelined across CUs, checkpointed traceback) */
 
 /* ta_result.flags */
 #define TA_F_I32 0x1          /* result computed by the int32 kernels */

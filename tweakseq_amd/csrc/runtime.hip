@@ -19,6 +19,7 @@
 #include "../../include/tweakalign.h"
 #include "common.h"
 #include "fill.h"
+#include "long.h"
 #include "runtime_internal.h"
 #include "traceback.h"
 
@@ -275,8 +276,19 @@ namespace {
 struct HostPair {
   uint64_t a_off, b_off;
   int32_t la, lb;
-  int path;  // 2 packed int16, 1 int32, 0 no DP (empty side)
+  int path;  // 3 long-pair pipeline, 2 packed int16, 1 int32, 0 no DP (empty side)
 };
+
+// Global pairs of at least this many cells go through the long-pair
+// pipeline: one task would otherwise hold a single wave for the whole sweep
+// (and 4 bits per cell of direction planes).
+constexpr int64_t kLongCells = 1ll << 25;
+
+bool long_pair(const ta_params &p, int la, int lb) {
+  if (p.mode != TA_GLOBAL || la == 0 || lb == 0) return false;
+  if (p.lanes == TA_LANES_LONG) return true;
+  return p.lanes == TA_LANES_AUTO && (int64_t)la * lb >= kLongCells;
+}
 
 struct Plan {
   std::vector<Task> tasks2[2], tasks1[2];  // [0] whole-wave groups, [1] 16-lane groups
@@ -383,6 +395,16 @@ struct ta_batch {
   uint32_t *d_ring_r = nullptr;
   uint8_t *d_tb_r = nullptr;
   int32_t n_rerun = 0;
+  // long-pair pipeline
+  std::vector<LongPair> longs;
+  std::vector<LongItem> litems;
+  std::vector<int32_t> litem_first;  // per long pair: its first queue item
+  LongPair *d_lp = nullptr;
+  LongItem *d_litems = nullptr;
+  int32_t *d_lring = nullptr, *d_lck = nullptr, *d_lprog = nullptr, *d_wring = nullptr;
+  uint8_t *d_wtb = nullptr;
+  uint64_t lring_elems = 0, lck_elems = 0, wring_stride = 0, wtb_stride = 0;
+  int32_t lprog_n = 0, long_wgs = 0, spin_limit = 1 << 20;  // about a second of polling
   // timing
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double ms[4] = {0, 0, 0, 0};
@@ -420,6 +442,8 @@ void ta_batch_destroy(ta_batch *B) {
   dfree(B->d_ptb); dfree(B->d_pairs); dfree(B->d_fo); dfree(B->d_res); dfree(B->d_cig);
   dfree(B->d_ctr); dfree(B->d_fo_init); dfree(B->d_order);
   free_rerun(B);
+  dfree(B->d_lp); dfree(B->d_litems); dfree(B->d_lring); dfree(B->d_lck); dfree(B->d_lprog);
+  dfree(B->d_wring); dfree(B->d_wtb);
   dfree(B->d_plain_tasks); dfree(B->d_prow); dfree(B->d_pcol); dfree(B->d_pring);
   dfree(B->d_ptbuf); dfree(B->d_pptb);
   for (auto &e : B->ev)
@@ -441,7 +465,10 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     return TA_E_INVALID;
   if (p->alphabet == TA_DNA && (p->match < -100 || p->match > 100 || p->mismatch < -100 || p->mismatch > 100))
     return TA_E_INVALID;
-  if (p->lanes != TA_LANES_AUTO && p->lanes != TA_LANES_I16 && p->lanes != TA_LANES_I32) return TA_E_INVALID;
+  if (p->lanes != TA_LANES_AUTO && p->lanes != TA_LANES_I16 && p->lanes != TA_LANES_I32 &&
+      p->lanes != TA_LANES_LONG)
+    return TA_E_INVALID;
+  if (p->lanes == TA_LANES_LONG && p->mode != TA_GLOBAL) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
 
   ta_batch *B = new ta_batch();
@@ -477,6 +504,11 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       delete B;
       return TA_E_INVALID;
     }
+    // Global scores must stay clear of the int32 kernels' -infinity (-2^29).
+    if (!local && (int64_t)(q.la + q.lb + 2) * -p->gap_extend + 2 * (int64_t)-p->gap_open >= (1ll << 28)) {
+      delete B;
+      return TA_E_INVALID;
+    }
     HostPair &h = B->hp[k];
     int rc = enc(q.a, q.la, &h.a_off);
     if (!rc) rc = enc(q.b, q.lb, &h.b_off);
@@ -492,6 +524,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
         f.score = len ? p->gap_open + (len - 1) * p->gap_extend : 0;
         f.end_i = q.la; f.end_j = q.lb;
       }
+    } else if (long_pair(*p, q.la, q.lb)) {
+      h.path = 3;
     } else if (p->lanes == TA_LANES_I32) {
       h.path = 1;
     } else if (local) {
@@ -568,6 +602,45 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     std::stable_sort(pl.tasks1[g].begin(), pl.tasks1[g].end(), by_cost);
   }
 
+  // Long pairs: strips of all pairs form one queue, largest pairs first.
+  {
+    std::vector<int> lk;
+    for (int32_t k = 0; k < n; ++k)
+      if (B->hp[k].path == 3) lk.push_back(k);
+    std::stable_sort(lk.begin(), lk.end(), [&](int x, int y) {
+      return (int64_t)B->hp[x].la * B->hp[x].lb > (int64_t)B->hp[y].la * B->hp[y].lb;
+    });
+    int max_rl = 0, max_n = 0;
+    for (int k : lk) {
+      const HostPair &h = B->hp[k];
+      LongPair L{};
+      L.pid = k; L.m = h.la; L.n = h.lb; L.S = ceil_div(h.la, LST);
+      L.a_off = h.a_off; L.b_off = h.b_off;
+      L.rl = ((h.lb + 7) & ~7) + 80;  // block loads / stores run up to 71 columns past n
+      L.ring_off = B->lring_elems;
+      if (L.S > 1) B->lring_elems += 4ull * L.rl;
+      L.ck_off = B->lck_elems;
+      if (p->traceback) B->lck_elems += (uint64_t)((L.S - 1) / KC) * 2 * L.rl;
+      L.prog_off = B->lprog_n;
+      B->lprog_n += L.S;
+      B->litem_first.push_back((int32_t)B->litems.size());
+      for (int s = 0; s < L.S; ++s) B->litems.push_back(LongItem{(int32_t)B->longs.size(), s});
+      B->longs.push_back(L);
+      pl.ptb[k].P = 3;
+      max_rl = std::max(max_rl, L.rl);
+      max_n = std::max(max_n, L.n);
+    }
+    if (!lk.empty() && p->traceback) {
+      B->wring_stride = 4ull * max_rl;
+      B->wtb_stride = (uint64_t)KC * GWAVE * ((max_n + GWAVE + 7) / 8) * 32;
+      // Direction planes of the traceback workgroups: at most 32 GB in all.
+      const uint64_t budget = 32ull << 30;
+      const int by_mem = (int)std::max<uint64_t>(1, budget / B->wtb_stride);
+      B->long_wgs = std::max(1, std::min({(int)lk.size(), 2 * ctx->cus, by_mem}));
+    }
+    if (const char *e = getenv("TWEAKALIGN_SPIN_LIMIT")) B->spin_limit = std::max(1, atoi(e));
+  }
+
   // Upload.
   hipStream_t st = B->stream;
   std::vector<int8_t> table = padded_table(B->alph);
@@ -596,7 +669,14 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   rc = rc ? rc : dalloc(&B->d_fo, (size_t)std::max(n, 1));
   rc = rc ? rc : dalloc(&B->d_res, (size_t)std::max(n, 1));
   rc = rc ? rc : dalloc(&B->d_cig, B->cig_bytes);
-  rc = rc ? rc : dalloc(&B->d_ctr, 8);
+  rc = rc ? rc : dalloc(&B->d_ctr, 16);
+  rc = rc ? rc : upload(&B->d_lp, B->longs, st);
+  rc = rc ? rc : upload(&B->d_litems, B->litems, st);
+  rc = rc ? rc : dalloc(&B->d_lring, B->lring_elems);
+  rc = rc ? rc : dalloc(&B->d_lck, B->lck_elems);
+  rc = rc ? rc : dalloc(&B->d_lprog, (size_t)B->lprog_n);
+  rc = rc ? rc : dalloc(&B->d_wring, B->wring_stride * B->long_wgs);
+  rc = rc ? rc : dalloc(&B->d_wtb, B->wtb_stride * B->long_wgs);
   for (auto &e : B->ev)
     if (!rc && hipEventCreate(&e) != hipSuccess) rc = TA_E_DEVICE;
   if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = TA_E_DEVICE;
@@ -611,19 +691,27 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
 int64_t batch_tb_estimate(const ta_params *p, const ta_pair *pairs, int32_t n) {
   if (!p || (p->mode != TA_LOCAL && !p->traceback)) return 0;
   int64_t t = 0;
-  for (int32_t k = 0; k < n; ++k)
-    if (pairs[k].la > 0 && pairs[k].lb > 0)
-      t += (int64_t)ceil_div(pairs[k].la, STRIP) * ceil_div(pairs[k].lb + G, 8) * 512;
+  for (int32_t k = 0; k < n; ++k) {
+    const int la = pairs[k].la, lb = pairs[k].lb;
+    if (la <= 0 || lb <= 0) continue;
+    if (long_pair(*p, la, lb)) {  // checkpoints + one traceback workgroup's planes
+      const int64_t rl = ((lb + 7) & ~7) + 80;
+      t += (int64_t)((ceil_div(la, LST) - 1) / KC) * 8 * rl + (int64_t)KC * GWAVE * ((lb + GWAVE + 7) / 8) * 32;
+    } else {
+      t += (int64_t)ceil_div(la, STRIP) * ceil_div(lb + G, 8) * 512;
+    }
+  }
   return t;
 }
 
 int ta_batch_info(ta_batch *B, int64_t *cells, int64_t *tb_bytes, int32_t *tasks) {
   if (!B) return TA_E_INVALID;
   if (cells) *cells = B->cells;
-  if (tb_bytes) *tb_bytes = (int64_t)B->plan.tb_bytes;
+  if (tb_bytes)
+    *tb_bytes = (int64_t)(B->plan.tb_bytes + B->wtb_stride * B->long_wgs + 4 * B->lck_elems);
   if (tasks)
     *tasks = (int32_t)(B->plan.tasks2[0].size() + B->plan.tasks2[1].size() + B->plan.tasks1[0].size() +
-                       B->plan.tasks1[1].size());
+                       B->plan.tasks1[1].size() + B->litems.size());
   return TA_OK;
 }
 
@@ -674,6 +762,54 @@ int launch_traceback(ta_ctx *ctx, ta_batch *B, const PairDev *pairs, const PairT
   (void)hipGetLastError();
   hipLaunchKernelGGL(traceback_kernel, dim3((count + TB_WALKERS - 1) / TB_WALKERS), dim3(TB_WALKERS), 0,
                      B->stream, a);
+  HIPCHK(hipGetLastError());
+  return TA_OK;
+}
+
+// Long pairs [lp0, lp0 + nlp) of the batch (queue items [it0, it0 + nit)).
+LongArgs long_args(ta_batch *B, int it0, int nit, int lp0, int nlp) {
+  LongArgs a{};
+  a.lp = B->d_lp + lp0; a.nlp = nlp;
+  a.items = B->d_litems + it0; a.nitems = nit;
+  a.ctr = B->d_ctr + 8;
+  a.table = B->d_table; a.Kp = B->Kp; a.codes = B->d_codes;
+  a.ring = B->d_lring; a.ckpt = B->d_lck; a.prog = B->d_lprog;
+  a.wring = B->d_wring; a.wring_stride = B->wring_stride;
+  a.wtb = B->d_wtb; a.wtb_stride = B->wtb_stride;
+  a.pd = B->d_pairs; a.fo = B->d_fo; a.res = B->d_res; a.cig = B->d_cig;
+  a.open = B->p.gap_open; a.ext = B->p.gap_extend;
+  a.ckpt_on = B->p.traceback;
+  a.spin_limit = B->spin_limit;
+  a.err = B->d_ctr + 10;
+  return a;
+}
+
+// Forward pass of long pairs: counters cleared, then one wave per workgroup
+// pulling strips.  Items index pairs relative to lp0 = 0, so a subset of
+// items must keep the full pair array.
+int launch_long_fwd(ta_ctx *ctx, ta_batch *B, int it0, int nit) {
+  if (nit <= 0) return TA_OK;
+  HIPCHK(hipMemsetAsync(B->d_ctr + 8, 0, 3 * sizeof(int32_t), B->stream));
+  HIPCHK(hipMemsetAsync(B->d_lprog, 0, sizeof(int32_t) * B->lprog_n, B->stream));
+  LongArgs a = long_args(B, it0, nit, 0, (int)B->longs.size());
+  const int lds = long_fwd_lds(B->Kp);
+  const int per_cu = std::max(1, std::min(16, (160 * 1024) / lds));  // 4 waves per SIMD (117 VGPRs)
+  const int grid = std::max(1, std::min(nit, ctx->cus * per_cu));
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(long_fwd_kernel, dim3(grid), dim3(64), lds, B->stream, a);
+  HIPCHK(hipGetLastError());
+  return TA_OK;
+}
+
+// Traceback of long pairs [lp0, lp0 + nlp): one workgroup of LW waves per pair.
+int launch_long_tb(ta_ctx *ctx, ta_batch *B, int lp0, int nlp) {
+  if (nlp <= 0 || !B->p.traceback) return TA_OK;
+  HIPCHK(hipMemsetAsync(B->d_ctr + 9, 0, sizeof(int32_t), B->stream));
+  LongArgs a = long_args(B, 0, 0, lp0, nlp);
+  const int lds = long_tb_lds(B->Kp);
+  const int grid = std::max(1, std::min(nlp, B->long_wgs));
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(long_tb_kernel, dim3(grid), dim3(64 * LW), lds, B->stream, a);
   HIPCHK(hipGetLastError());
   return TA_OK;
 }
@@ -736,6 +872,7 @@ int ta_batch_run(ta_ctx *ctx, ta_batch *B) {
   int rc = launch_fill<2>(ctx, B, B->d_tasks2, n2, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb, B->d_ctr, 0);
   if (!rc) rc = launch_fill<1>(ctx, B, B->d_tasks1, n1, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb,
                                B->d_ctr + 2, 0);
+  if (!rc) rc = launch_long_fwd(ctx, B, 0, (int)B->litems.size());
   if (rc) return rc;
   HIPCHK(hipEventRecord(B->ev[1], st));
   if (B->may_saturate) {
@@ -746,6 +883,7 @@ int ta_batch_run(ta_ctx *ctx, ta_batch *B) {
   }
   HIPCHK(hipEventRecord(B->ev[2], st));
   rc = launch_traceback(ctx, B, B->d_pairs, B->d_ptb, B->d_tb, 0, B->n);
+  if (!rc) rc = launch_long_tb(ctx, B, 0, (int)B->longs.size());
   if (rc) return rc;
   HIPCHK(hipEventRecord(B->ev[3], st));
   return TA_OK;
@@ -759,6 +897,14 @@ int ta_batch_wait(ta_ctx *ctx, ta_batch *B) {
   if (hipEventElapsedTime(&t, B->ev[1], B->ev[2]) == hipSuccess) B->ms[1] = t;
   if (hipEventElapsedTime(&t, B->ev[2], B->ev[3]) == hipSuccess) B->ms[2] = t;
   if (hipEventElapsedTime(&t, B->ev[0], B->ev[3]) == hipSuccess) B->ms[3] = t;
+  if (!B->longs.empty()) {
+    int32_t err = 0;
+    HIPCHK(hipMemcpy(&err, B->d_ctr + 10, sizeof(err), hipMemcpyDeviceToHost));
+    if (err) {
+      fprintf(stderr, "tweakalign: long-pair pipeline wait gave up (spin limit %d)\n", B->spin_limit);
+      return TA_E_DEVICE;
+    }
+  }
   return TA_OK;
 }
 
@@ -850,7 +996,8 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
       const HostPair &h = B->hp[k];
       const int P = h.path == 1 ? 1 : 2;
       const uint64_t tb0 = pl.tb_bytes, r0 = pl.ring_elems;
-      if (h.path == 0) {
+      if (h.path == 0 || h.path == 3) {
+        if (h.path == 3) pl.ptb[k].P = 3;  // the long-pair kernels write its result
         tasks.push_back(Task{});
         continue;
       }
@@ -878,8 +1025,18 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
   HIPCHK(hipEventRecord(B->ev[0], st));
   if (B->n > 0)
     HIPCHK(hipMemcpyAsync(B->d_fo, B->d_fo_init, sizeof(FillOut) * B->n, hipMemcpyDeviceToDevice, st));
+  std::vector<int> lp_of(B->n, -1);
+  for (size_t q = 0; q < B->longs.size(); ++q) lp_of[B->longs[q].pid] = (int)q;
   for (int k = first; k < first + count; ++k) {
     const HostPair &h = B->hp[k];
+    if (h.path == 3) {
+      const int q = lp_of[k];
+      int rc = launch_long_fwd(ctx, B, B->litem_first[q], B->longs[q].S);
+      if (!rc) rc = launch_traceback(ctx, B, B->d_pairs, B->d_pptb, B->d_ptbuf, k, 1);
+      if (!rc) rc = launch_long_tb(ctx, B, q, 1);
+      if (rc) return rc;
+      continue;
+    }
     if (h.path) {
       HIPCHK(hipMemsetAsync(B->d_ctr + 6, 0, 2 * sizeof(int32_t), st));
       const bool wave = B->plain_tasks[k].S && pick_gg(h.la, h.lb, 0.0) == GWAVE;

@@ -246,6 +246,7 @@ __global__ __launch_bounds__(TB_WALKERS) void traceback_kernel(TbArgs A) {
     A.res[p] = r;
     return;
   }
+  if (A.ptb[p].P == 3) return;  // long pair: its own traceback kernel writes the result
 
   uint32_t *slice = wlds + threadIdx.x * WSTRIDE;
   if (A.ptb[p].P == 2) walk<2>(A, p, pd, f, r, local, stab, slice);

@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(_HERE, "libtweakalign.so")
 
 GLOBAL, LOCAL = 0, 1
 PROTEIN, DNA = 0, 1
-LANES_AUTO, LANES_I16, LANES_I32 = 0, 16, 32
+LANES_AUTO, LANES_I16, LANES_I32, LANES_LONG = 0, 16, 32, 64
 F_I32, F_RERUN, F_LONG, F_EMPTY = 0x1, 0x2, 0x4, 0x8
 E_PENDING = 1
 
