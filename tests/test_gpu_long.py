@@ -57,6 +57,19 @@ def test_long_path_dna_shapes(eng):
     assert all(g.flags & ta.F_LONG for g in got)
 
 
+def test_long_path_long_gaps(eng):
+    """Long gaps move the path far off the diagonal: the traceback leaves
+    its regions through the left side and continues in new ones."""
+    rng = random.Random(13)
+    a = rand_prot(rng, 8000, DNA)
+    pairs = [(a, a[:3000] + rand_prot(rng, 2500, DNA) + a[3000:]),   # long D run
+             (a, a[:2000] + a[5000:]),                              # long I run
+             (a[:1500], a),                                         # b much longer
+             (a, a[6000:7000] + a[:200])]
+    got = run_check(eng, pairs, "long gaps", lanes=ta.LANES_LONG, **DNA_PARAMS)
+    assert all(g.flags & ta.F_LONG for g in got)
+
+
 def test_long_path_score_only(eng):
     pairs = dna_pairs(random.Random(6), SHAPES[:10])
     run_check(eng, pairs, "long dna score", lanes=ta.LANES_LONG, traceback=0, **DNA_PARAMS)

@@ -10,17 +10,25 @@
 // running, so the pipeline cannot deadlock.  Ring rows and counters cross
 // CUs (and the XCDs' separate L2s) with device-coherent (sc1) accesses and
 // an explicit wait for completion before a counter moves; no cache
-// write-back or invalidate is involved.  The forward pass is score only and
-// keeps the bottom row (H, F) of every KC-th strip as a checkpoint.
+// write-back or invalidate is involved.
 //
-// Traceback (long_tb_kernel, one workgroup of LW waves per pair): blocks of
-// KC strips from the bottom up.  A block is recomputed from its checkpoint
-// with direction planes for the columns left of the path's entry column
-// (its waves pipelined through a workgroup-local ring and LDS counters),
-// then one thread walks it and hands row, column and state to the block
-// above.  The recomputed cells are the forward pass's cells, so the walk is
-// the full-matrix walk of the specification (oracle/gotoh.c), bit for bit,
-// in O(KC strips x columns) memory per pair.
+// The forward pass is score only.  When a traceback follows it also keeps
+// checkpoints: the bottom row (H, F) of every KC-th strip, and the column
+// (H, E of every row) at every CKX-th column.
+//
+// Traceback (long_tb_kernel, one workgroup of LW waves per pair) walks the
+// matrix from (m, n) in regions.  A region is the rows of one block of KC
+// strips up to the walk's row and a band of columns ending at the walk's
+// column and starting at a column checkpoint far enough left for a path
+// near the diagonal to leave through the block's top: its top boundary is a
+// row checkpoint, its left boundary a column checkpoint.  The region is
+// recomputed with direction planes (its strips pipelined over the
+// workgroup's waves through a private ring and LDS counters), one thread
+// walks it until the path leaves through the top or the left, and the next
+// region starts there.  The recomputed cells are the forward pass's cells,
+// so the walk is the full-matrix walk of the specification (oracle/gotoh.c),
+// bit for bit; a region is at most LREG columns wide, so the traceback costs
+// about (KC strips x LREG) cells per block of rows instead of the matrix.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -35,12 +43,16 @@ constexpr int KC = LW;          // strips per checkpoint block (one per wave)
 constexpr int LST = GWAVE * R;  // query rows per strip
 constexpr int LPUB = 8;         // progress is published every LPUB blocks (64 columns)
 constexpr int LMAXS = 2047;     // strips per pair the counter tags can name
+constexpr int CKX = 1024;       // column checkpoint spacing
+constexpr int LMARGIN = 256;    // region columns beyond the diagonal through the walk's rows
+constexpr int LREG = KC * LST + LMARGIN + CKX;  // widest traceback region
 
 struct LongPair {
   int32_t pid, m, n, S;
   uint64_t a_off, b_off;  // residue codes (16-byte aligned, zero padded)
   uint64_t ring_off;      // int32 elements: 2 slots x (H, F) x rl
   uint64_t ck_off;        // int32 elements: (S - 1) / KC rows x (H, F) x rl
+  uint64_t colck_off;     // int2 elements: (n - 1) / CKX columns x S * LST rows of (H, E)
   int32_t prog_off;       // first of the pair's S progress counters
   int32_t rl;             // elements per row array (column index, + slack)
 };
@@ -59,6 +71,7 @@ struct LongArgs {
   int32_t Kp;
   const uint8_t *codes;
   int32_t *ring, *ckpt, *prog;
+  int2 *colck;
   int32_t *wring;       // traceback workgroups: private ring, wring_stride elements each
   uint64_t wring_stride;
   uint8_t *wtb;         // traceback workgroups: direction planes, wtb_stride bytes each
@@ -122,26 +135,45 @@ struct GroupSync {
 
 enum { TOP_ROW0 = 0, TOP_RING = 1, TOP_CKPT = 2 };
 
+// What one strip sweep covers and where its boundaries come from.  Columns
+// are c_left + 1 .. c_left + ncols (1-based); "local" column = column - c_left.
+struct Region {
+  int s, c_left, ncols;
+  int top_kind;
+  const int32_t *top;  // ring (local columns) or row checkpoint (absolute columns)
+  const int2 *left;    // H, E of column c_left for every row; null: matrix column 0
+  int32_t *out, *ck;   // bottom row -> ring (local), row checkpoint (absolute)
+  int2 *colck;         // column checkpoints to store (forward pass, c_left == 0)
+  uint8_t *tbs;        // direction planes (local columns)
+  int NBc;
+  bool score;
+};
+
 __device__ __forceinline__ uint32_t round16(int x) { return (uint32_t)((x + 15) & ~15); }
 
-// One strip (rows s*LST .. s*LST+511) over columns [0, ncols), swept by the
-// calling wave.  Top input: row 0, the strip above through the ring (waits on
-// its counter), or a complete checkpoint row.  Bottom row goes to `out`
-// (ring) and `ck` (checkpoint) when given; direction planes to `tbs` when TB.
+// One strip (rows s*LST .. s*LST+511) over the region's columns, swept by
+// the calling wave.  Top input: row 0, the strip above through the ring
+// (waits on its counter), or a complete checkpoint row; left input: column 0
+// or a column checkpoint.  Bottom row goes to the ring and the row
+// checkpoint when given, direction planes to `tbs` when TB.
 template <class Sync, bool TB>
-__device__ void long_strip(const LongArgs &A, const Sync &y, const LongPair &P, int s, int ncols, int top_kind,
-                           const int32_t *top, int32_t *out, int32_t *ck, uint8_t *tbs, int NBc,
-                           const int8_t *tab, int8_t *slot, bool score, bool &gave_up) {
+__device__ void long_strip(const LongArgs &A, const Sync &y, const LongPair &P, const Region &G,
+                           const int8_t *tab, int8_t *slot, bool &gave_up) {
   constexpr int GG = GWAVE;
   const int l = threadIdx.x & 63;
   const int Kp = A.Kp, open = A.open, ext = A.ext;
   const int rl = P.rl;
+  const int s = G.s, ncols = G.ncols, c_left = G.c_left, top_kind = G.top_kind;
+  int32_t *const out = G.out, *const ck = G.ck;
   const auto ra = brsrc(A.codes + P.a_off, round16(P.m));
-  const auto rb = brsrc(A.codes + P.b_off, round16(P.n));
-  const auto rt = brsrc(top, top ? 8u * (uint32_t)rl : 0u);
+  const auto rb = brsrc(A.codes + P.b_off + c_left, round16(P.n) - (uint32_t)c_left);
+  // A checkpoint row is read from column c_left on: same offsets as a ring.
+  const int32_t *top = top_kind == TOP_CKPT ? G.top + c_left : G.top;
+  const auto rt = brsrc(top, top ? 4u * (uint32_t)(2 * rl - (top_kind == TOP_CKPT ? c_left : 0)) : 0u);
   const auto ro = brsrc(out, out ? 8u * (uint32_t)rl : 0u);
   const auto rc = brsrc(ck, ck ? 8u * (uint32_t)rl : 0u);
   const int row0 = s * LST + l * R;
+  const uint64_t rows_pad = (uint64_t)P.S * LST;
 
   {  // profile slot of the lane's rows (codes past m read as 0; those rows feed nothing)
     const uint2 c8 = bload2(ra, (uint32_t)row0);
@@ -154,13 +186,26 @@ __device__ void long_strip(const LongArgs &A, const Sync &y, const LongPair &P, 
   }
   auto left = [&](int i) -> int { return i == 0 ? 0 : open + (i - 1) * ext; };
   int H[R], E[R], HoL[R];
+  int hd0;
+  if (G.left) {  // column checkpoint: rows row0 .. row0 + 7 (0-based) at column c_left
+    const int4 *lp = (const int4 *)(G.left + row0);
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    H[k] = left(row0 + k + 1);
-    E[k] = NEG32;
-    HoL[k] = H[k] + open;
+    for (int q = 0; q < R / 2; ++q) {
+      const int4 v = lp[q];
+      H[2 * q] = v.x; E[2 * q] = v.y; H[2 * q + 1] = v.z; E[2 * q + 1] = v.w;
+    }
+    hd0 = row0 == 0 ? open + (c_left - 1) * ext : G.left[row0 - 1].x;
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      H[k] = left(row0 + k + 1);
+      E[k] = NEG32;
+    }
+    hd0 = left(row0);
   }
-  int hd0 = left(row0), Hbot = 0, Fbot = 0, carryH = 0, carryF = 0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) HoL[k] = H[k] + open;
+  int Hbot = 0, Fbot = 0, carryH = 0, carryF = 0;
   const int NB = (ncols + GG + 7) / 8;
 
   // Columns of the top row known to be stored.
@@ -217,6 +262,9 @@ __device__ void long_strip(const LongArgs &A, const Sync &y, const LongPair &P, 
     const uint32_t tfw[8] = {tF0.x, tF0.y, tF0.z, tF0.w, tF1.x, tF1.y, tF1.z, tF1.w};
     uint32_t tbw[8];
     int rbH[8], rbF[8];
+    // Does a lane reach a checkpoint column (j + 1 a multiple of CKX) here?
+    const int hi = 8 * blk + 8;
+    const bool ckb = G.colck && hi >= CKX && (hi / CKX) * CKX >= 8 * blk - 62;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int t = 8 * blk + q;
@@ -228,7 +276,7 @@ __device__ void long_strip(const LongArgs &A, const Sync &y, const LongPair &P, 
       pa_n = *(const uint2 *)(slot + ccur * R);
       int topH, topF;
       if (top_kind == TOP_ROW0) {
-        topH = open + t * ext;
+        topH = open + (c_left + t) * ext;
         topF = NEG32;
       } else {
         topH = (int)thw[q];
@@ -269,12 +317,19 @@ __device__ void long_strip(const LongArgs &A, const Sync &y, const LongPair &P, 
         Fbot = F;
         hd0 = Hup;
         tbw[q] = acc;
+        if (ckb && ((j + 1) & (CKX - 1)) == 0 && j + 1 < P.n) {
+          int4 *d = (int4 *)(G.colck + (uint64_t)((j + 1) / CKX - 1) * rows_pad + row0);
+          d[0] = make_int4(H[0], E[0], H[1], E[1]);
+          d[1] = make_int4(H[2], E[2], H[3], E[3]);
+          d[2] = make_int4(H[4], E[4], H[5], E[5]);
+          d[3] = make_int4(H[6], E[6], H[7], E[7]);
+        }
       }
       rbH[q] = Hbot;
       rbF[q] = Fbot;
     }
     if constexpr (TB) {
-      uint4 *dst = (uint4 *)(tbs + (((uint64_t)l * NBc + blk) * 32));
+      uint4 *dst = (uint4 *)(G.tbs + (((uint64_t)l * G.NBc + blk) * 32));
       dst[0] = make_uint4(tbw[0], tbw[1], tbw[2], tbw[3]);
       dst[1] = make_uint4(tbw[4], tbw[5], tbw[6], tbw[7]);
     }
@@ -303,7 +358,7 @@ __device__ void long_strip(const LongArgs &A, const Sync &y, const LongPair &P, 
   }
   if (out) y.publish(prog_value(s, ncols));
 
-  if (score) {  // global score H(m, n): the lane holding row m - 1, last column
+  if (G.score) {  // global score H(m, n): the lane holding row m - 1, last column
     const int r = P.m - 1 - row0;
     if (r >= 0 && r < R) {
       int v = 0;
@@ -325,7 +380,7 @@ __device__ __forceinline__ void load_table(const LongArgs &A, int8_t *tab) {
 }
 
 // Forward pass: one wave per workgroup, strips off the device queue.
-__global__ __launch_bounds__(64) void long_fwd_kernel(LongArgs A) {
+__global__ __launch_bounds__(64, 4) void long_fwd_kernel(LongArgs A) {
   extern __shared__ __align__(16) uint8_t lds[];
   int8_t *tab = (int8_t *)lds;
   int8_t *slot = (int8_t *)lds + 1024 + threadIdx.x * fill_slot_stride(1, A.Kp);
@@ -343,12 +398,17 @@ __global__ __launch_bounds__(64) void long_fwd_kernel(LongArgs A) {
     int32_t *ring = A.ring + P.ring_off;
     int *pc = A.prog + P.prog_off;
     const AgentSync y{s > 0 ? pc + s - 1 : pc, pc + s};
-    const int32_t *top = s > 0 ? ring + ((s - 1) & 1) * 2 * rl : nullptr;
-    int32_t *out = s + 1 < P.S ? ring + (s & 1) * 2 * rl : nullptr;
-    int32_t *ck = (A.ckpt_on && s % KC == KC - 1 && s + 1 < P.S) ? A.ckpt + P.ck_off + (uint64_t)(s / KC) * 2 * rl
-                                                                  : nullptr;
-    long_strip<AgentSync, false>(A, y, P, s, P.n, s > 0 ? TOP_RING : TOP_ROW0, top, out, ck, nullptr, 0, tab, slot,
-                                 s == P.S - 1, gave_up);
+    Region G{};
+    G.s = s; G.c_left = 0; G.ncols = P.n;
+    G.top_kind = s > 0 ? TOP_RING : TOP_ROW0;
+    G.top = s > 0 ? ring + ((s - 1) & 1) * 2 * rl : nullptr;
+    G.left = nullptr;
+    G.out = s + 1 < P.S ? ring + (s & 1) * 2 * rl : nullptr;
+    G.ck = (A.ckpt_on && s % KC == KC - 1 && s + 1 < P.S) ? A.ckpt + P.ck_off + (uint64_t)(s / KC) * 2 * rl : nullptr;
+    G.colck = (A.ckpt_on && P.n > CKX) ? A.colck + P.colck_off : nullptr;
+    G.tbs = nullptr; G.NBc = 0;
+    G.score = s == P.S - 1;
+    long_strip<AgentSync, false>(A, y, P, G, tab, slot, gave_up);
   }
 }
 
@@ -391,32 +451,39 @@ __global__ __launch_bounds__(64 * LW) void long_tb_kernel(LongArgs A) {
       rw.wp = (uint32_t *)(A.cig + pd.cig_off + pd.cig_cap);
       sh[1] = P.m; sh[2] = P.n; sh[3] = 0;
     }
-    for (int beta = (P.S - 1) / KC; beta >= 0; --beta) {
+    for (;;) {
       __syncthreads();
       const int i = sh[1], j = sh[2];
       if (i == 0 || j == 0) break;
-      const int s0 = beta * KC;
-      if (i <= s0 * LST) continue;  // the path is already above this block
-      const int s1 = min(s0 + KC, (i - 1) / LST + 1);
-      const int NBc = (j + GWAVE + 7) / 8;
+      // Region: rows of the block holding row i, columns c_left + 1 .. j.
+      const int beta = (i - 1) / (KC * LST);
+      const int s0 = beta * KC, top = s0 * LST, s1 = (i - 1) / LST + 1;
+      int c_left = j - (i - top) - LMARGIN;
+      c_left = c_left <= 0 ? 0 : (c_left / CKX) * CKX;
+      const int ncols = j - c_left;
+      const int NBc = (ncols + GWAVE + 7) / 8;
       if (threadIdx.x < LW) slots[threadIdx.x] = 0;
       __syncthreads();
       const int s = s0 + wv;
       if (s < s1) {
         const GroupSync y{slots + (s + LW - 1) % LW, slots + s % LW};
-        const int32_t *top = s > s0 ? wring + ((s - 1) & 1) * 2 * rl
-                                    : (beta ? A.ckpt + P.ck_off + (uint64_t)(beta - 1) * 2 * rl : nullptr);
-        const int kind = s > s0 ? TOP_RING : (beta ? TOP_CKPT : TOP_ROW0);
-        int32_t *out = s + 1 < s1 ? wring + (s & 1) * 2 * rl : nullptr;
-        uint8_t *tbs = wtb + (uint64_t)(s - s0) * GWAVE * NBc * 32;
-        long_strip<GroupSync, true>(A, y, P, s, j, kind, top, out, nullptr, tbs, NBc, tab, slot, false, gave_up);
+        Region G{};
+        G.s = s; G.c_left = c_left; G.ncols = ncols;
+        G.top_kind = s > s0 ? TOP_RING : (beta ? TOP_CKPT : TOP_ROW0);
+        G.top = s > s0 ? wring + ((s - 1) & 1) * 2 * rl
+                       : (beta ? A.ckpt + P.ck_off + (uint64_t)(beta - 1) * 2 * rl : nullptr);
+        G.left = c_left ? A.colck + P.colck_off + (uint64_t)(c_left / CKX - 1) * P.S * LST : nullptr;
+        G.out = s + 1 < s1 ? wring + (s & 1) * 2 * rl : nullptr;
+        G.ck = nullptr; G.colck = nullptr;
+        G.tbs = wtb + (uint64_t)(s - s0) * GWAVE * NBc * 32; G.NBc = NBc;
+        G.score = false;
+        long_strip<GroupSync, true>(A, y, P, G, tab, slot, gave_up);
       }
       __syncthreads();
       if (threadIdx.x == 0) {
         int ii = i, jj = j, st = sh[3];
-        const int top = s0 * LST;
-        while (ii > top && jj > 0) {
-          const int code = long_code(wtb, s0, NBc, ii, jj);
+        while (ii > top && jj > c_left) {
+          const int code = long_code(wtb, s0, NBc, ii, jj - c_left);
           const int g = st ? st : ((code & 4) ? ((code & 8) ? 2 : 1) : 0);
           const int xb = g == 1 ? (code & 1) : (g == 2 ? ((code >> 1) & 1) : 0);
           rw.emit(g, 1);

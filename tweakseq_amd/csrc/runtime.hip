@@ -402,8 +402,9 @@ struct ta_batch {
   LongPair *d_lp = nullptr;
   LongItem *d_litems = nullptr;
   int32_t *d_lring = nullptr, *d_lck = nullptr, *d_lprog = nullptr, *d_wring = nullptr;
+  int2 *d_lcol = nullptr;
   uint8_t *d_wtb = nullptr;
-  uint64_t lring_elems = 0, lck_elems = 0, wring_stride = 0, wtb_stride = 0;
+  uint64_t lring_elems = 0, lck_elems = 0, lcol_elems = 0, wring_stride = 0, wtb_stride = 0;
   int32_t lprog_n = 0, long_wgs = 0, spin_limit = 1 << 20;  // about a second of polling
   // timing
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -442,7 +443,7 @@ void ta_batch_destroy(ta_batch *B) {
   dfree(B->d_ptb); dfree(B->d_pairs); dfree(B->d_fo); dfree(B->d_res); dfree(B->d_cig);
   dfree(B->d_ctr); dfree(B->d_fo_init); dfree(B->d_order);
   free_rerun(B);
-  dfree(B->d_lp); dfree(B->d_litems); dfree(B->d_lring); dfree(B->d_lck); dfree(B->d_lprog);
+  dfree(B->d_lp); dfree(B->d_litems); dfree(B->d_lring); dfree(B->d_lck); dfree(B->d_lcol); dfree(B->d_lprog);
   dfree(B->d_wring); dfree(B->d_wtb);
   dfree(B->d_plain_tasks); dfree(B->d_prow); dfree(B->d_pcol); dfree(B->d_pring);
   dfree(B->d_ptbuf); dfree(B->d_pptb);
@@ -621,6 +622,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       if (L.S > 1) B->lring_elems += 4ull * L.rl;
       L.ck_off = B->lck_elems;
       if (p->traceback) B->lck_elems += (uint64_t)((L.S - 1) / KC) * 2 * L.rl;
+      L.colck_off = B->lcol_elems;
+      if (p->traceback) B->lcol_elems += (uint64_t)((L.n - 1) / CKX) * L.S * LST;
       L.prog_off = B->lprog_n;
       B->lprog_n += L.S;
       B->litem_first.push_back((int32_t)B->litems.size());
@@ -632,11 +635,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     }
     if (!lk.empty() && p->traceback) {
       B->wring_stride = 4ull * max_rl;
-      B->wtb_stride = (uint64_t)KC * GWAVE * ((max_n + GWAVE + 7) / 8) * 32;
-      // Direction planes of the traceback workgroups: at most 32 GB in all.
-      const uint64_t budget = 32ull << 30;
-      const int by_mem = (int)std::max<uint64_t>(1, budget / B->wtb_stride);
-      B->long_wgs = std::max(1, std::min({(int)lk.size(), 2 * ctx->cus, by_mem}));
+      B->wtb_stride = (uint64_t)KC * GWAVE * ((std::min(max_n, LREG) + GWAVE + 7) / 8) * 32;
+      B->long_wgs = std::max(1, std::min((int)lk.size(), 2 * ctx->cus));
     }
     if (const char *e = getenv("TWEAKALIGN_SPIN_LIMIT")) B->spin_limit = std::max(1, atoi(e));
   }
@@ -674,6 +674,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   rc = rc ? rc : upload(&B->d_litems, B->litems, st);
   rc = rc ? rc : dalloc(&B->d_lring, B->lring_elems);
   rc = rc ? rc : dalloc(&B->d_lck, B->lck_elems);
+  rc = rc ? rc : dalloc(&B->d_lcol, B->lcol_elems);
   rc = rc ? rc : dalloc(&B->d_lprog, (size_t)B->lprog_n);
   rc = rc ? rc : dalloc(&B->d_wring, B->wring_stride * B->long_wgs);
   rc = rc ? rc : dalloc(&B->d_wtb, B->wtb_stride * B->long_wgs);
@@ -694,9 +695,10 @@ int64_t batch_tb_estimate(const ta_params *p, const ta_pair *pairs, int32_t n) {
   for (int32_t k = 0; k < n; ++k) {
     const int la = pairs[k].la, lb = pairs[k].lb;
     if (la <= 0 || lb <= 0) continue;
-    if (long_pair(*p, la, lb)) {  // checkpoints + one traceback workgroup's planes
-      const int64_t rl = ((lb + 7) & ~7) + 80;
-      t += (int64_t)((ceil_div(la, LST) - 1) / KC) * 8 * rl + (int64_t)KC * GWAVE * ((lb + GWAVE + 7) / 8) * 32;
+    if (long_pair(*p, la, lb)) {  // row + column checkpoints, one traceback workgroup's planes
+      const int64_t rl = ((lb + 7) & ~7) + 80, S = ceil_div(la, LST);
+      t += (S - 1) / KC * 8 * rl + (int64_t)((lb - 1) / CKX) * S * LST * 8 +
+           (int64_t)KC * GWAVE * ((std::min(lb, LREG) + GWAVE + 7) / 8) * 32;
     } else {
       t += (int64_t)ceil_div(la, STRIP) * ceil_div(lb + G, 8) * 512;
     }
@@ -708,7 +710,7 @@ int ta_batch_info(ta_batch *B, int64_t *cells, int64_t *tb_bytes, int32_t *tasks
   if (!B) return TA_E_INVALID;
   if (cells) *cells = B->cells;
   if (tb_bytes)
-    *tb_bytes = (int64_t)(B->plan.tb_bytes + B->wtb_stride * B->long_wgs + 4 * B->lck_elems);
+    *tb_bytes = (int64_t)(B->plan.tb_bytes + B->wtb_stride * B->long_wgs + 4 * B->lck_elems + 8 * B->lcol_elems);
   if (tasks)
     *tasks = (int32_t)(B->plan.tasks2[0].size() + B->plan.tasks2[1].size() + B->plan.tasks1[0].size() +
                        B->plan.tasks1[1].size() + B->litems.size());
@@ -773,7 +775,7 @@ LongArgs long_args(ta_batch *B, int it0, int nit, int lp0, int nlp) {
   a.items = B->d_litems + it0; a.nitems = nit;
   a.ctr = B->d_ctr + 8;
   a.table = B->d_table; a.Kp = B->Kp; a.codes = B->d_codes;
-  a.ring = B->d_lring; a.ckpt = B->d_lck; a.prog = B->d_lprog;
+  a.ring = B->d_lring; a.ckpt = B->d_lck; a.prog = B->d_lprog; a.colck = B->d_lcol;
   a.wring = B->d_wring; a.wring_stride = B->wring_stride;
   a.wtb = B->d_wtb; a.wtb_stride = B->wtb_stride;
   a.pd = B->d_pairs; a.fo = B->d_fo; a.res = B->d_res; a.cig = B->d_cig;
