@@ -8,8 +8,11 @@ to CIGAR, for the 65,536-pair protein workload (log-normal lengths, median
 
 Extra fields: the plain form (one launch per pair) on a prefix of the same
 workload, the all-vs-all guide-tree batch (512 proteins, global, score
-only), the fill kernel's roofline, and the CPU oracle timed on a bounded
-sample of the same workload on this host's cores.
+only), one 300 x 320 pair (latency), the long-DNA batch (256 pairs of
+10,000-100,000 bases, global, linear-memory traceback), the fill kernel's
+roofline, and the CPU oracle timed on bounded samples of the same workloads
+on this host's cores.  The fill kernel's HBM traffic per launch comes from
+the rocprofv3 counter runs summarised in profiles/.
 """
 import argparse
 import json
@@ -47,23 +50,13 @@ def timed_steps(batch, steps, warmup):
 
 def cpu_baseline(workload, budget_s=12.0):
     from oracle import pyoracle as po
-    import numpy as np
     nthreads = min(os.cpu_count() or 1, 16)
     p = workload.params
     op = po.params(mode=p.mode, alphabet=p.alphabet, gap_open=p.gap_open, gap_extend=p.gap_extend,
                    match=p.match, mismatch=p.mismatch, traceback=p.traceback)
 
     def run(npairs):
-        seqs, a_off, la, b_off, lb = [], [], [], [], []
-        pos = 0
-        for k in range(npairs):
-            a, b = workload.pair_bytes(k)
-            ea, eb = po.encode(a, p.alphabet), po.encode(b, p.alphabet)
-            seqs += [ea, eb]
-            a_off.append(pos); la.append(len(ea)); pos += len(ea)
-            b_off.append(pos); lb.append(len(eb)); pos += len(eb)
-        codes = np.concatenate(seqs + [np.zeros(1, np.uint8)])
-        cells = sum(x * y for x, y in zip(la, lb))
+        codes, a_off, la, b_off, lb, cells = oracle_inputs(workload, 0, npairs)
         t0 = time.perf_counter()
         po.align_batch(op, codes, a_off, la, b_off, lb, nthreads=nthreads)
         return cells, time.perf_counter() - t0
@@ -75,6 +68,69 @@ def cpu_baseline(workload, budget_s=12.0):
     return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": nthreads, "kind": "port",
             "sample": "first %d of the %d pairs (%.3g cells), oracle/gotoh.c with traceback, OpenMP %d threads, %.1f s"
                       % (n, workload.npairs, cells, nthreads, dt)}
+
+
+def oracle_inputs(workload, first, count):
+    from oracle import pyoracle as po
+    import numpy as np
+    p = workload.params
+    seqs, a_off, la, b_off, lb = [], [], [], [], []
+    pos = 0
+    for k in range(first, first + count):
+        a, b = workload.pair_bytes(k)
+        ea, eb = po.encode(a, p.alphabet), po.encode(b, p.alphabet)
+        seqs += [ea, eb]
+        a_off.append(pos); la.append(len(ea)); pos += len(ea)
+        b_off.append(pos); lb.append(len(eb)); pos += len(eb)
+    codes = np.concatenate(seqs + [np.zeros(1, np.uint8)])
+    return codes, a_off, la, b_off, lb, sum(x * y for x, y in zip(la, lb))
+
+
+def cpu_single_pair(workload, repeats=1000):
+    """Config 1 on one core: median of `repeats` alignments with traceback."""
+    from oracle import pyoracle as po
+    p = workload.params
+    op = po.params(mode=p.mode, alphabet=p.alphabet, gap_open=p.gap_open, gap_extend=p.gap_extend,
+                   match=p.match, mismatch=p.mismatch, traceback=1)
+    a, b = workload.pair_bytes(0)
+    ea, eb = po.encode(a, p.alphabet), po.encode(b, p.alphabet)
+    ts = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        po.align(op, ea, eb)
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    med = ts[len(ts) // 2]
+    return {"value": round(len(ea) * len(eb) / med / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
+            "median_ms": round(med * 1e3, 4),
+            "sample": "one %dx%d pair, median of %d runs (Python call overhead included)" % (len(ea), len(eb), repeats)}
+
+
+def cpu_long_dna(workload, npairs=4):
+    """Long-DNA fill on the CPU: score only (a full-matrix traceback of a
+    100,000-base pair needs 10 GB), first `npairs` pairs, one thread each."""
+    from oracle import pyoracle as po
+    p = workload.params
+    op = po.params(mode=p.mode, alphabet=p.alphabet, gap_open=p.gap_open, gap_extend=p.gap_extend,
+                   match=p.match, mismatch=p.mismatch, traceback=0)
+    codes, a_off, la, b_off, lb, cells = oracle_inputs(workload, 0, npairs)
+    t0 = time.perf_counter()
+    po.align_batch(op, codes, a_off, la, b_off, lb, nthreads=npairs, with_cigar=False)
+    dt = time.perf_counter() - t0
+    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": npairs, "kind": "port",
+            "sample": "first %d of %d pairs (%.3g cells), score only, one thread per pair, %.1f s"
+                      % (npairs, workload.npairs, cells, dt)}
+
+
+def fill_traffic():
+    """HBM bytes per launch of the config-3 fill kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
+    path = os.path.join(ROOT, "profiles", "fill_traffic_local_65536.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -134,10 +190,16 @@ def main():
         "pairs_int32": n_i32,
     }
     ach = fill_gcups * OPS_PER_CELL[1]
+    tr = fill_traffic()
     out["roofline"] = {"bound": "valu-int32", "achieved": round(ach, 1), "peak": VALU_PEAK_GOPS,
-                       "unit": "Gop/s", "frac": round(ach / VALU_PEAK_GOPS, 4), "traffic": None,
+                       "unit": "Gop/s", "frac": round(ach / VALU_PEAK_GOPS, 4),
+                       "traffic": tr["bytes_per_launch"] if tr else None,
                        "ops_per_cell": OPS_PER_CELL[1],
-                       "note": "achieved = fill GCUPS x 10 recurrence ops per local cell; HBM traffic in profiles/"}
+                       "note": "achieved = fill GCUPS x 10 recurrence ops per local cell over the fill launches' "
+                               "HIP-event time; traffic = HBM bytes per fill launch (rocprofv3 FETCH_SIZE + "
+                               "WRITE_SIZE, copy-kernel calibrated, profiles/)"}
+    if tr:
+        out["roofline"]["hbm_gbs"] = round(tr["bytes_per_launch"] / (fill_avg / 1e3) / 1e9, 1)
 
     if not args.no_extra:
         # Plain form: one fill + one traceback launch per pair, on a prefix.
@@ -163,10 +225,53 @@ def main():
         out["allvsall_gcups"] = round(c2 / (wall2 / max(2, args.steps // 2)) / 1e9, 2)
         out["allvsall_fill_gcups"] = round(c2 / (sum(f2) / len(f2) / 1e3) / 1e9, 2)
         log("all-vs-all (130,816 global pairs, score only): %.1f GCUPS" % out["allvsall_gcups"])
+        # One 300 x 320 global pair with traceback: launch-to-result latency.
+        w1 = ta.Workload(1)
+        b1 = eng.batch(w1.pairs_struct(), w1.params)
+        lat = []
+        for k in range(220):
+            t0 = time.perf_counter()
+            b1.run(); b1.wait()
+            if k >= 20:
+                lat.append(time.perf_counter() - t0)
+        b1.close()
+        lat.sort()
+        out["single_pair_ms"] = round(lat[len(lat) // 2] * 1e3, 4)
+        out["single_pair_gcups"] = round(w1.cells() / lat[len(lat) // 2] / 1e9, 3)
+        log("single 300x320 pair: median %.3f ms" % out["single_pair_ms"])
+        # Long DNA pairs: strips pipelined across CUs + checkpointed traceback.
+        w5 = ta.Workload(5)
+        b5 = eng.batch(w5.pairs_struct(), w5.params)
+        c5 = b5.info()["cells"]
+        wall5, f5, _ = timed_steps(b5, 2, 1)
+        t5 = b5.timings()
+        b5.close()
+        out["long_dna_gcups"] = round(c5 / (wall5 / 2) / 1e9, 1)
+        out["long_dna_ms"] = round(wall5 / 2 * 1e3, 1)
+        out["long_dna_fill_gcups"] = round(c5 / (sum(f5) / len(f5) / 1e3) / 1e9, 1)
+        out["long_dna_traceback_ms"] = round(t5[2], 1)
+        log("long DNA (256 pairs, %.3g cells): %.1f ms/run, %.1f GCUPS (forward %.1f GCUPS, traceback %.1f ms)"
+            % (c5, out["long_dna_ms"], out["long_dna_gcups"], out["long_dna_fill_gcups"], t5[2]))
+
+    if not args.no_extra:
+        # Measured VALU throughput (instruction-kind microbenchmark) beside the theoretical peak.
+        import subprocess
+        tool = os.path.join(ROOT, "tweakseq_amd", "tweakalign-valu-peak")
+        if os.path.exists(tool):
+            eng.close()
+            r = subprocess.run([tool], capture_output=True, text=True, timeout=120)
+            if r.returncode == 0:
+                out["valu_peak_measured"] = json.loads(r.stdout.strip().splitlines()[-1])
+            eng = ta.Engine(0)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w3)
         log("cpu baseline:", out["cpu_baseline"])
+        if not args.no_extra:
+            out["cpu_single_pair"] = cpu_single_pair(ta.Workload(1))
+            out["cpu_long_dna"] = cpu_long_dna(ta.Workload(5))
+            log("cpu single pair:", out["cpu_single_pair"])
+            log("cpu long dna:", out["cpu_long_dna"])
     eng.close()
     print(json.dumps(out), flush=True)
 

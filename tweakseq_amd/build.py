@@ -2,6 +2,8 @@
 
   libtweakalign.so  HIP kernels (gfx950) + host runtime + C ABI
   tweakalign        CLI honouring the external-aligner "-in FILE -out FILE" contract
+  tweakalign-serve-bench   open-loop serving benchmark
+  tweakalign-valu-peak     measured VALU throughput of the fill's instruction kinds
 
 Usage: python -m tweakseq_amd.build   (or build() from __graft_entry__)
 """
@@ -60,6 +62,7 @@ def build_tools():
             continue
         run(["g++", "-O2", "-std=c++17", "-Wall", "-pthread", src, "-o", out, "-L" + HERE, "-ltweakalign",
              "-Wl,-rpath,$ORIGIN"])
+    run(["hipcc"] + HIP_FLAGS + [os.path.join(CSRC, "valu_peak.hip"), "-o", os.path.join(HERE, "tweakalign-valu-peak")])
 
 
 def build():

@@ -23,6 +23,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace ta {
@@ -267,105 +269,117 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     uint32_t tbw[16];
     int rbH[8], rbF[8];
 
+    // The 8 steps, in two versions: with the per-step column test, and
+    // without it for blocks where every lane of the wave stays inside its
+    // task's columns (most of a strip), which also spares the merges of the
+    // registers a masked step leaves alone.
+    auto steps = [&](auto full_tag) {
+      constexpr bool FULL = decltype(full_tag)::value;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int t = 8 * b + q;
-      const int j = t - l;
-      // This step's residues and profile rows were fetched last step; start
-      // the LDS reads of the next step now (lane l's next residue is lane
-      // l-1's current one).
-      const uint2 pa = pa_n;
-      const uint2 pb = pb_n;
-      const int cthis = ccur;
-      const int code1 = q < 7 ? (int)((ccw[(q + 1) >> 1] >> (16 * ((q + 1) & 1))) & 0xffff) : (int)next_first;
-      ccur = shr1<GG>(code1, cthis);
-      pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
-      if constexpr (P == 2) pb_n = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
-      else pb_n = pa_n;
-      int topH, topF;
-      if (s == 0) {
-        topH = L::bits(L::splat(LOCAL ? 0 : A.open + t * A.ext));
-        topF = L::bits(vneg);
-      } else {
-        topH = (int)rhw[q];
-        topF = (int)rfw[q];
-      }
-      const V Hup = L::unbits(shr1<GG>(topH, L::bits(Hbot)));
-      const V Fup = L::unbits(shr1<GG>(topF, L::bits(Fbot)));
+      for (int q = 0; q < 8; ++q) {
+        const int t = 8 * b + q;
+        const int j = t - l;
+        // This step's residues and profile rows were fetched last step; start
+        // the LDS reads of the next step now (lane l's next residue is lane
+        // l-1's current one).
+        const uint2 pa = pa_n;
+        const uint2 pb = pb_n;
+        const int cthis = ccur;
+        const int code1 = q < 7 ? (int)((ccw[(q + 1) >> 1] >> (16 * ((q + 1) & 1))) & 0xffff) : (int)next_first;
+        ccur = shr1<GG>(code1, cthis);
+        pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
+        if constexpr (P == 2) pb_n = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
+        else pb_n = pa_n;
+        int topH, topF;
+        if (s == 0) {
+          topH = L::bits(L::splat(LOCAL ? 0 : A.open + t * A.ext));
+          topF = L::bits(vneg);
+        } else {
+          topH = (int)rhw[q];
+          topF = (int)rfw[q];
+        }
+        const V Hup = L::unbits(shr1<GG>(topH, L::bits(Hbot)));
+        const V Fup = L::unbits(shr1<GG>(topF, L::bits(Fbot)));
 
-      if (j >= 0 && j < n) {
-        const uint32_t pas_x = pa.x << 8, pas_y = pa.y << 8, pbs_x = pb.x << 8, pbs_y = pb.y << 8;
-        const uint32_t cj = 0xffffu - (uint32_t)j;
+        if (FULL || (j >= 0 && j < n)) {
+          const uint32_t pas_x = pa.x << 8, pas_y = pa.y << 8, pbs_x = pb.x << 8, pbs_y = pb.y << 8;
+          const uint32_t cj = 0xffffu - (uint32_t)j;
 
-        V hd = hd0;
-        V HoU = L::add(Hup, vopen);
-        V F = Fup;
-        uint32_t xc = 0, yc = 0, acc = 0;
+          V hd = hd0;
+          V HoU = L::add(Hup, vopen);
+          V F = Fup;
+          uint32_t xc = 0, yc = 0, acc = 0;
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-          V Sk;
-          if constexpr (P == 2) {
-            const bool hiw = k >= 4;
-            Sk = s2(prof2(hiw ? pa.y : pa.x, hiw ? pb.y : pb.x, hiw ? pas_y : pas_x,
-                          hiw ? pbs_y : pbs_x, k));
-          } else {
-            Sk = L::unbits(prof1(pa, k));
+          for (int k = 0; k < R; ++k) {
+            V Sk;
+            if constexpr (P == 2) {
+              const bool hiw = k >= 4;
+              Sk = s2(prof2(hiw ? pa.y : pa.x, hiw ? pb.y : pb.x, hiw ? pas_y : pas_x,
+                            hiw ? pbs_y : pbs_x, k));
+            } else {
+              Sk = L::unbits(prof1(pa, k));
+            }
+            const V HdS = L::add(hd, Sk);
+            const V Eext = L::add(E[k], vext);
+            const V En = L::vmax(HoL[k], Eext);
+            const V Fext = L::add(F, vext);
+            const V Fn = L::vmax(HoU, Fext);
+            const V EF = L::vmax(En, Fn);
+            V Hn = L::vmax(HdS, EF);
+            if constexpr (LOCAL) Hn = L::vmax(Hn, vzero);
+            const V Hon = L::add(Hn, vopen);
+            if constexpr (TB) {
+              const uint32_t dE = (uint32_t)L::bits(L::sub(HoL[k], Eext));
+              const uint32_t dF = (uint32_t)L::bits(L::sub(HoU, Fext));
+              const uint32_t dD = (uint32_t)L::bits(L::sub(HdS, EF));
+              const uint32_t dX = (uint32_t)L::bits(L::sub(En, Fn));
+              if constexpr (P == 2) {
+                // sign bytes (0x00 / 0xFF) of [e lo, e hi, f lo, f hi] and
+                // [d lo, d hi, x lo, x hi]; row k keeps bit k of each byte
+                const uint32_t X = __builtin_amdgcn_perm(dF, dE, 0x0B0A0908u);
+                const uint32_t Y = __builtin_amdgcn_perm(dX, dD, 0x0B0A0908u);
+                const uint32_t mk = 0x01010101u << k;
+                xc |= X & mk;
+                yc |= Y & mk;
+              } else {
+                acc = __builtin_amdgcn_alignbit(acc, dE, 31);
+                acc = __builtin_amdgcn_alignbit(acc, dF, 31);
+                acc = __builtin_amdgcn_alignbit(acc, dD, 31);
+                acc = __builtin_amdgcn_alignbit(acc, dX, 31);
+              }
+            }
+            if constexpr (LOCAL) {
+              if constexpr (P == 2) {
+                const uint32_t hu = u32(Hn);
+                key0[k] = max(key0[k], (hu << 16) | cj);
+                key1[k] = max(key1[k], (hu & 0xffff0000u) | cj);
+              } else {
+                const int hv = L::bits(Hn);
+                if (hv > rmax[k]) { rmax[k] = hv; rcol[k] = j; }
+              }
+            }
+            hd = H[k];
+            H[k] = Hn;
+            E[k] = En;
+            HoL[k] = Hon;
+            F = Fn;
+            HoU = Hon;
           }
-          const V HdS = L::add(hd, Sk);
-          const V Eext = L::add(E[k], vext);
-          const V En = L::vmax(HoL[k], Eext);
-          const V Fext = L::add(F, vext);
-          const V Fn = L::vmax(HoU, Fext);
-          const V EF = L::vmax(En, Fn);
-          V Hn = L::vmax(HdS, EF);
-          if constexpr (LOCAL) Hn = L::vmax(Hn, vzero);
-          const V Hon = L::add(Hn, vopen);
+          Hbot = H[R - 1];
+          Fbot = F;
+          hd0 = Hup;
           if constexpr (TB) {
-            const uint32_t dE = (uint32_t)L::bits(L::sub(HoL[k], Eext));
-            const uint32_t dF = (uint32_t)L::bits(L::sub(HoU, Fext));
-            const uint32_t dD = (uint32_t)L::bits(L::sub(HdS, EF));
-            const uint32_t dX = (uint32_t)L::bits(L::sub(En, Fn));
-            if constexpr (P == 2) {
-              const uint32_t X = __builtin_amdgcn_perm(dF, dE, 0x07050301u);
-              const uint32_t Y = __builtin_amdgcn_perm(dX, dD, 0x07050301u);
-              const uint32_t mk = 0x01010101u << k;
-              xc = ((X >> (7 - k)) & mk) | (xc & ~mk);
-              yc = ((Y >> (7 - k)) & mk) | (yc & ~mk);
-            } else {
-              acc = __builtin_amdgcn_alignbit(acc, dE, 31);
-              acc = __builtin_amdgcn_alignbit(acc, dF, 31);
-              acc = __builtin_amdgcn_alignbit(acc, dD, 31);
-              acc = __builtin_amdgcn_alignbit(acc, dX, 31);
-            }
+            if constexpr (P == 2) { tbw[2 * q] = xc; tbw[2 * q + 1] = yc; }
+            else tbw[q] = acc;
           }
-          if constexpr (LOCAL) {
-            if constexpr (P == 2) {
-              const uint32_t hu = u32(Hn);
-              key0[k] = max(key0[k], (hu << 16) | cj);
-              key1[k] = max(key1[k], (hu & 0xffff0000u) | cj);
-            } else {
-              const int hv = L::bits(Hn);
-              if (hv > rmax[k]) { rmax[k] = hv; rcol[k] = j; }
-            }
-          }
-          hd = H[k];
-          H[k] = Hn;
-          E[k] = En;
-          HoL[k] = Hon;
-          F = Fn;
-          HoU = Hon;
         }
-        Hbot = H[R - 1];
-        Fbot = F;
-        hd0 = Hup;
-        if constexpr (TB) {
-          if constexpr (P == 2) { tbw[2 * q] = xc; tbw[2 * q + 1] = yc; }
-          else tbw[q] = acc;
-        }
+        rbH[q] = L::bits(Hbot);
+        rbF[q] = L::bits(Fbot);
       }
-      rbH[q] = L::bits(Hbot);
-      rbF[q] = L::bits(Fbot);
-    }
+    };
+    const bool inside = 8 * b >= GG - 1 && 8 * b + 8 <= n;
+    if (__all(inside)) steps(std::true_type{});
+    else steps(std::false_type{});
 
     // ---- block outputs: direction planes (all lanes), ring (lane 15).
     if constexpr (TB) {

@@ -97,7 +97,9 @@ Alphabet make_alphabet(int alphabet, int match, int mismatch) {
     }
   }
   A.maxS = -128;
-  for (int x = 0; x < A.K * A.K; ++x) A.maxS = std::max<int>(A.maxS, A.S[x]);
+  for (int x = 0; x < A.K * A.K; ++x) {
+    A.maxS = std::max<int>(A.maxS, A.S[x]);
+  }
   return A;
 }
 
@@ -299,6 +301,15 @@ struct Plan {
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
+// Resident one-wave fill workgroups per CU: one per SIMD.  A fill wave keeps
+// its SIMD's VALU busy by itself (the recurrence's dependent chain per row is
+// a fraction of a step's instructions), and a second wave on the SIMD only
+// halves the speed of the long tasks that end the batch (measured on the
+// 65,536-pair local batch: 9.1 ms at 4 waves per CU, 9.9 at 8, 10.5 at 12).
+int fill_waves_per_cu(int P, int Kp, int want = 4) {
+  return std::max(1, std::min(want, (160 * 1024) / fill_lds_bytes(P, Kp)));
+}
+
 // Append one task (pair lo / hi; hi == lo for a single) to the plan, swept
 // by groups of GG lanes.
 void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const std::vector<HostPair> &hp,
@@ -379,6 +390,7 @@ struct ta_batch {
   // device
   uint8_t *d_codes = nullptr;
   int8_t *d_table = nullptr;
+  int fill_waves = 0;             // resident fill waves per CU (0: one per SIMD)
   Task *d_tasks2[2] = {nullptr, nullptr}, *d_tasks1[2] = {nullptr, nullptr};
   uint16_t *d_rowcodes = nullptr, *d_colcodes = nullptr;
   uint32_t *d_ring = nullptr;
@@ -480,6 +492,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   B->Kp = B->alph.K + 1;
   const bool local = p->mode == TA_LOCAL;
   B->trace = local || p->traceback;
+  if (const char *e = getenv("TWEAKALIGN_FILL_WAVES")) B->fill_waves = std::max(0, atoi(e));
 
   // Encode (sequences shared between pairs are encoded once).
   std::unordered_map<const char *, std::pair<uint64_t, int32_t>> seen;
@@ -584,7 +597,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     work += g16_cost(sh.first, sh.second);
   }
   for (int k : i32) work += g16_cost(B->hp[k].la, B->hp[k].lb);
-  const double slots = (double)ctx->cus * std::max(1, (160 * 1024) / fill_lds_bytes(2, B->Kp)) * 4;
+  const int wpc = fill_waves_per_cu(2, B->Kp, B->fill_waves ? B->fill_waves : 4);
+  const double slots = (double)ctx->cus * wpc * 4;
   const double mean_slot = work / slots;
   for (auto &c : couples) {
     const bool single = c.second < 0;
@@ -734,7 +748,7 @@ int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntas
   a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.sat_limit = 32767 - 2 * B->alph.maxS;
   const int lds = fill_lds_bytes(P, B->Kp);
-  const int per_cu = std::max(1, std::min(8, (160 * 1024) / lds));
+  const int per_cu = fill_waves_per_cu(P, B->Kp, B->fill_waves ? B->fill_waves : 4);
   int groups = ntasks[0] + (ntasks[1] + 3) / 4;
   if (max_groups > 0) groups = std::min(groups, max_groups);
   groups = std::max(1, std::min(groups, ctx->cus * per_cu));
