@@ -80,21 +80,28 @@ typedef struct {
   int32_t flags;               /* TA_F_* */
 } ta_result;
 
+/* Message of a TA_E_* code (the reference routes a failed aligner's stderr to
+ * its message window, tweakseq/UI/SeqEditMainWin.cpp:829-834). */
 const char *ta_strerror(int code);
 
-/* One context per device per host thread. */
+/* One context per device per host thread.  Replaces the per-project tool
+ * objects (tweakseq/Core/Project.cpp:1142-1152, deleted at :84-86). */
 int ta_create(ta_ctx **out, int device);
 void ta_destroy(ta_ctx *ctx);
 
 /* Synchronous batch: encode, upload, align on the GPU, return results and
- * CIGARs (packed back to back in cigar_buf).  TA_E_SPACE with *used set when
+ * CIGARs (packed back to back in cigar_buf).  Replaces one aligner run:
+ * makeCommand + QProcess start + the success test on exit status
+ * (tweakseq/Core/AlignmentTool.h:54, tweakseq/UI/SeqEditMainWin.cpp:1616-1663,
+ * :836-861).  TA_E_SPACE with *used set when
  * cap is too small (results are still written).  cigar_buf may be NULL when
  * p->traceback == 0. */
 int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n,
                    ta_result *out, char *cigar_buf, int64_t cap, int64_t *used);
 
 /* Device-resident batch: inputs encoded and uploaded once, the alignment
- * run as often as wanted with inputs already in HBM. */
+ * run as often as wanted with inputs already in HBM.  No counterpart in the
+ * reference (it starts one process per job). */
 typedef struct ta_batch ta_batch;
 int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n,
                     ta_batch **out);
@@ -114,7 +121,11 @@ void ta_batch_destroy(ta_batch *b);
 
 /* Serving: continuous batching.  ta_submit copies the pairs and returns at
  * once; requests that arrive while others run are admitted as traceback
- * memory (budget set by ta_serve_config) frees up, and run concurrently. */
+ * memory (budget set by ta_serve_config) frees up, and run concurrently.
+ * ta_submit / ta_poll replace the asynchronous QProcess start and its
+ * finished signal (tweakseq/UI/SeqEditMainWin.cpp:1660, :836-861), which
+ * allow one job per window (:1618-1624); ta_cancel replaces the kill of
+ * alignmentStop (:803-812).  ta_serve_config has no counterpart. */
 int ta_serve_config(ta_ctx *ctx, int64_t tb_budget_bytes, int32_t max_inflight);
 int ta_submit(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n,
               uint64_t *ticket);
@@ -122,9 +133,12 @@ int ta_poll(ta_ctx *ctx, uint64_t ticket, ta_result *out, char *cigar_buf, int64
             int64_t *used);                          /* TA_OK done, TA_E_PENDING, <0 error */
 int ta_cancel(ta_ctx *ctx, uint64_t ticket);
 
-/* Substitution table in use for an alphabet (row-major K x K, int8). */
+/* Substitution table in use for an alphabet (row-major K x K, int8): BLOSUM62
+ * of tweakseq/Core/Annotations/Consensus.cpp:34-59 for TA_PROTEIN. */
 int ta_matrix(int32_t alphabet, int32_t match, int32_t mismatch, int8_t *out, int32_t *K);
-/* Letter -> code, as the engine encodes input (TA_E_ALPHABET on others). */
+/* Letter -> code, as the engine encodes input (TA_E_ALPHABET on others): the
+ * letter map of Consensus.cpp:61-69; gaps must be removed first (the
+ * residues tweakseq exports keep them, tweakseq/Core/Sequence.cpp:57-69). */
 int ta_encode(int32_t alphabet, const char *s, int32_t len, uint8_t *out);
 
 #ifdef __cplusplus

@@ -542,6 +542,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       h.path = 3;
     } else if (p->lanes == TA_LANES_I32) {
       h.path = 1;
+    } else if (local && q.lb > 65535) {
+      h.path = 1;  // packed end-cell keys hold the column in 16 bits
     } else if (local) {
       h.path = 2;
       if ((long)std::min(q.la, q.lb) * B->alph.maxS > 32767 - 2L * B->alph.maxS) B->may_saturate = true;
