@@ -111,9 +111,15 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *b, int32_t first, int32_t count);
 int ta_batch_wait(ta_ctx *ctx, ta_batch *b);
 int ta_batch_results(ta_ctx *ctx, ta_batch *b, ta_result *out, char *cigar_buf,
                      int64_t cap, int64_t *used);
+/* Replay the batch's runs as one captured hipGraph (on != 0): the first run
+ * captures init, fill and traceback kernels, later runs launch the graph.
+ * Batches that may need the int32 rerun keep launching eagerly.  No
+ * counterpart in the reference. */
+int ta_batch_graph(ta_batch *b, int32_t on);
 /* Kernel timings of the last ta_batch_run, milliseconds (HIP events on the
- * context stream): [0] fill kernel(s), [1] int32 rerun, [2] traceback,
- * [3] whole run.  Returns the number of values written. */
+ * batch stream): [0] fill kernel(s), [1] int32 rerun, [2] traceback,
+ * [3] whole run; [0]-[2] are -1 after a captured (graph) run.  Returns the
+ * number of values written. */
 int ta_batch_timings(ta_batch *b, double *ms, int32_t cap);
 /* Cells (sum of la*lb) and direction-plane bytes of the batch. */
 int ta_batch_info(ta_batch *b, int64_t *cells, int64_t *tb_bytes, int32_t *tasks);
@@ -132,6 +138,29 @@ int ta_submit(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n,
 int ta_poll(ta_ctx *ctx, uint64_t ticket, ta_result *out, char *cigar_buf, int64_t cap,
             int64_t *used);                          /* TA_OK done, TA_E_PENDING, <0 error */
 int ta_cancel(ta_ctx *ctx, uint64_t ticket);
+
+/* Autotuning (no counterpart in the reference).  A tuning is the launch
+ * shape the batch runtime uses for one kind of batch: mode, traceback,
+ * alphabet and length bucket (ta_tune_bucket). */
+typedef struct {
+  int32_t waves_per_cu;  /* resident fill waves per CU (4 = one per SIMD); 0 = default */
+  int32_t lanes;         /* TA_LANES_AUTO | TA_LANES_I16 | TA_LANES_I32 (an explicit
+                            ta_params.lanes takes precedence) */
+  int32_t wave_tasks;    /* 0: whole-wave groups by the cost rule; 1: for every task
+                            taller than one 16-lane strip */
+} ta_tuning;
+/* Length bucket of a batch by its median max(la, lb): 0 (< 256), 1 (< 1024),
+ * 2 (< 4096), 3. */
+int ta_tune_bucket(const ta_pair *pairs, int32_t n);
+/* Times candidate tunings on these pairs (best of `repeats` runs each),
+ * stores the fastest in the context for this kind of batch and returns it;
+ * ta_batch_create / ta_align_batch / serving on this context then use it. */
+int ta_autotune(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, int32_t repeats,
+                ta_tuning *best, double *best_ms);
+/* Set / read the context's tuning for (p's mode, traceback, alphabet, bucket)
+ * directly (ta_tune_get: TA_E_INVALID when none is stored). */
+int ta_tune_set(ta_ctx *ctx, const ta_params *p, int32_t bucket, const ta_tuning *t);
+int ta_tune_get(ta_ctx *ctx, const ta_params *p, int32_t bucket, ta_tuning *t);
 
 /* Substitution table in use for an alphabet (row-major K x K, int8): BLOSUM62
  * of tweakseq/Core/Annotations/Consensus.cpp:34-59 for TA_PROTEIN. */

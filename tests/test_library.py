@@ -107,3 +107,12 @@ def test_no_gpu_fails_loudly():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT,
                          env={**os.environ, "HIP_VISIBLE_DEVICES": "-1", "ROCR_VISIBLE_DEVICES": "-1"})
     assert "refused" in out.stdout
+
+
+def test_tune_bucket_by_median_length():
+    from tweakseq_amd.engine import _Pairs
+    L = ta.lib()
+    for lens, want in (([10, 20, 300], 0), ([300, 300, 10], 1), ([2000, 5, 1500], 2), ([5000, 9000, 1], 3)):
+        ps = _Pairs([("A" * n, "C" * max(1, n // 2)) for n in lens])
+        assert L.ta_tune_bucket(ps.arr, ps.n) == want, lens
+    assert L.ta_tune_bucket(None, 0) == 0

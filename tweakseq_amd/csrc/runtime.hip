@@ -363,9 +363,24 @@ double task_cost(const Task &T) { return (double)T.S * T.NB; }
 // sweep: more than 0.75x the mean work of a 16-lane slot (a list-scheduling
 // simulation of the 65,536-pair log-normal batch gives 99% slot efficiency
 // there, 68% at 1.5x), and tall enough (>= 384 rows) not to waste most of a
-// 512-row strip.
-int pick_gg(int mmax, int n, double mean_slot) {
+// 512-row strip.  With fewer tasks than resident waves (`few`) the run takes
+// as long as one task's sweep, so every task taller than one 16-lane strip
+// gets a whole wave (a lone 300 x 320 pair: 384 steps instead of 1,008).
+int pick_gg(int mmax, int n, double mean_slot, bool few) {
+  if (mmax <= G * R) return G;
+  if (few) return GWAVE;
   return (mmax >= 3 * G * R && g16_cost(mmax, n) > 0.75 * mean_slot) ? GWAVE : G;
+}
+
+// Start of a run: work-queue counters to 0, fill outputs to their initial
+// values (empty pairs keep theirs), long-pair progress counters to 0.  A
+// kernel rather than memset / memcpy nodes, so a captured run is all kernels.
+__global__ void init_kernel(int32_t *ctr, int nctr, FillOut *fo, const FillOut *fo_init, int n, int32_t *prog,
+                            int nprog) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  for (int k = tid; k < nctr; k += stride) ctr[k] = 0;
+  for (int k = tid; k < n; k += stride) fo[k] = fo_init[k];
+  for (int k = tid; k < nprog; k += stride) prog[k] = 0;
 }
 
 }  // namespace
@@ -390,7 +405,8 @@ struct ta_batch {
   // device
   uint8_t *d_codes = nullptr;
   int8_t *d_table = nullptr;
-  int fill_waves = 0;             // resident fill waves per CU (0: one per SIMD)
+  int fill_waves = 0;             // resident fill waves per CU (0: default_waves)
+  bool wave_tasks = false;        // whole-wave groups for every multi-strip task
   Task *d_tasks2[2] = {nullptr, nullptr}, *d_tasks1[2] = {nullptr, nullptr};
   uint16_t *d_rowcodes = nullptr, *d_colcodes = nullptr;
   uint32_t *d_ring = nullptr;
@@ -418,6 +434,10 @@ struct ta_batch {
   uint8_t *d_wtb = nullptr;
   uint64_t lring_elems = 0, lck_elems = 0, lcol_elems = 0, wring_stride = 0, wtb_stride = 0;
   int32_t lprog_n = 0, long_wgs = 0, spin_limit = 1 << 20;  // about a second of polling
+  // captured run
+  bool graph_on = false, phases_timed = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   // timing
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double ms[4] = {0, 0, 0, 0};
@@ -438,6 +458,13 @@ bool global_i16_safe(const ta_params &p, int maxS, int la, int lb) {
   const long hi = (long)std::min(la, lb) * maxS + maxS;
   return lo < 16000 && hi < 32000;
 }
+
+// Resident fill waves per CU a batch asks for: its tuning, else one per SIMD
+// when direction planes are written (those kernels keep a SIMD busy alone and
+// a second wave halves the speed of the batch's longest tasks), else two per
+// SIMD (the score-only kernels are leaner and latency-exposed: packed int16
+// ops issue at 77% of peak with one wave, 90% with two).
+int waves_wanted(const ta_batch *B) { return B->fill_waves ? B->fill_waves : (B->trace ? 4 : 8); }
 
 void free_rerun(ta_batch *B) {
   dfree(B->d_tasks_r); dfree(B->d_rowcodes_r); dfree(B->d_colcodes_r);
@@ -461,6 +488,8 @@ void ta_batch_destroy(ta_batch *B) {
   dfree(B->d_ptbuf); dfree(B->d_pptb);
   for (auto &e : B->ev)
     if (e) (void)hipEventDestroy(e);
+  if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
   delete B;
 }
 
@@ -469,7 +498,7 @@ int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
 }
 
 int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, hipStream_t stream,
-                    ta_batch **out) {
+                    ta_batch **out, const ta_tuning *force) {
   if (!ctx || !p || !out || n < 0 || (n > 0 && !pairs)) return TA_E_INVALID;
   *out = nullptr;
   if (p->mode != TA_GLOBAL && p->mode != TA_LOCAL) return TA_E_INVALID;
@@ -492,7 +521,19 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   B->Kp = B->alph.K + 1;
   const bool local = p->mode == TA_LOCAL;
   B->trace = local || p->traceback;
-  if (const char *e = getenv("TWEAKALIGN_FILL_WAVES")) B->fill_waves = std::max(0, atoi(e));
+  // Launch shape: the autotuner's candidate, else the context's tuning for
+  // this kind of batch, else the defaults.
+  ta_tuning tu{0, TA_LANES_AUTO, 0};
+  if (force) {
+    tu = *force;
+  } else {
+    std::lock_guard<std::mutex> g(ctx->tune_mu);
+    auto it = ctx->tune.find(tune_key(p, ta_tune_bucket(pairs, n)));
+    if (it != ctx->tune.end()) tu = it->second;
+  }
+  B->fill_waves = tu.waves_per_cu;
+  B->wave_tasks = tu.wave_tasks != 0;
+  const int lanes = p->lanes != TA_LANES_AUTO ? p->lanes : tu.lanes;
 
   // Encode (sequences shared between pairs are encoded once).
   std::unordered_map<const char *, std::pair<uint64_t, int32_t>> seen;
@@ -540,7 +581,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       }
     } else if (long_pair(*p, q.la, q.lb)) {
       h.path = 3;
-    } else if (p->lanes == TA_LANES_I32) {
+    } else if (lanes == TA_LANES_I32) {
       h.path = 1;
     } else if (local && q.lb > 65535) {
       h.path = 1;  // packed end-cell keys hold the column in 16 bits
@@ -599,18 +640,19 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     work += g16_cost(sh.first, sh.second);
   }
   for (int k : i32) work += g16_cost(B->hp[k].la, B->hp[k].lb);
-  const int wpc = fill_waves_per_cu(2, B->Kp, B->fill_waves ? B->fill_waves : 4);
+  const int wpc = fill_waves_per_cu(2, B->Kp, waves_wanted(B));
   const double slots = (double)ctx->cus * wpc * 4;
   const double mean_slot = work / slots;
+  const bool few = B->wave_tasks || couples.size() + i32.size() <= (size_t)ctx->cus * wpc;
   for (auto &c : couples) {
     const bool single = c.second < 0;
     const int hi = single ? c.first : c.second;
     auto sh = shape(c.first, hi);
-    const int GG = pick_gg(sh.first, sh.second, mean_slot);
+    const int GG = pick_gg(sh.first, sh.second, mean_slot, few);
     add_task(pl, pl.tasks2[GG == GWAVE ? 0 : 1], 2, GG, B->trace, B->hp, B->codes, c.first, hi, single, B->Kp);
   }
   for (int k : i32) {
-    const int GG = pick_gg(B->hp[k].la, B->hp[k].lb, mean_slot);
+    const int GG = pick_gg(B->hp[k].la, B->hp[k].lb, mean_slot, few);
     add_task(pl, pl.tasks1[GG == GWAVE ? 0 : 1], 1, GG, B->trace, B->hp, B->codes, k, k, true, B->Kp);
   }
   auto by_cost = [](const Task &x, const Task &y) { return task_cost(x) > task_cost(y); };
@@ -750,7 +792,7 @@ int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntas
   a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.sat_limit = 32767 - 2 * B->alph.maxS;
   const int lds = fill_lds_bytes(P, B->Kp);
-  const int per_cu = fill_waves_per_cu(P, B->Kp, B->fill_waves ? B->fill_waves : 4);
+  const int per_cu = fill_waves_per_cu(P, B->Kp, waves_wanted(B));
   int groups = ntasks[0] + (ntasks[1] + 3) / 4;
   if (max_groups > 0) groups = std::min(groups, max_groups);
   groups = std::max(1, std::min(groups, ctx->cus * per_cu));
@@ -805,10 +847,12 @@ LongArgs long_args(ta_batch *B, int it0, int nit, int lp0, int nlp) {
 // Forward pass of long pairs: counters cleared, then one wave per workgroup
 // pulling strips.  Items index pairs relative to lp0 = 0, so a subset of
 // items must keep the full pair array.
-int launch_long_fwd(ta_ctx *ctx, ta_batch *B, int it0, int nit) {
+int launch_long_fwd(ta_ctx *ctx, ta_batch *B, int it0, int nit, bool clear) {
   if (nit <= 0) return TA_OK;
-  HIPCHK(hipMemsetAsync(B->d_ctr + 8, 0, 3 * sizeof(int32_t), B->stream));
-  HIPCHK(hipMemsetAsync(B->d_lprog, 0, sizeof(int32_t) * B->lprog_n, B->stream));
+  if (clear) {  // the per-pair (plain) form; a batch run starts with init_kernel
+    HIPCHK(hipMemsetAsync(B->d_ctr + 8, 0, 3 * sizeof(int32_t), B->stream));
+    HIPCHK(hipMemsetAsync(B->d_lprog, 0, sizeof(int32_t) * B->lprog_n, B->stream));
+  }
   LongArgs a = long_args(B, it0, nit, 0, (int)B->longs.size());
   const int lds = long_fwd_lds(B->Kp);
   const int per_cu = std::max(1, std::min(16, (160 * 1024) / lds));  // 4 waves per SIMD (117 VGPRs)
@@ -820,9 +864,9 @@ int launch_long_fwd(ta_ctx *ctx, ta_batch *B, int it0, int nit) {
 }
 
 // Traceback of long pairs [lp0, lp0 + nlp): one workgroup of LW waves per pair.
-int launch_long_tb(ta_ctx *ctx, ta_batch *B, int lp0, int nlp) {
+int launch_long_tb(ta_ctx *ctx, ta_batch *B, int lp0, int nlp, bool clear) {
   if (nlp <= 0 || !B->p.traceback) return TA_OK;
-  HIPCHK(hipMemsetAsync(B->d_ctr + 9, 0, sizeof(int32_t), B->stream));
+  if (clear) HIPCHK(hipMemsetAsync(B->d_ctr + 9, 0, sizeof(int32_t), B->stream));
   LongArgs a = long_args(B, 0, 0, lp0, nlp);
   const int lds = long_tb_lds(B->Kp);
   const int grid = std::max(1, std::min(nlp, B->long_wgs));
@@ -845,7 +889,7 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   Plan pl;
   pl.ptb = B->plan.ptb;
   for (int k : bad) {
-    const int GG = pick_gg(B->hp[k].la, B->hp[k].lb, 0.0);
+    const int GG = pick_gg(B->hp[k].la, B->hp[k].lb, 0.0, bad.size() <= (size_t)ctx->cus * 4);
     add_task(pl, pl.tasks1[GG == GWAVE ? 0 : 1], 1, GG, B->trace, B->hp, B->codes, k, k, true, B->Kp);
     pl.tasks1[GG == GWAVE ? 0 : 1].back().flags_or = TA_F_RERUN;
   }
@@ -876,21 +920,69 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
 
 }  // namespace
 
-int ta_batch_run(ta_ctx *ctx, ta_batch *B) {
-  if (!ctx || !B) return TA_E_INVALID;
-  HIPCHK(hipSetDevice(ctx->device));
-  hipStream_t st = B->stream;
+namespace {
+
+int launch_init(ta_batch *B) {
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(init_kernel, dim3(std::max(1, std::min(1024, (B->n + 255) / 256))), dim3(256), 0, B->stream,
+                     B->d_ctr, 16, B->d_fo, B->d_fo_init, B->n, B->d_lprog, B->lprog_n);
+  HIPCHK(hipGetLastError());
+  return TA_OK;
+}
+
+int launch_fills(ta_ctx *ctx, ta_batch *B) {
   const Plan &pl = B->plan;
-  if (B->n > 0)
-    HIPCHK(hipMemcpyAsync(B->d_fo, B->d_fo_init, sizeof(FillOut) * B->n, hipMemcpyDeviceToDevice, st));
-  HIPCHK(hipMemsetAsync(B->d_ctr, 0, 4 * sizeof(int32_t), st));
-  HIPCHK(hipEventRecord(B->ev[0], st));
   const int n2[2] = {(int)pl.tasks2[0].size(), (int)pl.tasks2[1].size()};
   const int n1[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size()};
   int rc = launch_fill<2>(ctx, B, B->d_tasks2, n2, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb, B->d_ctr, 0);
   if (!rc) rc = launch_fill<1>(ctx, B, B->d_tasks1, n1, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb,
                                B->d_ctr + 2, 0);
-  if (!rc) rc = launch_long_fwd(ctx, B, 0, (int)B->litems.size());
+  if (!rc) rc = launch_long_fwd(ctx, B, 0, (int)B->litems.size(), false);
+  return rc;
+}
+
+int launch_tracebacks(ta_ctx *ctx, ta_batch *B) {
+  int rc = launch_traceback(ctx, B, B->d_pairs, B->d_ptb, B->d_tb, 0, B->n);
+  if (!rc) rc = launch_long_tb(ctx, B, 0, (int)B->longs.size(), false);
+  return rc;
+}
+
+// A captured run: init, fills, tracebacks -- kernels only.  Batches that may
+// need the int32 rerun are not captured (the rerun is a host decision).
+int run_graph(ta_ctx *ctx, ta_batch *B) {
+  hipStream_t st = B->stream;
+  if (!B->gexec) {
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = launch_init(B);
+    if (!rc) rc = launch_fills(ctx, B);
+    if (!rc) rc = launch_tracebacks(ctx, B);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : TA_E_DEVICE;
+    }
+    B->graph = g;
+    HIPCHK(hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0));
+  }
+  HIPCHK(hipEventRecord(B->ev[0], st));
+  HIPCHK(hipGraphLaunch(B->gexec, st));
+  HIPCHK(hipEventRecord(B->ev[3], st));
+  B->phases_timed = false;
+  return TA_OK;
+}
+
+}  // namespace
+
+int ta_batch_run(ta_ctx *ctx, ta_batch *B) {
+  if (!ctx || !B) return TA_E_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  if (B->graph_on && !B->may_saturate) return run_graph(ctx, B);
+  hipStream_t st = B->stream;
+  int rc = launch_init(B);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(B->ev[0], st));
+  rc = launch_fills(ctx, B);
   if (rc) return rc;
   HIPCHK(hipEventRecord(B->ev[1], st));
   if (B->may_saturate) {
@@ -900,10 +992,16 @@ int ta_batch_run(ta_ctx *ctx, ta_batch *B) {
     free_rerun(B);
   }
   HIPCHK(hipEventRecord(B->ev[2], st));
-  rc = launch_traceback(ctx, B, B->d_pairs, B->d_ptb, B->d_tb, 0, B->n);
-  if (!rc) rc = launch_long_tb(ctx, B, 0, (int)B->longs.size());
+  rc = launch_tracebacks(ctx, B);
   if (rc) return rc;
   HIPCHK(hipEventRecord(B->ev[3], st));
+  B->phases_timed = true;
+  return TA_OK;
+}
+
+int ta_batch_graph(ta_batch *B, int32_t on) {
+  if (!B) return TA_E_INVALID;
+  B->graph_on = on != 0;
   return TA_OK;
 }
 
@@ -911,9 +1009,12 @@ int ta_batch_wait(ta_ctx *ctx, ta_batch *B) {
   if (!ctx || !B) return TA_E_INVALID;
   HIPCHK(hipStreamSynchronize(B->stream));
   float t = 0;
-  if (hipEventElapsedTime(&t, B->ev[0], B->ev[1]) == hipSuccess) B->ms[0] = t;
-  if (hipEventElapsedTime(&t, B->ev[1], B->ev[2]) == hipSuccess) B->ms[1] = t;
-  if (hipEventElapsedTime(&t, B->ev[2], B->ev[3]) == hipSuccess) B->ms[2] = t;
+  B->ms[0] = B->ms[1] = B->ms[2] = -1.0;  // a captured run is timed as a whole only
+  if (B->phases_timed) {
+    if (hipEventElapsedTime(&t, B->ev[0], B->ev[1]) == hipSuccess) B->ms[0] = t;
+    if (hipEventElapsedTime(&t, B->ev[1], B->ev[2]) == hipSuccess) B->ms[1] = t;
+    if (hipEventElapsedTime(&t, B->ev[2], B->ev[3]) == hipSuccess) B->ms[2] = t;
+  }
   if (hipEventElapsedTime(&t, B->ev[0], B->ev[3]) == hipSuccess) B->ms[3] = t;
   if (!B->longs.empty()) {
     int32_t err = 0;
@@ -1019,7 +1120,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
         tasks.push_back(Task{});
         continue;
       }
-      const int GG = pick_gg(h.la, h.lb, 0.0);
+      const int GG = pick_gg(h.la, h.lb, 0.0, true);
       add_task(pl, tasks, P, GG, B->trace, B->hp, B->codes, k, k, true, B->Kp);
       // Every plain launch reuses the same scratch: rebase to 0.
       Task &T = tasks.back();
@@ -1049,15 +1150,15 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     const HostPair &h = B->hp[k];
     if (h.path == 3) {
       const int q = lp_of[k];
-      int rc = launch_long_fwd(ctx, B, B->litem_first[q], B->longs[q].S);
+      int rc = launch_long_fwd(ctx, B, B->litem_first[q], B->longs[q].S, true);
       if (!rc) rc = launch_traceback(ctx, B, B->d_pairs, B->d_pptb, B->d_ptbuf, k, 1);
-      if (!rc) rc = launch_long_tb(ctx, B, q, 1);
+      if (!rc) rc = launch_long_tb(ctx, B, q, 1, true);
       if (rc) return rc;
       continue;
     }
     if (h.path) {
       HIPCHK(hipMemsetAsync(B->d_ctr + 6, 0, 2 * sizeof(int32_t), st));
-      const bool wave = B->plain_tasks[k].S && pick_gg(h.la, h.lb, 0.0) == GWAVE;
+      const bool wave = B->plain_tasks[k].S && pick_gg(h.la, h.lb, 0.0, true) == GWAVE;
       Task *const tk[2] = {B->d_plain_tasks + k, B->d_plain_tasks + k};
       const int nt[2] = {wave ? 1 : 0, wave ? 0 : 1};
       int rc = h.path == 1 ? launch_fill<1>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1)
@@ -1070,5 +1171,77 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
   HIPCHK(hipEventRecord(B->ev[1], st));
   HIPCHK(hipEventRecord(B->ev[2], st));
   HIPCHK(hipEventRecord(B->ev[3], st));
+  return TA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: candidate launch shapes timed on the caller's pairs, the fastest
+// kept per kind of batch on the context.
+int ta_tune_bucket(const ta_pair *pairs, int32_t n) {
+  if (!pairs || n <= 0) return 0;
+  std::vector<int32_t> len(n);
+  for (int32_t k = 0; k < n; ++k) len[k] = std::max(pairs[k].la, pairs[k].lb);
+  std::nth_element(len.begin(), len.begin() + n / 2, len.end());
+  const int32_t med = len[n / 2];
+  return med < 256 ? 0 : (med < 1024 ? 1 : (med < 4096 ? 2 : 3));
+}
+
+int ta_tune_set(ta_ctx *ctx, const ta_params *p, int32_t bucket, const ta_tuning *t) {
+  if (!ctx || !p || !t || bucket < 0 || bucket > 3 || t->waves_per_cu < 0 || t->waves_per_cu > 32) return TA_E_INVALID;
+  if (t->lanes != TA_LANES_AUTO && t->lanes != TA_LANES_I16 && t->lanes != TA_LANES_I32) return TA_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  ctx->tune[tune_key(p, bucket)] = *t;
+  return TA_OK;
+}
+
+int ta_tune_get(ta_ctx *ctx, const ta_params *p, int32_t bucket, ta_tuning *t) {
+  if (!ctx || !p || !t || bucket < 0 || bucket > 3) return TA_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  auto it = ctx->tune.find(tune_key(p, bucket));
+  if (it == ctx->tune.end()) return TA_E_INVALID;
+  *t = it->second;
+  return TA_OK;
+}
+
+int ta_autotune(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, int32_t repeats, ta_tuning *best,
+                double *best_ms) {
+  if (!ctx || !p || !pairs || n <= 0) return TA_E_INVALID;
+  repeats = std::max(1, std::min(repeats, 50));
+  std::vector<int32_t> lanes_c;
+  if (p->lanes == TA_LANES_AUTO) lanes_c = {TA_LANES_I16, TA_LANES_I32};
+  else lanes_c = {p->lanes};
+  const int Kp = make_alphabet(p->alphabet, p->match, p->mismatch).K + 1;
+  ta_tuning win{0, TA_LANES_AUTO, 0};
+  double win_ms = 1e300;
+  for (int32_t lanes : lanes_c) {
+    const int P = lanes == TA_LANES_I32 ? 1 : 2;
+    int last_eff = -1;
+    for (int waves : {4, 8, 12}) {
+      const int eff = fill_waves_per_cu(P, Kp, waves);  // LDS caps what fits
+      if (eff == last_eff) continue;
+      last_eff = eff;
+      for (int wt = 0; wt < 2; ++wt) {
+        const ta_tuning cand{waves, lanes, wt};
+        ta_batch *B = nullptr;
+        int rc = batch_create_on(ctx, p, pairs, n, ctx->stream, &B, &cand);
+        if (rc) return rc;
+        double ms = 1e300;
+        for (int r = 0; r <= repeats && !rc; ++r) {  // first run warms up
+          rc = ta_batch_run(ctx, B);
+          if (!rc) rc = ta_batch_wait(ctx, B);
+          if (!rc && r > 0) ms = std::min(ms, B->ms[3]);
+        }
+        ta_batch_destroy(B);
+        if (rc) return rc;
+        if (ms < win_ms) { win_ms = ms; win = cand; }
+      }
+    }
+  }
+  {
+    std::lock_guard<std::mutex> g(ctx->tune_mu);
+    ctx->tune[tune_key(p, ta_tune_bucket(pairs, n))] = win;
+  }
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
   return TA_OK;
 }

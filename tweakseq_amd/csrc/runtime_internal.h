@@ -2,6 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include "../../include/tweakalign.h"
 
 struct Serve;
@@ -12,12 +15,20 @@ struct ta_ctx {
   int cus = 256;
   hipStream_t stream = nullptr;
   Serve *serve = nullptr;
+  std::mutex tune_mu;                // the serving thread creates batches too
+  std::map<int, ta_tuning> tune;     // tune_key -> tuning
 };
+
+// Key of a kind of batch in ta_ctx::tune.
+inline int tune_key(const ta_params *p, int bucket) {
+  return ((p->mode * 2 + (p->traceback ? 1 : 0)) * 2 + p->alphabet) * 8 + bucket;
+}
 
 // ta_batch_create on a given stream (the serving loop gives each in-flight
 // batch its own).
+// `force`: use this tuning instead of the context's table (the autotuner).
 int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, hipStream_t stream,
-                    ta_batch **out);
+                    ta_batch **out, const ta_tuning *force = nullptr);
 // Direction-plane bytes a batch of these pairs would allocate (admission).
 int64_t batch_tb_estimate(const ta_params *p, const ta_pair *pairs, int32_t n);
 // Stops the serving thread (called by ta_destroy).

@@ -28,6 +28,10 @@ class Pair(ctypes.Structure):
                 ("b", ctypes.c_void_p), ("lb", ctypes.c_int32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("waves_per_cu", ctypes.c_int32), ("lanes", ctypes.c_int32), ("wave_tasks", ctypes.c_int32)]
+
+
 class CResult(ctypes.Structure):
     _fields_ = [("score", ctypes.c_int32), ("a_begin", ctypes.c_int32),
                 ("a_end", ctypes.c_int32), ("b_begin", ctypes.c_int32),
@@ -58,6 +62,11 @@ def lib():
         "ta_batch_wait": ([vp, vp], ctypes.c_int),
         "ta_batch_results": ([vp, vp, P(CResult), ctypes.c_char_p, i64, P(i64)], ctypes.c_int),
         "ta_batch_timings": ([vp, P(ctypes.c_double), i32], ctypes.c_int),
+        "ta_batch_graph": ([vp, i32], ctypes.c_int),
+        "ta_tune_bucket": ([P(Pair), i32], ctypes.c_int),
+        "ta_autotune": ([vp, P(Params), P(Pair), i32, i32, P(Tuning), P(ctypes.c_double)], ctypes.c_int),
+        "ta_tune_set": ([vp, P(Params), i32, P(Tuning)], ctypes.c_int),
+        "ta_tune_get": ([vp, P(Params), i32, P(Tuning)], ctypes.c_int),
         "ta_batch_info": ([vp, P(i64), P(i64), P(i32)], ctypes.c_int),
         "ta_batch_destroy": ([vp], None),
         "ta_serve_config": ([vp, i64, i32], ctypes.c_int),
@@ -162,6 +171,10 @@ class Batch:
     def wait(self):
         _check(lib().ta_batch_wait(self.engine.ctx, self.h), "ta_batch_wait")
 
+    def graph(self, on=True):
+        """Replay later runs as one captured hipGraph."""
+        _check(lib().ta_batch_graph(self.h, 1 if on else 0), "ta_batch_graph")
+
     def timings(self):
         ms = (ctypes.c_double * 4)()
         k = lib().ta_batch_timings(self.h, ms, 4)
@@ -211,6 +224,25 @@ class Engine:
             return b.results()
         finally:
             b.close()
+
+    # autotuning
+    def autotune(self, pairs, params, repeats=2):
+        """Time candidate launch shapes on these pairs; the fastest is kept for
+        this kind of batch.  Returns (Tuning, ms)."""
+        ps = pairs if isinstance(pairs, _Pairs) else _Pairs(pairs)
+        t, ms = Tuning(), ctypes.c_double()
+        _check(lib().ta_autotune(self.ctx, ctypes.byref(params), ps.arr, ps.n, repeats, ctypes.byref(t),
+                                 ctypes.byref(ms)), "ta_autotune")
+        return t, ms.value
+
+    def tune_set(self, params, bucket, waves_per_cu=0, lanes=LANES_AUTO, wave_tasks=0):
+        _check(lib().ta_tune_set(self.ctx, ctypes.byref(params), bucket,
+                                 ctypes.byref(Tuning(waves_per_cu, lanes, wave_tasks))), "ta_tune_set")
+
+    def tune_get(self, params, bucket):
+        t = Tuning()
+        rc = lib().ta_tune_get(self.ctx, ctypes.byref(params), bucket, ctypes.byref(t))
+        return t if rc == 0 else None
 
     # serving
     def serve_config(self, tb_budget_bytes, max_inflight=0):
