@@ -27,7 +27,8 @@ struct Task {
   int32_t n;           // columns of the task
   int32_t S;           // strips = ceil(max m / (GG * R))
   int32_t NB;          // 8-step blocks per strip = ceil((n + GG) / 8)
-  int32_t single;      // hi half duplicates lo (results of hi discarded)
+  int32_t single;      // 1: hi half duplicates lo (results of hi discarded);
+                       // 2: filler of a quad (computes, writes no results)
   uint64_t row_off;    // rowcodes element offset (S * STRIP uint16)
   uint64_t col_off;    // colcodes element offset (8 * NB + 8 uint16)
   uint64_t ring_off;   // ring element offset (2 * ring_len uint32)

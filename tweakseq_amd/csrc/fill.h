@@ -15,11 +15,13 @@
 //    register (saturating v_pk_* arithmetic); P == 1 is one pair in int32;
 //  * the substitution scores of a lane's R rows for every column residue
 //    are staged in a private LDS slot at the start of each strip, so a step
-//    costs one LDS read per half;
+//    costs one LDS read per half; packed couples keep them as 4-bit scores
+//    (biased to 0..15), which halves the slot so two waves fit per SIMD;
 //  * direction bits (4 per cell) leave as sign bits of int differences,
-//    packed per step and stored per 8-step block with dwordx4 stores, lane
-//    major ([strip][lane][block]) so a traceback reads a lane's consecutive
-//    blocks in one go.
+//    packed per step and stored per 8-step block with dwordx4 stores, block
+//    major ([strip][block][lane]) so the lanes of a group store one
+//    contiguous run per block (lane-major writes scattered 16-byte pieces
+//    over 64 cache lines and cost ~10% of the fill).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -62,8 +64,8 @@ struct Lane<1> {
 };
 
 struct FillArgs {
-  const Task *tasks[2];  // [0] whole-wave tasks (GG = 64), [1] 16-lane tasks
-  int32_t ntasks[2];
+  const Task *tasks[2];  // [0] whole-wave tasks (GG = 64), [1] quads of 16-lane tasks
+  int32_t ntasks[2];     // tasks, quads
   int32_t *ctr;          // work-queue heads ctr[0], ctr[1]
   const int8_t *table;   // Kp x Kp, row/column Kp-1 = pad
   int32_t Kp;
@@ -74,6 +76,7 @@ struct FillArgs {
   FillOut *out;
   int32_t open, ext;
   int32_t sat_limit;     // local int16: best score above this -> int32 rerun
+  int32_t bias;          // P == 2: table entries are score + bias (0..15)
 };
 
 // Shift one lane up inside a group: DPP row_shr:1 (16-lane rows) or
@@ -83,8 +86,10 @@ __device__ __forceinline__ int shr1(int old, int v) {
   return __builtin_amdgcn_update_dpp(old, v, GG == 16 ? 0x111 : 0x138, 0xf, 0xf, false);
 }
 
-// LDS bytes the fill kernel needs for a table of Kp codes.
-__host__ __device__ constexpr int fill_slot_stride(int P, int Kp) { return P * Kp * R + 8; }
+// LDS bytes the fill kernel needs for a table of Kp codes: per lane, R byte
+// scores per code (P == 1, padded against bank conflicts) or one dword of R
+// 4-bit scores per code and half (P == 2).
+__host__ __device__ constexpr int fill_slot_stride(int P, int Kp) { return P == 2 ? 8 * Kp : Kp * R + 8; }
 __host__ __device__ constexpr int fill_lds_bytes(int P, int Kp) {
   return 1024 + 64 * fill_slot_stride(P, Kp);
 }
@@ -93,17 +98,6 @@ __host__ __device__ constexpr int fill_lds_bytes(int P, int Kp) {
 __device__ __forceinline__ int prof1(uint2 p, int k) {
   uint32_t w = k < 4 ? p.x : p.y;
   return (int)(int8_t)(w >> (8 * (k & 3)));
-}
-
-// Packed (sext A[k], sext B[k]) from two 8-byte profile rows.  v_perm can
-// sign-extend only odd bytes, so even rows are moved up one byte first.
-__device__ __forceinline__ uint32_t prof2(uint32_t a, uint32_t b, uint32_t as, uint32_t bs, int k) {
-  switch (k & 3) {
-    case 0: return __builtin_amdgcn_perm(bs, as, 0x0A050801u);
-    case 1: return __builtin_amdgcn_perm(b, a, 0x0A050801u);
-    case 2: return __builtin_amdgcn_perm(bs, as, 0x0B070903u);
-    default: return __builtin_amdgcn_perm(b, a, 0x0B070903u);
-  }
 }
 
 // A lane's profile slot for its R = 8 rows: slot[h][c][k] = S(row code k of
@@ -151,6 +145,46 @@ __device__ __forceinline__ void build_slot(const int8_t *tab, int8_t *slot, int 
   }
 }
 
+// Packed-couple slot: slot32[h * Kp + c] holds the biased 4-bit scores of
+// the lane's rows k = 0..7 for column code c, row 2q in the low and row
+// 2q + 1 in the high nibble of byte q.  Table entries are 0..15.
+__device__ __forceinline__ void build_slot_nib(const int8_t *tab, uint32_t *slot32, int Kp, uint4 rc) {
+  const uint32_t rcw[4] = {rc.x, rc.y, rc.z, rc.w};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    uint4 lo[R], hi[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int code = (rcw[k >> 1] >> (16 * (k & 1) + 8 * h)) & 0xff;
+      lo[k] = *(const uint4 *)(tab + 32 * code);
+      hi[k] = *(const uint4 *)(tab + 32 * code + 16);
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      if (4 * d >= Kp) break;
+      uint32_t w[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const uint4 &q = d < 4 ? lo[k] : hi[k];
+        const int e = d & 3;
+        w[k] = e == 0 ? q.x : (e == 1 ? q.y : (e == 2 ? q.z : q.w));
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int c = 4 * d + b;
+        if (c >= Kp) break;
+        const uint32_t sel = (uint32_t)b | ((uint32_t)(b + 4) << 8);
+        // byte b of rows (0,2) (4,6) -> even rows; (1,3) (5,7) -> odd rows
+        const uint32_t e02 = __builtin_amdgcn_perm(w[2], w[0], sel), e46 = __builtin_amdgcn_perm(w[6], w[4], sel);
+        const uint32_t o13 = __builtin_amdgcn_perm(w[3], w[1], sel), o57 = __builtin_amdgcn_perm(w[7], w[5], sel);
+        const uint32_t ev = __builtin_amdgcn_perm(e46, e02, 0x05040100u);
+        const uint32_t od = __builtin_amdgcn_perm(o57, o13, 0x05040100u);
+        slot32[h * Kp + c] = ev | (od << 4);
+      }
+    }
+  }
+}
+
 template <int GG, int P, bool LOCAL, bool TB>
 __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks, int ntasks, int32_t *ctr,
                                            const int8_t *tab, int8_t *slot) {
@@ -178,7 +212,10 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
   V H[R], E[R], HoL[R];
   V hd0 = vzero, Hbot = vzero, Fbot = vzero;
   int ccur = 0;             // residue codes of the current step (lo | hi << 8)
-  uint2 pa_n = {0, 0}, pb_n = {0, 0};  // profile rows of the current step, read one step early
+  uint2 pa_n = {0, 0};                 // P == 1: profile row of the current step, read one step early
+  uint32_t qa_n = 0, qb_n = 0;         // P == 2: 4-bit profile dwords of the current step (lo / hi half)
+  const uint32_t *slot32 = (const uint32_t *)slot;
+  const V vbias = L::splat(A.bias);
   uint32_t key0[R], key1[R];  // local end-cell keys (P == 2: lo / hi half)
   int rmax[R], rcol[R];       // local end-cell tracking (P == 1)
   int bestH0 = 0, besti0 = 0, bestj0 = 0, bestH1 = 0, besti1 = 0, bestj1 = 0;
@@ -193,14 +230,19 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     key0[k] = 0; key1[k] = 0; rmax[k] = 0; rcol[k] = 0;
   }
 
+  // A wave takes one unit of work: a whole-wave task, or a quad of four
+  // 16-lane tasks of one shape (same strips and blocks), so the strips of the
+  // four groups start and end together and a block's form (full / masked) is
+  // the same for the whole wave.
   auto fetch = [&]() {
     int t = 0;
-    if (l == 0) t = atomicAdd(ctr, 1);
-    t = __shfl(t, lane & ~(GG - 1), 64);
+    if (lane == 0) t = atomicAdd(ctr, 1);
+    t = __shfl(t, 0, 64);
     if (t >= ntasks) {
       alive = false;
       return;
     }
+    if constexpr (GG == G) t = 4 * t + (lane >> 4);
     task = t;
     const Task &T = tasks[t];
     n = T.n; S = T.S; NB = T.NB; single = T.single;
@@ -225,7 +267,8 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
       // ---- strip start: profile slot, left boundary, first block inputs.
       const uint4 rc = s == 0 ? *(const uint4 *)(A.rowcodes + row_off + l * R) : rc_next;
       if (s + 1 < S) rc_next = *(const uint4 *)(A.rowcodes + row_off + (uint64_t)(s + 1) * ST + l * R);
-      build_slot<P>(tab, slot, Kp, rc);
+      if constexpr (P == 2) build_slot_nib(tab, (uint32_t *)slot, Kp, rc);
+      else build_slot<P>(tab, slot, Kp, rc);
       const int row0 = s * ST + l * R;  // 0-based first row of this lane
 #pragma unroll
       for (int k = 0; k < R; ++k) {
@@ -239,9 +282,12 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
       // Residue of step 0: lane 0 takes column 0, the others a pad code
       // (they are idle until their first column arrives).
       ccur = shr1<GG>((int)(ccodes_n.x & 0xffff), (Kp - 1) | ((Kp - 1) << 8));
-      pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
-      pb_n = pa_n;
-      if constexpr (P == 2) pb_n = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
+      if constexpr (P == 2) {
+        qa_n = slot32[ccur & 0xff];
+        qb_n = slot32[Kp + ((ccur >> 8) & 0xff)];
+      } else {
+        pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
+      }
       if (s > 0) {
         const uint32_t *rh = A.ring + ring_off + GG;
         const uint32_t *rf = rh + ring_len;
@@ -263,7 +309,6 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
       }
     }
     const uint32_t ccw[4] = {ccodes.x, ccodes.y, ccodes.z, ccodes.w};
-    const uint32_t next_first = ccodes_n.x & 0xffff;  // lane 0's residue at step 8b + 8
     const uint32_t rhw[8] = {rH0.x, rH0.y, rH0.z, rH0.w, rH1.x, rH1.y, rH1.z, rH1.w};
     const uint32_t rfw[8] = {rF0.x, rF0.y, rF0.z, rF0.w, rF1.x, rF1.y, rF1.z, rF1.w};
     uint32_t tbw[16];
@@ -283,13 +328,20 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
         // the LDS reads of the next step now (lane l's next residue is lane
         // l-1's current one).
         const uint2 pa = pa_n;
-        const uint2 pb = pb_n;
+        const uint32_t qa = qa_n, qb = qb_n;
         const int cthis = ccur;
-        const int code1 = q < 7 ? (int)((ccw[(q + 1) >> 1] >> (16 * ((q + 1) & 1))) & 0xffff) : (int)next_first;
+        // lane 0's residue at step t + 1; at q = 7 the first of the next block,
+        // read here, a block after its load was issued (reading it at the
+        // block start would wait for the load, and for the stores before it)
+        const int code1 = q < 7 ? (int)((ccw[(q + 1) >> 1] >> (16 * ((q + 1) & 1))) & 0xffff)
+                                : (int)(ccodes_n.x & 0xffff);
         ccur = shr1<GG>(code1, cthis);
-        pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
-        if constexpr (P == 2) pb_n = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
-        else pb_n = pa_n;
+        if constexpr (P == 2) {
+          qa_n = slot32[ccur & 0xff];
+          qb_n = slot32[Kp + ((ccur >> 8) & 0xff)];
+        } else {
+          pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
+        }
         int topH, topF;
         if (s == 0) {
           topH = L::bits(L::splat(LOCAL ? 0 : A.open + t * A.ext));
@@ -302,7 +354,9 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
         const V Fup = L::unbits(shr1<GG>(topF, L::bits(Fbot)));
 
         if (FULL || (j >= 0 && j < n)) {
-          const uint32_t pas_x = pa.x << 8, pas_y = pa.y << 8, pbs_x = pb.x << 8, pbs_y = pb.y << 8;
+          // 4-bit scores: even rows in the low nibbles, odd rows in the high ones.
+          const uint32_t aev = qa & 0x0F0F0F0Fu, aod = (qa >> 4) & 0x0F0F0F0Fu;
+          const uint32_t bev = qb & 0x0F0F0F0Fu, bod = (qb >> 4) & 0x0F0F0F0Fu;
           const uint32_t cj = 0xffffu - (uint32_t)j;
 
           V hd = hd0;
@@ -313,9 +367,11 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
           for (int k = 0; k < R; ++k) {
             V Sk;
             if constexpr (P == 2) {
-              const bool hiw = k >= 4;
-              Sk = s2(prof2(hiw ? pa.y : pa.x, hiw ? pb.y : pb.x, hiw ? pas_y : pas_x,
-                            hiw ? pbs_y : pbs_x, k));
+              // byte k/2 of the lo / hi half's nibbles -> (lo16, hi16), unbiased
+              const uint32_t q8 = (uint32_t)(k >> 1);
+              const uint32_t sb = __builtin_amdgcn_perm((k & 1) ? bod : bev, (k & 1) ? aod : aev,
+                                                        q8 | (0x0Cu << 8) | ((4u + q8) << 16) | (0x0Cu << 24));
+              Sk = L::sub(s2(sb), vbias);
             } else {
               Sk = L::unbits(prof1(pa, k));
             }
@@ -381,21 +437,22 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     if (__all(inside)) steps(std::true_type{});
     else steps(std::false_type{});
 
-    // ---- block outputs: direction planes (all lanes), ring (lane 15).
-    if constexpr (TB) {
+    // ---- block outputs: direction planes (all lanes), ring (lane 15); a
+    // quad's filler group writes nothing (it shares the offsets of a task).
+    if (TB && single != 2) {
       if constexpr (P == 2) {
-        uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * GG + l) * NB + b) * 64));
+        uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * NB + b) * GG + l) * 64));
         dst[0] = make_uint4(tbw[0], tbw[1], tbw[2], tbw[3]);
         dst[1] = make_uint4(tbw[4], tbw[5], tbw[6], tbw[7]);
         dst[2] = make_uint4(tbw[8], tbw[9], tbw[10], tbw[11]);
         dst[3] = make_uint4(tbw[12], tbw[13], tbw[14], tbw[15]);
       } else {
-        uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * GG + l) * NB + b) * 32));
+        uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * NB + b) * GG + l) * 32));
         dst[0] = make_uint4(tbw[0], tbw[1], tbw[2], tbw[3]);
         dst[1] = make_uint4(tbw[4], tbw[5], tbw[6], tbw[7]);
       }
     }
-    if (l == GG - 1 && s + 1 < S) {
+    if (l == GG - 1 && s + 1 < S && single != 2) {
       // The last lane handles column t - (GG-1) at step t; ring index =
       // column + GG, so block b carries indices 8b .. 8b+7: the previous
       // block's q = 7 then q 0..6.
@@ -434,7 +491,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
       for (int h = 0; h < P; ++h) {
         const int mh = h ? m1 : m0;
         const int r = mh - 1 - row0;
-        if (mh > 0 && r >= 0 && r < R && !(h == 1 && single)) {
+        if (mh > 0 && r >= 0 && r < R && single != 2 && !(h == 1 && single)) {
           int v = 0;
 #pragma unroll
           for (int k = 0; k < R; ++k)
@@ -463,7 +520,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
           const bool take = oh > bh || (oh == bh && (oi < bi || (oi == bi && oj < bj)));
           if (take) { bh = oh; bi = oi; bj = oj; }
         }
-        if (l == 0 && !(h == 1 && single)) {
+        if (l == 0 && single != 2 && !(h == 1 && single)) {
           const int off = h ? off1 : off0;
           FillOut o;
           o.score = bh;

@@ -64,7 +64,7 @@ struct Alphabet {
   int K = 0;
   int8_t S[32 * 32];  // K x K
   int8_t map[256];    // byte -> code, -1 outside the alphabet
-  int maxS = 0;
+  int maxS = 0, minS = 0;
 };
 
 Alphabet make_alphabet(int alphabet, int match, int mismatch) {
@@ -97,10 +97,26 @@ Alphabet make_alphabet(int alphabet, int match, int mismatch) {
     }
   }
   A.maxS = -128;
+  A.minS = 127;
   for (int x = 0; x < A.K * A.K; ++x) {
     A.maxS = std::max<int>(A.maxS, A.S[x]);
+    A.minS = std::min<int>(A.minS, A.S[x]);
   }
   return A;
+}
+
+// Packed couples keep 4-bit scores: the table must span at most 16 values,
+// and for local couples the pad code (scored as the minimum) must be negative
+// so that leading pad columns hold H at 0.  Other tables run in int32.
+bool nibble_ok(const Alphabet &A, bool local) { return A.maxS - A.minS <= 15 && (!local || A.minS < 0); }
+
+// (K+1) x (K+1) table of score - minS (0..15), pad row / column at 0.
+std::vector<int8_t> nibble_table(const Alphabet &A) {
+  const int Kp = A.K + 1;
+  std::vector<int8_t> t((size_t)Kp * Kp, 0);
+  for (int r = 0; r < A.K; ++r)
+    for (int c = 0; c < A.K; ++c) t[(size_t)r * Kp + c] = (int8_t)(A.S[r * A.K + c] - A.minS);
+  return t;
 }
 
 // (K+1) x (K+1) table with a pad row / column (index K) of PAD_SCORE.
@@ -312,16 +328,19 @@ int fill_waves_per_cu(int P, int Kp, int want = 4) {
 
 // Append one task (pair lo / hi; hi == lo for a single) to the plan, swept
 // by groups of GG lanes.
+// n_force / S_force (quads): columns and strips of the task's quad; extra
+// columns lead (local couples only), extra strips are pad rows.
 void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const std::vector<HostPair> &hp,
-              const std::vector<uint8_t> &codes, int lo, int hi, bool single, int Kp) {
+              const std::vector<uint8_t> &codes, int lo, int hi, bool single, int Kp, int n_force = 0,
+              int S_force = 0) {
   const HostPair &A = hp[lo], &B = hp[hi];
   const int ST = GG * R;
   Task T{};
   T.pair[0] = lo; T.pair[1] = hi;
   T.m[0] = A.la; T.m[1] = B.la;
-  T.n = std::max(A.lb, B.lb);
+  T.n = n_force ? n_force : std::max(A.lb, B.lb);
   T.off[0] = T.n - A.lb; T.off[1] = T.n - B.lb;
-  T.S = ceil_div(std::max(A.la, B.la), ST);
+  T.S = S_force ? S_force : ceil_div(std::max(A.la, B.la), ST);
   T.NB = ceil_div(T.n + GG, 8);
   T.single = single ? 1 : 0;
   const uint16_t pad = (uint16_t)(Kp - 1);
@@ -352,6 +371,50 @@ void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const
     d.tb_off = T.tb_off; d.P = P; d.half = h; d.NB = T.NB; d.off = T.off[h]; d.GG = GG;
   }
   dst.push_back(T);
+}
+
+// A couple (or single) waiting to be grouped into a quad of 16-lane tasks.
+struct Unit {
+  int lo, hi;
+  bool single;
+  int m, n;  // rows and columns (maxima over the two halves)
+};
+
+// Quads: four 16-lane tasks of one shape that one wave runs in lockstep.
+// Local units are ordered by (strips, columns) and take their quad's column
+// count as leading pad columns; global units need equal column counts
+// (H(m, n) is read at the last column), so they are grouped by columns.
+// A quad short of four is filled with copies of its first task that write
+// nothing.  Quads go out largest first, flattened (4 tasks per quad).
+void add_quads(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, bool local, bool trace,
+               const std::vector<HostPair> &hp, const std::vector<uint8_t> &codes, int Kp) {
+  auto strips = [](int m) { return ceil_div(std::max(m, 1), G * R); };
+  std::stable_sort(u.begin(), u.end(), [&](const Unit &x, const Unit &y) {
+    if (!local && x.n != y.n) return x.n > y.n;
+    if (strips(x.m) != strips(y.m)) return strips(x.m) > strips(y.m);
+    return x.n > y.n;
+  });
+  std::vector<std::vector<Task>> quads;
+  for (size_t k = 0; k < u.size();) {
+    size_t e = k;
+    while (e < u.size() && e - k < 4 && (local || u[e].n == u[k].n)) ++e;
+    int nq = 0, mq = 0;
+    for (size_t i = k; i < e; ++i) { nq = std::max(nq, u[i].n); mq = std::max(mq, u[i].m); }
+    std::vector<Task> q;
+    for (size_t i = k; i < e; ++i)
+      add_task(pl, q, P, G, trace, hp, codes, u[i].lo, u[i].hi, u[i].single, Kp, nq, strips(mq));
+    while (q.size() < 4) {
+      Task f = q[0];
+      f.single = 2;
+      q.push_back(f);
+    }
+    quads.push_back(std::move(q));
+    k = e;
+  }
+  std::stable_sort(quads.begin(), quads.end(), [](const std::vector<Task> &x, const std::vector<Task> &y) {
+    return (double)x[0].S * x[0].NB > (double)y[0].S * y[0].NB;
+  });
+  for (auto &q : quads) dst.insert(dst.end(), q.begin(), q.end());
 }
 
 // Steps a 16-lane group needs for a task of these shapes, in 8-step blocks.
@@ -405,6 +468,8 @@ struct ta_batch {
   // device
   uint8_t *d_codes = nullptr;
   int8_t *d_table = nullptr;
+  int8_t *d_table_nib = nullptr;  // packed couples: 4-bit biased scores
+  bool nib = false;
   int fill_waves = 0;             // resident fill waves per CU (0: default_waves)
   bool wave_tasks = false;        // whole-wave groups for every multi-strip task
   Task *d_tasks2[2] = {nullptr, nullptr}, *d_tasks1[2] = {nullptr, nullptr};
@@ -476,7 +541,7 @@ void free_rerun(ta_batch *B) {
 
 void ta_batch_destroy(ta_batch *B) {
   if (!B) return;
-  dfree(B->d_codes); dfree(B->d_table);
+  dfree(B->d_codes); dfree(B->d_table); dfree(B->d_table_nib);
   for (int g = 0; g < 2; ++g) { dfree(B->d_tasks2[g]); dfree(B->d_tasks1[g]); }
   dfree(B->d_rowcodes); dfree(B->d_colcodes); dfree(B->d_ring); dfree(B->d_tb);
   dfree(B->d_ptb); dfree(B->d_pairs); dfree(B->d_fo); dfree(B->d_res); dfree(B->d_cig);
@@ -521,6 +586,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   B->Kp = B->alph.K + 1;
   const bool local = p->mode == TA_LOCAL;
   B->trace = local || p->traceback;
+  B->nib = nibble_ok(B->alph, local);
   // Launch shape: the autotuner's candidate, else the context's tuning for
   // this kind of batch, else the defaults.
   ta_tuning tu{0, TA_LANES_AUTO, 0};
@@ -581,7 +647,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       }
     } else if (long_pair(*p, q.la, q.lb)) {
       h.path = 3;
-    } else if (lanes == TA_LANES_I32) {
+    } else if (lanes == TA_LANES_I32 || !B->nib) {
       h.path = 1;
     } else if (local && q.lb > 65535) {
       h.path = 1;  // packed end-cell keys hold the column in 16 bits
@@ -644,22 +710,27 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   const double slots = (double)ctx->cus * wpc * 4;
   const double mean_slot = work / slots;
   const bool few = B->wave_tasks || couples.size() + i32.size() <= (size_t)ctx->cus * wpc;
+  std::vector<Unit> u2, u1;
   for (auto &c : couples) {
     const bool single = c.second < 0;
     const int hi = single ? c.first : c.second;
     auto sh = shape(c.first, hi);
-    const int GG = pick_gg(sh.first, sh.second, mean_slot, few);
-    add_task(pl, pl.tasks2[GG == GWAVE ? 0 : 1], 2, GG, B->trace, B->hp, B->codes, c.first, hi, single, B->Kp);
+    if (pick_gg(sh.first, sh.second, mean_slot, few) == GWAVE)
+      add_task(pl, pl.tasks2[0], 2, GWAVE, B->trace, B->hp, B->codes, c.first, hi, single, B->Kp);
+    else
+      u2.push_back(Unit{c.first, hi, single, sh.first, sh.second});
   }
   for (int k : i32) {
-    const int GG = pick_gg(B->hp[k].la, B->hp[k].lb, mean_slot, few);
-    add_task(pl, pl.tasks1[GG == GWAVE ? 0 : 1], 1, GG, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+    if (pick_gg(B->hp[k].la, B->hp[k].lb, mean_slot, few) == GWAVE)
+      add_task(pl, pl.tasks1[0], 1, GWAVE, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+    else
+      u1.push_back(Unit{k, k, true, B->hp[k].la, B->hp[k].lb});
   }
+  add_quads(pl, pl.tasks2[1], 2, u2, local, B->trace, B->hp, B->codes, B->Kp);
+  add_quads(pl, pl.tasks1[1], 1, u1, local, B->trace, B->hp, B->codes, B->Kp);
   auto by_cost = [](const Task &x, const Task &y) { return task_cost(x) > task_cost(y); };
-  for (int g = 0; g < 2; ++g) {
-    std::stable_sort(pl.tasks2[g].begin(), pl.tasks2[g].end(), by_cost);
-    std::stable_sort(pl.tasks1[g].begin(), pl.tasks1[g].end(), by_cost);
-  }
+  std::stable_sort(pl.tasks2[0].begin(), pl.tasks2[0].end(), by_cost);
+  std::stable_sort(pl.tasks1[0].begin(), pl.tasks1[0].end(), by_cost);
 
   // Long pairs: strips of all pairs form one queue, largest pairs first.
   {
@@ -705,6 +776,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   int rc = TA_OK;
   rc = rc ? rc : upload(&B->d_codes, B->codes, st);
   rc = rc ? rc : upload(&B->d_table, table, st);
+  if (B->nib) rc = rc ? rc : upload(&B->d_table_nib, nibble_table(B->alph), st);
   for (int g = 0; g < 2; ++g) {
     rc = rc ? rc : upload(&B->d_tasks2[g], pl.tasks2[g], st);
     rc = rc ? rc : upload(&B->d_tasks1[g], pl.tasks1[g], st);
@@ -787,13 +859,14 @@ int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntas
   a.tasks[0] = d_tasks[0]; a.tasks[1] = d_tasks[1];
   a.ntasks[0] = ntasks[0]; a.ntasks[1] = ntasks[1];
   a.ctr = ctr;
-  a.table = B->d_table; a.Kp = B->Kp;
+  a.table = P == 2 ? B->d_table_nib : B->d_table; a.Kp = B->Kp;
+  a.bias = -B->alph.minS;
   a.rowcodes = rows; a.colcodes = cols; a.ring = ring; a.tb = tb; a.out = B->d_fo;
   a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.sat_limit = 32767 - 2 * B->alph.maxS;
   const int lds = fill_lds_bytes(P, B->Kp);
   const int per_cu = fill_waves_per_cu(P, B->Kp, waves_wanted(B));
-  int groups = ntasks[0] + (ntasks[1] + 3) / 4;
+  int groups = ntasks[0] + ntasks[1];  // one wave per whole-wave task or quad
   if (max_groups > 0) groups = std::min(groups, max_groups);
   groups = std::max(1, std::min(groups, ctx->cus * per_cu));
   const bool local = B->p.mode == TA_LOCAL;
@@ -888,11 +961,17 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   if (bad.empty()) return TA_OK;
   Plan pl;
   pl.ptb = B->plan.ptb;
+  std::vector<Unit> u;
   for (int k : bad) {
-    const int GG = pick_gg(B->hp[k].la, B->hp[k].lb, 0.0, bad.size() <= (size_t)ctx->cus * 4);
-    add_task(pl, pl.tasks1[GG == GWAVE ? 0 : 1], 1, GG, B->trace, B->hp, B->codes, k, k, true, B->Kp);
-    pl.tasks1[GG == GWAVE ? 0 : 1].back().flags_or = TA_F_RERUN;
+    if (pick_gg(B->hp[k].la, B->hp[k].lb, 0.0, bad.size() <= (size_t)ctx->cus * 4) == GWAVE) {
+      add_task(pl, pl.tasks1[0], 1, GWAVE, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+      pl.tasks1[0].back().flags_or = TA_F_RERUN;
+    } else {
+      u.push_back(Unit{k, k, true, B->hp[k].la, B->hp[k].lb});
+    }
   }
+  add_quads(pl, pl.tasks1[1], 1, u, B->p.mode == TA_LOCAL, B->trace, B->hp, B->codes, B->Kp);
+  for (Task &t : pl.tasks1[1]) t.flags_or = TA_F_RERUN;
   std::vector<Task> all(pl.tasks1[0]);
   all.insert(all.end(), pl.tasks1[1].begin(), pl.tasks1[1].end());
   hipStream_t st = B->stream;
@@ -914,7 +993,7 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   HIPCHK(hipMemcpyAsync(B->d_ptb, ptb.data(), sizeof(PairTB) * B->n, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(B->d_ctr + 4, 0, 2 * sizeof(int32_t), st));
   Task *const tk[2] = {B->d_tasks_r, B->d_tasks_r + pl.tasks1[0].size()};
-  const int nt[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size()};
+  const int nt[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / 4};
   return launch_fill<1>(ctx, B, tk, nt, B->d_rowcodes_r, B->d_colcodes_r, B->d_ring_r, B->d_tb_r, B->d_ctr + 4, 0);
 }
 
@@ -932,8 +1011,8 @@ int launch_init(ta_batch *B) {
 
 int launch_fills(ta_ctx *ctx, ta_batch *B) {
   const Plan &pl = B->plan;
-  const int n2[2] = {(int)pl.tasks2[0].size(), (int)pl.tasks2[1].size()};
-  const int n1[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size()};
+  const int n2[2] = {(int)pl.tasks2[0].size(), (int)pl.tasks2[1].size() / 4};
+  const int n1[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / 4};
   int rc = launch_fill<2>(ctx, B, B->d_tasks2, n2, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb, B->d_ctr, 0);
   if (!rc) rc = launch_fill<1>(ctx, B, B->d_tasks1, n1, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb,
                                B->d_ctr + 2, 0);
@@ -1110,7 +1189,14 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     Plan pl;
     pl.ptb.assign(B->n, PairTB{0, 0, 0, 0, 0, 0, 0});
     uint64_t max_tb = 0, max_ring = 0;
+    // Four entries per pair: the task, then three quad fillers (a 16-lane
+    // task runs as a quad; fillers write nothing).
     std::vector<Task> tasks;
+    auto fill_quad = [&]() {
+      Task f = tasks.back();
+      f.single = 2;
+      for (int q = 0; q < 3; ++q) tasks.push_back(f);
+    };
     for (int k = 0; k < B->n; ++k) {
       const HostPair &h = B->hp[k];
       const int P = h.path == 1 ? 1 : 2;
@@ -1118,6 +1204,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
       if (h.path == 0 || h.path == 3) {
         if (h.path == 3) pl.ptb[k].P = 3;  // the long-pair kernels write its result
         tasks.push_back(Task{});
+        fill_quad();
         continue;
       }
       const int GG = pick_gg(h.la, h.lb, 0.0, true);
@@ -1129,6 +1216,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
       T.tb_off = 0; T.ring_off = 0;
       pl.ptb[k].tb_off = 0;
       pl.tb_bytes = tb0; pl.ring_elems = r0;
+      fill_quad();
     }
     int rc = TA_OK;
     rc = rc ? rc : upload(&B->d_plain_tasks, tasks, st);
@@ -1158,8 +1246,8 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     }
     if (h.path) {
       HIPCHK(hipMemsetAsync(B->d_ctr + 6, 0, 2 * sizeof(int32_t), st));
-      const bool wave = B->plain_tasks[k].S && pick_gg(h.la, h.lb, 0.0, true) == GWAVE;
-      Task *const tk[2] = {B->d_plain_tasks + k, B->d_plain_tasks + k};
+      const bool wave = B->plain_tasks[4 * k].S && pick_gg(h.la, h.lb, 0.0, true) == GWAVE;
+      Task *const tk[2] = {B->d_plain_tasks + 4 * k, B->d_plain_tasks + 4 * k};
       const int nt[2] = {wave ? 1 : 0, wave ? 0 : 1};
       int rc = h.path == 1 ? launch_fill<1>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1)
                            : launch_fill<2>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1);

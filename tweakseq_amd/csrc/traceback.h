@@ -8,11 +8,11 @@
 // the walker knows by carrying the score backwards (H(i-1,j-1) = H - S,
 // gap open / extend undone likewise), so no stop code is stored.
 //
-// Memory: the fill stores direction planes lane-major ([strip][lane][block]
-// of 8 steps).  While the walk stays in one lane (8 rows) it needs at most
-// two consecutive blocks, so each lane visit is one 128-byte window, kept in
-// a padded per-thread LDS slice (dynamic indexing stays out of scratch) and
-// usually prefetched while the previous lane is walked.  Residue codes come
+// Memory: the fill stores direction planes block-major ([strip][block of 8
+// steps][lane]).  While the walk stays in one lane (8 rows) it needs at most
+// two consecutive blocks of that lane, so each lane visit is one two-block
+// window (two loads), kept in a padded per-thread LDS slice (dynamic indexing
+// stays out of scratch).  Residue codes come
 // through 16-byte register windows read backwards.  One iteration emits one
 // CIGAR op without branching on the state; runs leave as binary dwords.  The
 // walk is latency-bound, so a wave carries few walkers (more waves per SIMD
@@ -137,19 +137,22 @@ __device__ __forceinline__ void walk(const TbArgs &A, int p, const PairDev &pd, 
     const bool hasB = sB >= 0;
     const int fB = hasB ? first_block(tB) : 0;
     const int laneA = s * GG + l, laneB = sB * GG + lB;
+    auto block_at = [&](int ss, int ll, int bb) {
+      return (const uint4 *)(tbase + (((uint64_t)ss * NB + bb) * GG + ll) * esz);
+    };
     {
-      const uint4 *src = (const uint4 *)(tbase + (((uint64_t)s * GG + l) * NB + fA) * esz);
+      const uint4 *src0 = block_at(s, l, fA), *src1 = block_at(s, l, fA + 1);
       uint4 wa[NU];
 #pragma unroll
-      for (int u = 0; u < NU; ++u) wa[u] = src[u];
+      for (int u = 0; u < NU / 2; ++u) { wa[u] = src0[u]; wa[NU / 2 + u] = src1[u]; }
 #pragma unroll
       for (int u = 0; u < NU; ++u) *(uint4 *)(slice + 4 * u) = wa[u];
     }
     if (hasB) {
-      const uint4 *src = (const uint4 *)(tbase + (((uint64_t)sB * GG + lB) * NB + fB) * esz);
+      const uint4 *src0 = block_at(sB, lB, fB), *src1 = block_at(sB, lB, fB + 1);
       uint4 wb[NU];
 #pragma unroll
-      for (int u = 0; u < NU; ++u) wb[u] = src[u];
+      for (int u = 0; u < NU / 2; ++u) { wb[u] = src0[u]; wb[NU / 2 + u] = src1[u]; }
 #pragma unroll
       for (int u = 0; u < NU; ++u) *(uint4 *)(slice + 32 + 4 * u) = wb[u];
     }
