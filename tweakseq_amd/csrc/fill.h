@@ -15,8 +15,11 @@
 //    register (saturating v_pk_* arithmetic); P == 1 is one pair in int32;
 //  * the substitution scores of a lane's R rows for every column residue
 //    are staged in a private LDS slot at the start of each strip, so a step
-//    costs one LDS read per half; packed couples keep them as 4-bit scores
-//    (biased to 0..15), which halves the slot so two waves fit per SIMD;
+//    costs one LDS read per half; the score-only packed kernels keep them as
+//    4-bit scores (biased to 0..15), which halves the slot so that two or
+//    three waves fit per SIMD (they need them to hide the packed ops'
+//    latency); the kernels that write direction planes run one wave per SIMD
+//    and keep bytes (fewer instructions per cell);
 //  * direction bits (4 per cell) leave as sign bits of int differences,
 //    packed per step and stored per 8-step block with dwordx4 stores, block
 //    major ([strip][block][lane]) so the lanes of a group store one
@@ -76,7 +79,7 @@ struct FillArgs {
   FillOut *out;
   int32_t open, ext;
   int32_t sat_limit;     // local int16: best score above this -> int32 rerun
-  int32_t bias;          // P == 2: table entries are score + bias (0..15)
+  int32_t bias;          // 4-bit slots: table entries are score + bias (0..15)
 };
 
 // Shift one lane up inside a group: DPP row_shr:1 (16-lane rows) or
@@ -87,11 +90,24 @@ __device__ __forceinline__ int shr1(int old, int v) {
 }
 
 // LDS bytes the fill kernel needs for a table of Kp codes: per lane, R byte
-// scores per code (P == 1, padded against bank conflicts) or one dword of R
-// 4-bit scores per code and half (P == 2).
-__host__ __device__ constexpr int fill_slot_stride(int P, int Kp) { return P == 2 ? 8 * Kp : Kp * R + 8; }
-__host__ __device__ constexpr int fill_lds_bytes(int P, int Kp) {
-  return 1024 + 64 * fill_slot_stride(P, Kp);
+// scores per code and half (padded against bank conflicts) or, with 4-bit
+// scores, one dword of R scores per code and half.
+__host__ __device__ constexpr int fill_slot_stride(int P, int Kp, bool nib = false) {
+  return nib ? 8 * Kp : P * Kp * R + 8;
+}
+__host__ __device__ constexpr int fill_lds_bytes(int P, int Kp, bool nib = false) {
+  return 1024 + 64 * fill_slot_stride(P, Kp, nib);
+}
+
+// Packed (sext A[k], sext B[k]) from two 8-byte profile rows.  v_perm can
+// sign-extend only odd bytes, so even rows are moved up one byte first.
+__device__ __forceinline__ uint32_t prof2(uint32_t a, uint32_t b, uint32_t as, uint32_t bs, int k) {
+  switch (k & 3) {
+    case 0: return __builtin_amdgcn_perm(bs, as, 0x0A050801u);
+    case 1: return __builtin_amdgcn_perm(b, a, 0x0A050801u);
+    case 2: return __builtin_amdgcn_perm(bs, as, 0x0B070903u);
+    default: return __builtin_amdgcn_perm(b, a, 0x0B070903u);
+  }
 }
 
 // Signed byte k of the 8-byte profile row (rows 0..3 in x, 4..7 in y).
@@ -191,6 +207,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
   using L = Lane<P>;
   using V = typename L::V;
   constexpr int ST = GG * R;  // rows per strip
+  constexpr bool NIB = P == 2 && !TB;  // 4-bit profile slot
   const int lane = threadIdx.x;
   const int l = lane & (GG - 1);
   const int Kp = A.Kp;
@@ -212,14 +229,17 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
   V H[R], E[R], HoL[R];
   V hd0 = vzero, Hbot = vzero, Fbot = vzero;
   int ccur = 0;             // residue codes of the current step (lo | hi << 8)
-  uint2 pa_n = {0, 0};                 // P == 1: profile row of the current step, read one step early
-  uint32_t qa_n = 0, qb_n = 0;         // P == 2: 4-bit profile dwords of the current step (lo / hi half)
+  uint2 pa_n = {0, 0}, pb_n = {0, 0};  // profile rows (lo / hi half) of the current step, read one step early
+  uint32_t qa_n = 0, qb_n = 0;         // NIB: 4-bit profile dwords of the current step
   const uint32_t *slot32 = (const uint32_t *)slot;
   const V vbias = L::splat(A.bias);
-  uint32_t key0[R], key1[R];  // local end-cell keys (P == 2: lo / hi half)
+  // Local end-cell keys (P == 2), one per pair of rows (2p, 2p+1) and half:
+  // (H << 16) | (row 2p ? 0x8000 : 0) | (0x7FFF - j); the maximum is the
+  // largest H, then the upper row, then the first column.
+  uint32_t kp0[R / 2], kp1[R / 2];
   int rmax[R], rcol[R];       // local end-cell tracking (P == 1)
   int bestH0 = 0, besti0 = 0, bestj0 = 0, bestH1 = 0, besti1 = 0, bestj1 = 0;
-  uint4 ccodes = {0, 0, 0, 0}, ccodes_n = {0, 0, 0, 0};
+  uint4 ccodes = {0, 0, 0, 0}, ccodes_n = {0, 0, 0, 0}, ccodes_nn = {0, 0, 0, 0};  // blocks b, b+1, b+2
   uint4 rc_next = {0, 0, 0, 0};  // row codes of the next strip (prefetched)
   uint4 rH0 = {0, 0, 0, 0}, rH1 = {0, 0, 0, 0}, rF0 = {0, 0, 0, 0}, rF1 = {0, 0, 0, 0};
   uint4 nH0 = rH0, nH1 = rH0, nF0 = rH0, nF1 = rH0;
@@ -227,7 +247,8 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     H[k] = vzero; E[k] = vneg; HoL[k] = vzero;
-    key0[k] = 0; key1[k] = 0; rmax[k] = 0; rcol[k] = 0;
+    rmax[k] = 0; rcol[k] = 0;
+    if (k < R / 2) { kp0[k] = 0; kp1[k] = 0; }
   }
 
   // A wave takes one unit of work: a whole-wave task, or a quad of four
@@ -267,7 +288,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
       // ---- strip start: profile slot, left boundary, first block inputs.
       const uint4 rc = s == 0 ? *(const uint4 *)(A.rowcodes + row_off + l * R) : rc_next;
       if (s + 1 < S) rc_next = *(const uint4 *)(A.rowcodes + row_off + (uint64_t)(s + 1) * ST + l * R);
-      if constexpr (P == 2) build_slot_nib(tab, (uint32_t *)slot, Kp, rc);
+      if constexpr (NIB) build_slot_nib(tab, (uint32_t *)slot, Kp, rc);
       else build_slot<P>(tab, slot, Kp, rc);
       const int row0 = s * ST + l * R;  // 0-based first row of this lane
 #pragma unroll
@@ -275,18 +296,21 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
         H[k] = L::splat(left(row0 + k + 1));
         E[k] = vneg;
         HoL[k] = L::add(H[k], vopen);
-        key0[k] = 0; key1[k] = 0; rmax[k] = 0; rcol[k] = 0;
+        rmax[k] = 0; rcol[k] = 0;
+        if (k < R / 2) { kp0[k] = 0; kp1[k] = 0; }
       }
       hd0 = L::splat(left(row0));
       ccodes_n = *(const uint4 *)(A.colcodes + col_off);
+      ccodes_nn = *(const uint4 *)(A.colcodes + col_off + 8);
       // Residue of step 0: lane 0 takes column 0, the others a pad code
       // (they are idle until their first column arrives).
       ccur = shr1<GG>((int)(ccodes_n.x & 0xffff), (Kp - 1) | ((Kp - 1) << 8));
-      if constexpr (P == 2) {
+      if constexpr (NIB) {
         qa_n = slot32[ccur & 0xff];
         qb_n = slot32[Kp + ((ccur >> 8) & 0xff)];
       } else {
         pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
+        if constexpr (P == 2) pb_n = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
       }
       if (s > 0) {
         const uint32_t *rh = A.ring + ring_off + GG;
@@ -297,10 +321,13 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     }
 
     // ---- one block of 8 steps.
+    // Residues are read two blocks ahead (block NB is the column slack), the
+    // ring one block ahead.
     ccodes = ccodes_n;
+    ccodes_n = ccodes_nn;
+    if (b + 2 <= NB) ccodes_nn = *(const uint4 *)(A.colcodes + col_off + 8 * (b + 2));
     rH0 = nH0; rH1 = nH1; rF0 = nF0; rF1 = nF1;
-    if (b + 1 < NB) {  // prefetch the next block's lane-0 inputs
-      ccodes_n = *(const uint4 *)(A.colcodes + col_off + 8 * (b + 1));
+    if (b + 1 < NB) {  // prefetch the next block's lane-0 ring inputs
       if (s > 0) {
         const uint32_t *rh = A.ring + ring_off + GG + 8 * (b + 1);
         const uint32_t *rf = rh + ring_len;
@@ -327,7 +354,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
         // This step's residues and profile rows were fetched last step; start
         // the LDS reads of the next step now (lane l's next residue is lane
         // l-1's current one).
-        const uint2 pa = pa_n;
+        const uint2 pa = pa_n, pb = pb_n;
         const uint32_t qa = qa_n, qb = qb_n;
         const int cthis = ccur;
         // lane 0's residue at step t + 1; at q = 7 the first of the next block,
@@ -336,11 +363,12 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
         const int code1 = q < 7 ? (int)((ccw[(q + 1) >> 1] >> (16 * ((q + 1) & 1))) & 0xffff)
                                 : (int)(ccodes_n.x & 0xffff);
         ccur = shr1<GG>(code1, cthis);
-        if constexpr (P == 2) {
+        if constexpr (NIB) {
           qa_n = slot32[ccur & 0xff];
           qb_n = slot32[Kp + ((ccur >> 8) & 0xff)];
         } else {
           pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
+          if constexpr (P == 2) pb_n = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
         }
         int topH, topF;
         if (s == 0) {
@@ -357,7 +385,10 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
           // 4-bit scores: even rows in the low nibbles, odd rows in the high ones.
           const uint32_t aev = qa & 0x0F0F0F0Fu, aod = (qa >> 4) & 0x0F0F0F0Fu;
           const uint32_t bev = qb & 0x0F0F0F0Fu, bod = (qb >> 4) & 0x0F0F0F0Fu;
-          const uint32_t cj = 0xffffu - (uint32_t)j;
+          // byte scores: copies moved up one byte for the even rows' sign extension
+          const uint32_t pas_x = pa.x << 8, pas_y = pa.y << 8, pbs_x = pb.x << 8, pbs_y = pb.y << 8;
+          const uint32_t cje = 0xffffu - (uint32_t)j, cjo = 0x7fffu - (uint32_t)j;
+          uint32_t ke0 = 0, ke1 = 0;
 
           V hd = hd0;
           V HoU = L::add(Hup, vopen);
@@ -366,12 +397,15 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
 #pragma unroll
           for (int k = 0; k < R; ++k) {
             V Sk;
-            if constexpr (P == 2) {
+            if constexpr (NIB) {
               // byte k/2 of the lo / hi half's nibbles -> (lo16, hi16), unbiased
               const uint32_t q8 = (uint32_t)(k >> 1);
               const uint32_t sb = __builtin_amdgcn_perm((k & 1) ? bod : bev, (k & 1) ? aod : aev,
                                                         q8 | (0x0Cu << 8) | ((4u + q8) << 16) | (0x0Cu << 24));
               Sk = L::sub(s2(sb), vbias);
+            } else if constexpr (P == 2) {
+              const bool hiw = k >= 4;
+              Sk = s2(prof2(hiw ? pa.y : pa.x, hiw ? pb.y : pb.x, hiw ? pas_y : pas_x, hiw ? pbs_y : pbs_x, k));
             } else {
               Sk = L::unbits(prof1(pa, k));
             }
@@ -394,9 +428,12 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
                 // [d lo, d hi, x lo, x hi]; row k keeps bit k of each byte
                 const uint32_t X = __builtin_amdgcn_perm(dF, dE, 0x0B0A0908u);
                 const uint32_t Y = __builtin_amdgcn_perm(dX, dD, 0x0B0A0908u);
+                // acc | (sign bytes & bit k of each byte): one v_bitop3 each
+                // (truth table 0xEA = (a & b) | c), a chain the compiler would
+                // otherwise turn into a wider AND / OR3 tree
                 const uint32_t mk = 0x01010101u << k;
-                xc |= X & mk;
-                yc |= Y & mk;
+                xc = __builtin_amdgcn_bitop3_b32(X, mk, xc, 0xEA);
+                yc = __builtin_amdgcn_bitop3_b32(Y, mk, yc, 0xEA);
               } else {
                 acc = __builtin_amdgcn_alignbit(acc, dE, 31);
                 acc = __builtin_amdgcn_alignbit(acc, dF, 31);
@@ -407,8 +444,15 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
             if constexpr (LOCAL) {
               if constexpr (P == 2) {
                 const uint32_t hu = u32(Hn);
-                key0[k] = max(key0[k], (hu << 16) | cj);
-                key1[k] = max(key1[k], (hu & 0xffff0000u) | cj);
+                const uint32_t c = (k & 1) ? cjo : cje;
+                const uint32_t k0 = (hu << 16) | c, k1 = (hu & 0xffff0000u) | c;
+                if (k & 1) {
+                  kp0[k >> 1] = max(max(kp0[k >> 1], ke0), k0);
+                  kp1[k >> 1] = max(max(kp1[k >> 1], ke1), k1);
+                } else {
+                  ke0 = k0;
+                  ke1 = k1;
+                }
               } else {
                 const int hv = L::bits(Hn);
                 if (hv > rmax[k]) { rmax[k] = hv; rcol[k] = j; }
@@ -474,14 +518,21 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     const int row0 = s * ST + l * R;
     if constexpr (LOCAL) {
+      if constexpr (P == 2) {
+        // A pad row (>= m) never holds more than the running maximum of the
+        // real row above it, so a pair's key decodes to a real row.
 #pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const int i = row0 + k;  // 0-based row
-        if constexpr (P == 2) {
-          const int h0 = (int)(key0[k] >> 16), h1 = (int)(key1[k] >> 16);
-          if (i < m0 && h0 > bestH0) { bestH0 = h0; besti0 = i; bestj0 = 0xffff - (int)(key0[k] & 0xffff); }
-          if (i < m1 && h1 > bestH1) { bestH1 = h1; besti1 = i; bestj1 = 0xffff - (int)(key1[k] & 0xffff); }
-        } else {
+        for (int q = 0; q < R / 2; ++q) {
+          const uint32_t a = kp0[q], c = kp1[q];
+          const int ia = row0 + 2 * q + ((a & 0x8000u) ? 0 : 1), ic = row0 + 2 * q + ((c & 0x8000u) ? 0 : 1);
+          const int h0 = (int)(a >> 16), h1 = (int)(c >> 16);
+          if (ia < m0 && h0 > bestH0) { bestH0 = h0; besti0 = ia; bestj0 = 0x7fff - (int)(a & 0x7fffu); }
+          if (ic < m1 && h1 > bestH1) { bestH1 = h1; besti1 = ic; bestj1 = 0x7fff - (int)(c & 0x7fffu); }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const int i = row0 + k;  // 0-based row
           if (i < m0 && rmax[k] > bestH0) { bestH0 = rmax[k]; besti0 = i; bestj0 = rcol[k]; }
         }
       }
@@ -546,7 +597,7 @@ __global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
     tab[x] = (r < A.Kp && c < A.Kp) ? A.table[r * A.Kp + c] : (int8_t)0;
   }
   __syncthreads();
-  int8_t *slot = (int8_t *)lds + 1024 + lane * fill_slot_stride(P, A.Kp);
+  int8_t *slot = (int8_t *)lds + 1024 + lane * fill_slot_stride(P, A.Kp, P == 2 && !TB);
   // Long tasks first, one per wave; then 16-lane groups take the rest.
   if (A.ntasks[0]) fill_phase<GWAVE, P, LOCAL, TB>(A, A.tasks[0], A.ntasks[0], A.ctr, tab, slot);
   if (A.ntasks[1]) fill_phase<G, P, LOCAL, TB>(A, A.tasks[1], A.ntasks[1], A.ctr + 1, tab, slot);

@@ -105,9 +105,10 @@ Alphabet make_alphabet(int alphabet, int match, int mismatch) {
   return A;
 }
 
-// Packed couples keep 4-bit scores: the table must span at most 16 values,
-// and for local couples the pad code (scored as the minimum) must be negative
-// so that leading pad columns hold H at 0.  Other tables run in int32.
+// Score-only packed couples keep 4-bit scores: the table must span at most
+// 16 values, and for local couples the pad code (scored as the minimum) must
+// be negative so that leading pad columns hold H at 0.  Other score-only
+// tables run in int32.
 bool nibble_ok(const Alphabet &A, bool local) { return A.maxS - A.minS <= 15 && (!local || A.minS < 0); }
 
 // (K+1) x (K+1) table of score - minS (0..15), pad row / column at 0.
@@ -322,8 +323,8 @@ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // a fraction of a step's instructions), and a second wave on the SIMD only
 // halves the speed of the long tasks that end the batch (measured on the
 // 65,536-pair local batch: 9.1 ms at 4 waves per CU, 9.9 at 8, 10.5 at 12).
-int fill_waves_per_cu(int P, int Kp, int want = 4) {
-  return std::max(1, std::min(want, (160 * 1024) / fill_lds_bytes(P, Kp)));
+int fill_waves_per_cu(int P, int Kp, bool nib, int want) {
+  return std::max(1, std::min(want, (160 * 1024) / fill_lds_bytes(P, Kp, nib)));
 }
 
 // Append one task (pair lo / hi; hi == lo for a single) to the plan, swept
@@ -526,10 +527,11 @@ bool global_i16_safe(const ta_params &p, int maxS, int la, int lb) {
 
 // Resident fill waves per CU a batch asks for: its tuning, else one per SIMD
 // when direction planes are written (those kernels keep a SIMD busy alone and
-// a second wave halves the speed of the batch's longest tasks), else two per
-// SIMD (the score-only kernels are leaner and latency-exposed: packed int16
-// ops issue at 77% of peak with one wave, 90% with two).
-int waves_wanted(const ta_batch *B) { return B->fill_waves ? B->fill_waves : (B->trace ? 4 : 8); }
+// more waves slowed the 65,536-pair batch: 8.4 ms at 4 per CU, 9.1 at 8),
+// else three per SIMD (the score-only kernels are leaner and latency-exposed:
+// packed int16 ops issue at 77% of peak with one wave, 90% with two; the
+// all-vs-all batch runs 5.6 ms at 4 per CU, 4.3 at 8, 4.0 at 12).
+int waves_wanted(const ta_batch *B) { return B->fill_waves ? B->fill_waves : (B->trace ? 4 : 12); }
 
 void free_rerun(ta_batch *B) {
   dfree(B->d_tasks_r); dfree(B->d_rowcodes_r); dfree(B->d_colcodes_r);
@@ -647,10 +649,10 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       }
     } else if (long_pair(*p, q.la, q.lb)) {
       h.path = 3;
-    } else if (lanes == TA_LANES_I32 || !B->nib) {
+    } else if (lanes == TA_LANES_I32 || (!B->nib && !B->trace)) {
       h.path = 1;
-    } else if (local && q.lb > 65535) {
-      h.path = 1;  // packed end-cell keys hold the column in 16 bits
+    } else if (local && q.lb > 32767) {
+      h.path = 1;  // packed end-cell keys hold the column in 15 bits
     } else if (local) {
       h.path = 2;
       if ((long)std::min(q.la, q.lb) * B->alph.maxS > 32767 - 2L * B->alph.maxS) B->may_saturate = true;
@@ -706,7 +708,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     work += g16_cost(sh.first, sh.second);
   }
   for (int k : i32) work += g16_cost(B->hp[k].la, B->hp[k].lb);
-  const int wpc = fill_waves_per_cu(2, B->Kp, waves_wanted(B));
+  const int wpc = fill_waves_per_cu(2, B->Kp, !B->trace, waves_wanted(B));
   const double slots = (double)ctx->cus * wpc * 4;
   const double mean_slot = work / slots;
   const bool few = B->wave_tasks || couples.size() + i32.size() <= (size_t)ctx->cus * wpc;
@@ -776,7 +778,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   int rc = TA_OK;
   rc = rc ? rc : upload(&B->d_codes, B->codes, st);
   rc = rc ? rc : upload(&B->d_table, table, st);
-  if (B->nib) rc = rc ? rc : upload(&B->d_table_nib, nibble_table(B->alph), st);
+  if (B->nib && !B->trace) rc = rc ? rc : upload(&B->d_table_nib, nibble_table(B->alph), st);
   for (int g = 0; g < 2; ++g) {
     rc = rc ? rc : upload(&B->d_tasks2[g], pl.tasks2[g], st);
     rc = rc ? rc : upload(&B->d_tasks1[g], pl.tasks1[g], st);
@@ -855,17 +857,18 @@ template <int P>
 int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntasks[2], uint16_t *rows,
                 uint16_t *cols, uint32_t *ring, uint8_t *tb, int32_t *ctr, int max_groups) {
   if (ntasks[0] + ntasks[1] == 0) return TA_OK;
+  const bool nib = P == 2 && !B->trace;  // score-only packed kernels keep 4-bit scores
   FillArgs a;
   a.tasks[0] = d_tasks[0]; a.tasks[1] = d_tasks[1];
   a.ntasks[0] = ntasks[0]; a.ntasks[1] = ntasks[1];
   a.ctr = ctr;
-  a.table = P == 2 ? B->d_table_nib : B->d_table; a.Kp = B->Kp;
+  a.table = nib ? B->d_table_nib : B->d_table; a.Kp = B->Kp;
   a.bias = -B->alph.minS;
   a.rowcodes = rows; a.colcodes = cols; a.ring = ring; a.tb = tb; a.out = B->d_fo;
   a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.sat_limit = 32767 - 2 * B->alph.maxS;
-  const int lds = fill_lds_bytes(P, B->Kp);
-  const int per_cu = fill_waves_per_cu(P, B->Kp, waves_wanted(B));
+  const int lds = fill_lds_bytes(P, B->Kp, nib);
+  const int per_cu = fill_waves_per_cu(P, B->Kp, nib, waves_wanted(B));
   int groups = ntasks[0] + ntasks[1];  // one wave per whole-wave task or quad
   if (max_groups > 0) groups = std::min(groups, max_groups);
   groups = std::max(1, std::min(groups, ctx->cus * per_cu));
@@ -1305,7 +1308,7 @@ int ta_autotune(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n
     const int P = lanes == TA_LANES_I32 ? 1 : 2;
     int last_eff = -1;
     for (int waves : {4, 8, 12}) {
-      const int eff = fill_waves_per_cu(P, Kp, waves);  // LDS caps what fits
+      const int eff = fill_waves_per_cu(P, Kp, P == 2 && !(p->mode == TA_LOCAL || p->traceback), waves);
       if (eff == last_eff) continue;
       last_eff = eff;
       for (int wt = 0; wt < 2; ++wt) {
