@@ -31,6 +31,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "traceback.h"
 
 namespace ta {
 
@@ -80,7 +81,108 @@ struct FillArgs {
   int32_t open, ext;
   int32_t sat_limit;     // local int16: best score above this -> int32 rerun
   int32_t bias;          // 4-bit slots: table entries are score + bias (0..15)
+  int32_t walk;          // direction-plane kernels: a second wave walks the pairs the first fills
+  TbArgs wk;             // the walks' arguments (walk != 0)
 };
+
+// Walks inside the fill kernel.  A workgroup is then one fill wave and one
+// 16-lane walker wave; the fill wave hands each finished task's pairs to the
+// walker through an LDS ring, and the walker walks them while the fill goes
+// on.  Both waves sit in one workgroup (one CU), so a workgroup-scope release
+// / acquire orders the fill's plane and result stores before the walker's
+// loads, and neither wave ever waits on a wave outside its workgroup: the
+// walker waits only for its fill wave, the fill wave only for ring space the
+// walker frees.
+constexpr int WQ_CAP = 64;  // ring entries (pair ids)
+struct WalkQ {
+  int32_t ids[WQ_CAP];
+  int32_t tail;  // written by the fill wave
+  int32_t head;  // written by the walker
+  int32_t done;  // fill wave finished (tail is final)
+  int32_t pad_;
+};
+constexpr int WALK_FW = 4;  // fill waves per workgroup when walking (one per SIMD)
+// LDS of a walking workgroup: table, WALK_FW fill waves' slots, walker slices, rings.
+__host__ __device__ constexpr int walk_lds_bytes(int P, int Kp) {
+  return 1024 + WALK_FW * (64 * (P * Kp * R + 8) + TB_WALKERS * WSTRIDE * 4 + (int)sizeof(WalkQ));
+}
+
+// Fill side, at a task's end: append its pairs to the ring.  NG groups
+// (1 for a whole-wave task, 4 for a quad) with their `single` codes and pair
+// ids as seen by every lane; a filler (single 2) has no pair, a single
+// (1) only its lo pair.
+template <int NG>
+__device__ __forceinline__ void walk_push(WalkQ *q, int &tail, const int (&sg)[NG], const int (&a0)[NG],
+                                          const int (&a1)[NG]) {
+  int cnt = 0;
+#pragma unroll
+  for (int g = 0; g < NG; ++g) cnt += (sg[g] != 2) + (sg[g] == 0);
+  // every plane and result store of the task acknowledged, then the ring
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if ((threadIdx.x & 63) == 0) {
+    while (tail + cnt - __hip_atomic_load(&q->head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) > WQ_CAP)
+      __builtin_amdgcn_s_sleep(2);
+    int t = tail;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      if (sg[g] != 2) q->ids[t++ & (WQ_CAP - 1)] = a0[g];
+      if (sg[g] == 0) q->ids[t++ & (WQ_CAP - 1)] = a1[g];
+    }
+    __hip_atomic_store(&q->tail, t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  tail += cnt;
+}
+
+// Walker side: each lane walks one pair at a time and takes the next
+// handed-over pair as soon as its walk ends, until the fill wave is done and
+// the ring is empty.
+template <int P>
+__device__ __forceinline__ void walk_loop(const FillArgs &A, WalkQ *q, const int8_t *tab, uint32_t *wlds) {
+  const int wl = threadIdx.x & (TB_WALKERS - 1);
+  uint32_t *slice = wlds + wl * WSTRIDE;
+  Walker<P> w;
+  bool active = false;
+  int head = 0;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    if (idle) {
+      const int done = __hip_atomic_load(&q->done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const int tail = __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const int avail = tail - head;
+      if (avail > 0) {
+        const int nidle = __popcll(idle);
+        const int take = avail < nidle ? avail : nidle;
+        // rank of this lane among the idle ones
+        const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
+        if (!active && rank < take) {
+          const int p = q->ids[(head + rank) & (WQ_CAP - 1)];
+          PairDev pd;
+          FillOut f;
+          ResultDev r;
+          if (walk_prologue(A.wk, p, pd, f, r)) {
+            w.begin(A.wk, p, pd, f, r);
+            active = true;
+          }
+        }
+        head += take;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // ids read before the slots are freed
+        if (wl == 0) __hip_atomic_store(&q->head, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else if (done && idle == __ballot(true)) {
+        return;  // done was set after the last push, so tail is final; no walk left
+      }
+    }
+    if (__any(active)) {
+      if (active && w.epoch(A.wk, tab, slice)) active = false;
+    } else {
+      // nothing to walk: a task takes the fill wave tens of microseconds or
+      // more, so poll every few microseconds only (polls cost the fill wave
+      // LDS and issue slots)
+      __builtin_amdgcn_s_sleep(127);
+    }
+  }
+}
 
 // Shift one lane up inside a group: DPP row_shr:1 (16-lane rows) or
 // wave_shr:1 (whole wave); the group's lane 0 takes `old`.
@@ -203,12 +305,12 @@ __device__ __forceinline__ void build_slot_nib(const int8_t *tab, uint32_t *slot
 
 template <int GG, int P, bool LOCAL, bool TB>
 __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks, int ntasks, int32_t *ctr,
-                                           const int8_t *tab, int8_t *slot) {
+                                           const int8_t *tab, int8_t *slot, WalkQ *wq, int &wtail) {
   using L = Lane<P>;
   using V = typename L::V;
   constexpr int ST = GG * R;  // rows per strip
   constexpr bool NIB = P == 2 && !TB;  // 4-bit profile slot
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int l = lane & (GG - 1);
   const int Kp = A.Kp;
 
@@ -582,25 +684,64 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
         }
       }
     }
+    if constexpr (TB) {
+      if (A.walk) {  // hand the task's pairs to the walker wave
+        constexpr int NG = GG == GWAVE ? 1 : 4;
+        int sg[NG], a0[NG], a1[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+          sg[g] = NG == 1 ? single : __shfl(single, 16 * g, 64);
+          a0[g] = NG == 1 ? pid0 : __shfl(pid0, 16 * g, 64);
+          a1[g] = NG == 1 ? pid1 : __shfl(pid1, 16 * g, 64);
+        }
+        walk_push<NG>(wq, wtail, sg, a0, a1);
+      }
+    }
     fetch();
   }
 }
 
+// Workgroup: one fill wave or, with A.walk, WALK_FW fill waves (waves
+// 0..WALK_FW-1) and WALK_FW walker waves (wave WALK_FW + f serves fill wave
+// f with its first 16 lanes).  Waves of a workgroup are dealt to the SIMDs in
+// turn, so the fill waves land on distinct SIMDs, each beside one walker.
 template <int P, bool LOCAL, bool TB>
-__global__ __launch_bounds__(64) void fill_kernel(FillArgs A) {
+__global__ __launch_bounds__(TB && LOCAL ? 128 * WALK_FW : 64) void fill_kernel(FillArgs A) {
   extern __shared__ __align__(16) uint8_t lds[];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // Substitution table in LDS with 32-byte rows (row r at tab + 32 r).
   int8_t *tab = (int8_t *)lds;
-  for (int x = lane; x < 32 * 32; x += 64) {
+  for (int x = threadIdx.x; x < 32 * 32; x += blockDim.x) {
     const int r = x >> 5, c = x & 31;
     tab[x] = (r < A.Kp && c < A.Kp) ? A.table[r * A.Kp + c] : (int8_t)0;
   }
+  const int stride = fill_slot_stride(P, A.Kp, P == 2 && !TB);
+  WalkQ *wq = nullptr;
+  uint32_t *wlds = nullptr;
+  if constexpr (TB && LOCAL) {
+    if (A.walk) {
+      const int f = wave & (WALK_FW - 1);
+      const int base = 1024 + WALK_FW * 64 * stride;
+      wlds = (uint32_t *)(lds + base) + f * TB_WALKERS * WSTRIDE;
+      wq = (WalkQ *)(lds + base + WALK_FW * TB_WALKERS * WSTRIDE * 4) + f;
+      if (wave < WALK_FW && lane == 0) { wq->tail = 0; wq->head = 0; wq->done = 0; }
+    }
+  }
   __syncthreads();
-  int8_t *slot = (int8_t *)lds + 1024 + lane * fill_slot_stride(P, A.Kp, P == 2 && !TB);
+  if constexpr (TB && LOCAL) {
+    if (wave >= WALK_FW) {
+      if (lane < TB_WALKERS) walk_loop<P>(A, wq, tab, wlds);
+      return;
+    }
+  }
+  int8_t *slot = (int8_t *)lds + 1024 + threadIdx.x * stride;
+  int wtail = 0;
   // Long tasks first, one per wave; then 16-lane groups take the rest.
-  if (A.ntasks[0]) fill_phase<GWAVE, P, LOCAL, TB>(A, A.tasks[0], A.ntasks[0], A.ctr, tab, slot);
-  if (A.ntasks[1]) fill_phase<G, P, LOCAL, TB>(A, A.tasks[1], A.ntasks[1], A.ctr + 1, tab, slot);
+  if (A.ntasks[0]) fill_phase<GWAVE, P, LOCAL, TB>(A, A.tasks[0], A.ntasks[0], A.ctr, tab, slot, wq, wtail);
+  if (A.ntasks[1]) fill_phase<G, P, LOCAL, TB>(A, A.tasks[1], A.ntasks[1], A.ctr + 1, tab, slot, wq, wtail);
+  if constexpr (TB && LOCAL) {
+    if (A.walk && lane == 0) __hip_atomic_store(&wq->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
 }
 
 }  // namespace ta

@@ -272,6 +272,9 @@ int ta_create(ta_ctx **out, int device) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     c->cus = cus;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    c->lds_max = lds;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return TA_E_DEVICE;
@@ -323,7 +326,10 @@ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // a fraction of a step's instructions), and a second wave on the SIMD only
 // halves the speed of the long tasks that end the batch (measured on the
 // 65,536-pair local batch: 9.1 ms at 4 waves per CU, 9.9 at 8, 10.5 at 12).
-int fill_waves_per_cu(int P, int Kp, bool nib, int want) {
+// Fill waves per CU: as wanted, within LDS; walking fills have WALK_FW (one
+// workgroup per CU).
+int fill_waves_per_cu(int P, int Kp, bool nib, int want, bool walk = false) {
+  if (walk) return WALK_FW;
   return std::max(1, std::min(want, (160 * 1024) / fill_lds_bytes(P, Kp, nib)));
 }
 
@@ -463,6 +469,11 @@ struct ta_batch {
   std::vector<FillOut> fo_init;
   FillOut *d_fo_init = nullptr;
   int32_t *d_order = nullptr;  // traceback walker -> pair, longest first
+  // Walks inside the fill kernel (direction planes, no int32 rerun possible):
+  // the traceback kernel then takes only the pairs no fill task holds.
+  bool walk = false;
+  int32_t n_rest = 0;
+  int32_t *d_order_rest = nullptr;
   std::vector<PairDev> pairs_h;
   uint64_t cig_bytes = 0;
   Plan plan;
@@ -547,7 +558,7 @@ void ta_batch_destroy(ta_batch *B) {
   for (int g = 0; g < 2; ++g) { dfree(B->d_tasks2[g]); dfree(B->d_tasks1[g]); }
   dfree(B->d_rowcodes); dfree(B->d_colcodes); dfree(B->d_ring); dfree(B->d_tb);
   dfree(B->d_ptb); dfree(B->d_pairs); dfree(B->d_fo); dfree(B->d_res); dfree(B->d_cig);
-  dfree(B->d_ctr); dfree(B->d_fo_init); dfree(B->d_order);
+  dfree(B->d_ctr); dfree(B->d_fo_init); dfree(B->d_order); dfree(B->d_order_rest);
   free_rerun(B);
   dfree(B->d_lp); dfree(B->d_litems); dfree(B->d_lring); dfree(B->d_lck); dfree(B->d_lcol); dfree(B->d_lprog);
   dfree(B->d_wring); dfree(B->d_wtb);
@@ -708,7 +719,13 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     work += g16_cost(sh.first, sh.second);
   }
   for (int k : i32) work += g16_cost(B->hp[k].la, B->hp[k].lb);
-  const int wpc = fill_waves_per_cu(2, B->Kp, !B->trace, waves_wanted(B));
+  // Walks in the fill kernel: local batches whose int16 fill cannot saturate
+  // (global packed fills hold too many registers to share a SIMD with a
+  // walker wave).
+  const char *wenv = getenv("TWEAKALIGN_WALK_IN_FILL");  // "0": walks only after the fill
+  B->walk = local && B->trace && !B->may_saturate && !(wenv && wenv[0] == '0') &&
+            walk_lds_bytes(2, B->Kp) <= ctx->lds_max;
+  const int wpc = fill_waves_per_cu(2, B->Kp, !B->trace, waves_wanted(B), B->walk);
   const double slots = (double)ctx->cus * wpc * 4;
   const double mean_slot = work / slots;
   const bool few = B->wave_tasks || couples.size() + i32.size() <= (size_t)ctx->cus * wpc;
@@ -795,6 +812,13 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       return B->hp[x].la + B->hp[x].lb > B->hp[y].la + B->hp[y].lb;
     });
     rc = rc ? rc : upload(&B->d_order, order, st);
+    if (B->walk) {  // pairs without a fill task: empty and long pairs
+      std::vector<int32_t> rest;
+      for (int32_t k : order)
+        if (pl.ptb[k].P != 1 && pl.ptb[k].P != 2) rest.push_back(k);
+      B->n_rest = (int32_t)rest.size();
+      if (!rest.empty()) rc = rc ? rc : upload(&B->d_order_rest, rest, st);
+    }
   }
   rc = rc ? rc : dalloc(&B->d_ring, pl.ring_elems);
   rc = rc ? rc : dalloc(&B->d_tb, pl.tb_bytes);
@@ -853,9 +877,22 @@ int ta_batch_info(ta_batch *B, int64_t *cells, int64_t *tb_bytes, int32_t *tasks
 // Launch helpers.
 namespace {
 
+TbArgs tb_args(ta_batch *B) {
+  TbArgs a;
+  a.pairs = B->d_pairs; a.npairs = B->n; a.order = nullptr;
+  a.fo = B->d_fo; a.ptb = B->d_ptb;
+  a.codes = B->d_codes; a.table = B->d_table; a.Kp = B->Kp; a.tb = B->d_tb; a.cig = B->d_cig;
+  a.res = B->d_res;
+  a.local = B->p.mode == TA_LOCAL; a.open = B->p.gap_open; a.ext = B->p.gap_extend;
+  a.do_cigar = B->p.traceback;
+  return a;
+}
+
+// `walk`: the batch's own fill (planes in d_tb); each workgroup's walker wave
+// walks the pairs its fill wave finishes.
 template <int P>
 int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntasks[2], uint16_t *rows,
-                uint16_t *cols, uint32_t *ring, uint8_t *tb, int32_t *ctr, int max_groups) {
+                uint16_t *cols, uint32_t *ring, uint8_t *tb, int32_t *ctr, int max_groups, bool walk = false) {
   if (ntasks[0] + ntasks[1] == 0) return TA_OK;
   const bool nib = P == 2 && !B->trace;  // score-only packed kernels keep 4-bit scores
   FillArgs a;
@@ -867,17 +904,21 @@ int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntas
   a.rowcodes = rows; a.colcodes = cols; a.ring = ring; a.tb = tb; a.out = B->d_fo;
   a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.sat_limit = 32767 - 2 * B->alph.maxS;
-  const int lds = fill_lds_bytes(P, B->Kp, nib);
-  const int per_cu = fill_waves_per_cu(P, B->Kp, nib, waves_wanted(B));
+  walk = walk && B->trace && B->p.mode == TA_LOCAL;
+  a.walk = walk;
+  if (walk) a.wk = tb_args(B);
+  const int lds = walk ? walk_lds_bytes(P, B->Kp) : fill_lds_bytes(P, B->Kp, nib);
+  const int per_cu = fill_waves_per_cu(P, B->Kp, nib, waves_wanted(B), walk);
   int groups = ntasks[0] + ntasks[1];  // one wave per whole-wave task or quad
   if (max_groups > 0) groups = std::min(groups, max_groups);
   groups = std::max(1, std::min(groups, ctx->cus * per_cu));
+  if (walk) groups = (groups + WALK_FW - 1) / WALK_FW;  // WALK_FW fill waves per workgroup
   const bool local = B->p.mode == TA_LOCAL;
   void (*k)(FillArgs) = nullptr;
   if (local) k = B->trace ? fill_kernel<P, true, true> : fill_kernel<P, true, false>;
   else k = B->trace ? fill_kernel<P, false, true> : fill_kernel<P, false, false>;
   (void)hipGetLastError();  // drop an earlier, already-handled error
-  hipLaunchKernelGGL(k, dim3(groups), dim3(64), lds, B->stream, a);
+  hipLaunchKernelGGL(k, dim3(groups), dim3(walk ? 128 * WALK_FW : 64), lds, B->stream, a);
   HIPCHK(hipGetLastError());
   return TA_OK;
 }
@@ -885,7 +926,7 @@ int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntas
 int launch_traceback(ta_ctx *ctx, ta_batch *B, const PairDev *pairs, const PairTB *ptb, const uint8_t *tb,
                      int first, int count) {
   if (count <= 0) return TA_OK;
-  TbArgs a;
+  TbArgs a = tb_args(B);
   const bool whole = first == 0 && count == B->n;
   a.pairs = whole ? pairs : pairs + first; a.npairs = count;
   a.order = whole ? B->d_order : nullptr;
@@ -1016,15 +1057,27 @@ int launch_fills(ta_ctx *ctx, ta_batch *B) {
   const Plan &pl = B->plan;
   const int n2[2] = {(int)pl.tasks2[0].size(), (int)pl.tasks2[1].size() / 4};
   const int n1[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / 4};
-  int rc = launch_fill<2>(ctx, B, B->d_tasks2, n2, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb, B->d_ctr, 0);
+  int rc = launch_fill<2>(ctx, B, B->d_tasks2, n2, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb, B->d_ctr, 0,
+                          B->walk);
   if (!rc) rc = launch_fill<1>(ctx, B, B->d_tasks1, n1, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb,
-                               B->d_ctr + 2, 0);
+                               B->d_ctr + 2, 0, B->walk);
   if (!rc) rc = launch_long_fwd(ctx, B, 0, (int)B->litems.size(), false);
   return rc;
 }
 
 int launch_tracebacks(ta_ctx *ctx, ta_batch *B) {
-  int rc = launch_traceback(ctx, B, B->d_pairs, B->d_ptb, B->d_tb, 0, B->n);
+  int rc = TA_OK;
+  if (!B->walk) {
+    rc = launch_traceback(ctx, B, B->d_pairs, B->d_ptb, B->d_tb, 0, B->n);
+  } else if (B->n_rest) {  // the fill walked every pair it filled
+    TbArgs a = tb_args(B);
+    a.npairs = B->n_rest;
+    a.order = B->d_order_rest;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(traceback_kernel, dim3((B->n_rest + TB_WALKERS - 1) / TB_WALKERS), dim3(TB_WALKERS), 0,
+                       B->stream, a);
+    HIPCHK(hipGetLastError());
+  }
   if (!rc) rc = launch_long_tb(ctx, B, 0, (int)B->longs.size(), false);
   return rc;
 }

@@ -13,6 +13,7 @@ struct Serve;
 struct ta_ctx {
   int device = 0;
   int cus = 256;
+  int lds_max = 65536;               // LDS bytes one workgroup may hold
   hipStream_t stream = nullptr;
   Serve *serve = nullptr;
   std::mutex tune_mu;                // the serving thread creates batches too

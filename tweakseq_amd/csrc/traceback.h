@@ -76,86 +76,145 @@ constexpr int WSTRIDE = 68;     // dwords per walker LDS slice: two 32-dword win
 constexpr int TB_WALKERS = 16;  // walkers (threads) per workgroup = per wave
 constexpr int EPOCH = 8;        // steps per epoch (a walk moves at most 8 rows / columns)
 
-// One walk.  Epochs: load everything the next EPOCH steps can touch -- the
-// current lane's two-block window, the window of the lane above, and 32-byte
-// windows of both sequences -- wait once, then step with nothing in flight.
-template <int P>
-__device__ __forceinline__ void walk(const TbArgs &A, int p, const PairDev &pd, const FillOut &f, ResultDev &r,
-                                     bool local, const int8_t *stab, uint32_t *slice) {
-  const PairTB d = A.ptb[p];
-  RunWriter cw;
-  cw.on = A.do_cigar != 0;
-  if (cw.on) {
-    cw.lo = (uint32_t *)(A.cig + pd.cig_off);
-    cw.wp = (uint32_t *)(A.cig + pd.cig_off + pd.cig_cap);
+// The result fields known before any walk.  False when there is nothing to
+// walk: the result is then written (long pairs excepted: their own traceback
+// kernel writes it).
+__device__ __forceinline__ bool walk_prologue(const TbArgs &A, int p, PairDev &pd, FillOut &f, ResultDev &r) {
+  pd = A.pairs[p];
+  f = A.fo[p];
+  r.score = f.score;
+  r.flags = f.flags;
+  r.cigar_off = (int64_t)(pd.cig_off + pd.cig_cap);
+  r.cigar_len = 0;
+  const bool local = A.local != 0;
+  const bool empty = local && f.score <= 0;
+  if (local) {
+    r.a_begin = r.a_end = r.b_begin = r.b_end = -1;
+    if (!empty) { r.a_end = f.end_i - 1; r.b_end = f.end_j - 1; }
+  } else {
+    r.a_begin = 0; r.a_end = pd.la - 1; r.b_begin = 0; r.b_end = pd.lb - 1;
   }
-  const uint8_t *pa = A.codes + pd.a_off;
-  const uint8_t *pb = A.codes + pd.b_off;
-  const int Kp = A.Kp;
-  const int GG = d.GG > 0 ? d.GG : G;
-  const int stsh = (GG == 64 ? 6 : 4) + 3;  // log2(GG * R)
-  constexpr bool p2 = P == 2;
-  constexpr int esz = p2 ? 64 : 32;
-  constexpr int NU = p2 ? 8 : 4;  // uint4 per two-block window
-  const int hsh = 8 * d.half;
-  const int NB = d.NB;
-  const uint8_t *tbase = A.tb + d.tb_off;
-  const int open = A.open, ext = A.ext;
+  if (empty || (!local && !A.do_cigar)) {
+    if (empty) r.flags |= 8;
+    A.res[p] = r;
+    return false;
+  }
+  return A.ptb[p].P != 3;
+}
 
-  int i = local ? f.end_i : pd.la;
-  int j = local ? f.end_j : pd.lb;
-  int state = 0;  // 0 H, 1 E, 2 F
-  int h = f.score;
-  int s = 0, l = 0, k = 0;  // strip, lane, row-in-lane of row i
-  if (i > 0) {
-    const int r0 = i - 1;
-    s = r0 >> stsh;
-    l = (r0 >> 3) & (GG - 1);
-    k = r0 & 7;
+// One walk, resumable epoch by epoch (a walker lane can start another pair
+// as soon as its walk ends).  An epoch loads everything the next EPOCH steps
+// can touch -- the current lane's two-block window, the window of the lane
+// above, and 32-byte windows of both sequences -- waits once, then steps with
+// nothing in flight.
+template <int P>
+struct Walker {
+  static constexpr bool p2 = P == 2;
+  static constexpr int esz = p2 ? 64 : 32;
+
+  RunWriter cw;
+  ResultDev r;
+  const uint8_t *pa, *pb, *tbase;
+  char *cig_end;
+  int p, GG, hsh, NB, off;
+  int i, j, state, h, s, l, k, guard;
+  bool done;
+
+  __device__ __forceinline__ void begin(const TbArgs &A, int p_, const PairDev &pd, const FillOut &f,
+                                        const ResultDev &r_) {
+    p = p_;
+    r = r_;
+    const PairTB d = A.ptb[p];
+    cw = RunWriter();
+    cw.on = A.do_cigar != 0;
+    cig_end = A.cig + pd.cig_off + pd.cig_cap;
+    if (cw.on) {
+      cw.lo = (uint32_t *)(A.cig + pd.cig_off);
+      cw.wp = (uint32_t *)cig_end;
+    }
+    pa = A.codes + pd.a_off;
+    pb = A.codes + pd.b_off;
+    GG = d.GG > 0 ? d.GG : G;
+    hsh = 8 * d.half;
+    NB = d.NB;
+    off = d.off;
+    tbase = A.tb + d.tb_off;
+    const bool local = A.local != 0;
+    i = local ? f.end_i : pd.la;
+    j = local ? f.end_j : pd.lb;
+    state = 0;  // 0 H, 1 E, 2 F
+    h = f.score;
+    s = 0; l = 0; k = 0;  // strip, lane, row-in-lane of row i
+    if (i > 0) {
+      const int r0 = i - 1;
+      s = r0 >> ((GG == 64 ? 6 : 4) + 3);  // strips of GG * R rows
+      l = (r0 >> 3) & (GG - 1);
+      k = r0 & 7;
+    }
+    done = false;
+    guard = 2 * (pd.la + pd.lb) + 2;
   }
-  auto first_block = [&](int t) {
+
+  __device__ __forceinline__ int first_block(int t) const {
     int ff = (t >> 3) - 1;
     ff = ff > NB - 2 ? NB - 2 : ff;
     return ff < 0 ? 0 : ff;
-  };
+  }
+  __device__ __forceinline__ const uint4 *block_at(int ss, int ll, int bb) const {
+    return (const uint4 *)(tbase + (((uint64_t)ss * NB + bb) * GG + ll) * esz);
+  }
 
-  bool done = false;
-  int guard = 2 * (pd.la + pd.lb) + 2;
-  while (!done && guard > 0) {
+  // Two blocks of one lane into a slice window: every load issued before
+  // the first LDS store.
+  __device__ __forceinline__ static void copy_window(uint32_t *dst, const uint4 *src0, const uint4 *src1) {
+    if constexpr (p2) {
+      const uint4 x0 = src0[0], x1 = src0[1], x2 = src0[2], x3 = src0[3];
+      const uint4 y0 = src1[0], y1 = src1[1], y2 = src1[2], y3 = src1[3];
+      uint4 *d = (uint4 *)dst;
+      d[0] = x0; d[1] = x1; d[2] = x2; d[3] = x3;
+      d[4] = y0; d[5] = y1; d[6] = y2; d[7] = y3;
+    } else {
+      const uint4 x0 = src0[0], x1 = src0[1];
+      const uint4 y0 = src1[0], y1 = src1[1];
+      uint4 *d = (uint4 *)dst;
+      d[0] = x0; d[1] = x1; d[2] = y0; d[3] = y1;
+    }
+  }
+
+  // Result of the finished walk.
+  __device__ __forceinline__ void finish(const TbArgs &A) {
+    cw.finish();
+    if (guard <= 0 || cw.overflow) r.flags |= 0x100;
+    if (A.local) { r.a_begin = i; r.b_begin = j; }
+    if (cw.on) {
+      r.cigar_off = (int64_t)((char *)cw.wp - A.cig);
+      r.cigar_len = (int32_t)(cig_end - (char *)cw.wp);  // bytes of runs
+    }
+    A.res[p] = r;
+  }
+
+  // One epoch; true when the walk has ended and its result is written.
+  __device__ __forceinline__ bool epoch(const TbArgs &A, const int8_t *stab, uint32_t *slice) {
+    const bool local = A.local != 0;
+    const int open = A.open, ext = A.ext;
     if (!local) {
-      if (i == 0) { if (j) cw.emit(1, j); break; }
-      if (j == 0) { cw.emit(2, i); break; }
+      if (i == 0) { if (j) cw.emit(1, j); finish(A); return true; }
+      if (j == 0) { cw.emit(2, i); finish(A); return true; }
     } else if (i <= 0 || j <= 0) {
       r.flags |= 0x100;
-      break;
+      finish(A);
+      return true;
     }
     // ---- epoch loads
-    const int t0 = j - 1 + d.off + l;
+    const int t0 = j - 1 + off + l;
     const int fA = first_block(t0);
     int sB = s, lB = l - 1, tB = t0 - 1;
     if (lB < 0) { lB = GG - 1; sB = s - 1; tB += GG; }
     const bool hasB = sB >= 0;
     const int fB = hasB ? first_block(tB) : 0;
     const int laneA = s * GG + l, laneB = sB * GG + lB;
-    auto block_at = [&](int ss, int ll, int bb) {
-      return (const uint4 *)(tbase + (((uint64_t)ss * NB + bb) * GG + ll) * esz);
-    };
-    {
-      const uint4 *src0 = block_at(s, l, fA), *src1 = block_at(s, l, fA + 1);
-      uint4 wa[NU];
-#pragma unroll
-      for (int u = 0; u < NU / 2; ++u) { wa[u] = src0[u]; wa[NU / 2 + u] = src1[u]; }
-#pragma unroll
-      for (int u = 0; u < NU; ++u) *(uint4 *)(slice + 4 * u) = wa[u];
-    }
-    if (hasB) {
-      const uint4 *src0 = block_at(sB, lB, fB), *src1 = block_at(sB, lB, fB + 1);
-      uint4 wb[NU];
-#pragma unroll
-      for (int u = 0; u < NU / 2; ++u) { wb[u] = src0[u]; wb[NU / 2 + u] = src1[u]; }
-#pragma unroll
-      for (int u = 0; u < NU; ++u) *(uint4 *)(slice + 32 + 4 * u) = wb[u];
-    }
+    copy_window(slice, block_at(s, l, fA), block_at(s, l, fA + 1));
+    if (hasB) copy_window(slice + 32, block_at(sB, lB, fB), block_at(sB, lB, fB + 1));
     uint4 ra0 = {0, 0, 0, 0}, ra1 = ra0, rb0 = ra0, rb1 = ra0;
     const int abase = i - 1 >= 0 ? ((i - 1 - EPOCH) < 0 ? 0 : (i - 1 - EPOCH)) & ~15 : 0;
     const int bbase = j - 1 >= 0 ? ((j - 1 - EPOCH) < 0 ? 0 : (j - 1 - EPOCH)) & ~15 : 0;
@@ -168,7 +227,7 @@ __device__ __forceinline__ void walk(const TbArgs &A, int p, const PairDev &pd, 
     // ---- up to EPOCH steps inside the two windows
     for (int e = 0; e < EPOCH && guard > 0; ++e, --guard) {
       if (!local && (i == 0 || j == 0)) break;  // boundary handled by the epoch head
-      const int t = j - 1 + d.off + l;
+      const int t = j - 1 + off + l;
       const int lane_now = s * GG + l;
       int wsel, wi;
       if (lane_now == laneA && t >= 8 * fA && t < 8 * fA + 16) { wsel = 0; wi = t - 8 * fA; }
@@ -191,7 +250,7 @@ __device__ __forceinline__ void walk(const TbArgs &A, int p, const PairDev &pd, 
       const int g = state ? state : ((code & 4) ? ((code & 8) ? 2 : 1) : 0);
       const int xb = g == 1 ? (code & 1) : (g == 2 ? ((code >> 1) & 1) : 0);
       if (local) {
-        const int sc = g == 0 ? (int)stab[win32_byte(ra0, ra1, i - 1 - abase) * Kp + win32_byte(rb0, rb1, j - 1 - bbase)]
+        const int sc = g == 0 ? (int)stab[win32_byte(ra0, ra1, i - 1 - abase) * 32 + win32_byte(rb0, rb1, j - 1 - bbase)]
                               : (xb ? ext : open);
         h -= sc;
       }
@@ -209,51 +268,43 @@ __device__ __forceinline__ void walk(const TbArgs &A, int p, const PairDev &pd, 
         if (local && g == 0 && (h == 0 || i == 0 || j == 0)) { done = true; break; }
       }
     }
+    if (done || guard <= 0) {
+      finish(A);
+      return true;
+    }
+    return false;
   }
-  cw.finish();
-  if (guard <= 0 || cw.overflow) r.flags |= 0x100;
-  if (local) { r.a_begin = i; r.b_begin = j; }
-  if (cw.on) {
-    r.cigar_off = (int64_t)((char *)cw.wp - A.cig);
-    r.cigar_len = (int32_t)(A.cig + pd.cig_off + pd.cig_cap - (char *)cw.wp);  // bytes of runs
+};
+
+// One pair, walked to the end by one thread.
+__device__ __forceinline__ void walk_pair(const TbArgs &A, int p, const int8_t *stab, uint32_t *slice) {
+  PairDev pd;
+  FillOut f;
+  ResultDev r;
+  if (!walk_prologue(A, p, pd, f, r)) return;
+  if (A.ptb[p].P == 2) {
+    Walker<2> w;
+    w.begin(A, p, pd, f, r);
+    while (!w.epoch(A, stab, slice)) {}
+  } else {
+    Walker<1> w;
+    w.begin(A, p, pd, f, r);
+    while (!w.epoch(A, stab, slice)) {}
   }
-  A.res[p] = r;
 }
 
 __global__ __launch_bounds__(TB_WALKERS) void traceback_kernel(TbArgs A) {
   __shared__ uint32_t wlds[TB_WALKERS * WSTRIDE];
   __shared__ int8_t stab[1024];
-  for (int x = threadIdx.x; x < A.Kp * A.Kp; x += TB_WALKERS) stab[x] = A.table[x];
+  for (int x = threadIdx.x; x < 1024; x += TB_WALKERS) {
+    const int r = x >> 5, c = x & 31;
+    stab[x] = (r < A.Kp && c < A.Kp) ? A.table[r * A.Kp + c] : (int8_t)0;
+  }
   __syncthreads();
 
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= A.npairs) return;
-  const int p = A.order ? A.order[w] : w;
-  const PairDev pd = A.pairs[p];
-  const FillOut f = A.fo[p];
-  ResultDev r;
-  r.score = f.score;
-  r.flags = f.flags;
-  r.cigar_off = (int64_t)(pd.cig_off + pd.cig_cap);
-  r.cigar_len = 0;
-  const bool local = A.local != 0;
-  const bool empty = local && f.score <= 0;
-  if (local) {
-    r.a_begin = r.a_end = r.b_begin = r.b_end = -1;
-    if (!empty) { r.a_end = f.end_i - 1; r.b_end = f.end_j - 1; }
-  } else {
-    r.a_begin = 0; r.a_end = pd.la - 1; r.b_begin = 0; r.b_end = pd.lb - 1;
-  }
-  if (empty || (!local && !A.do_cigar)) {
-    if (empty) r.flags |= 8;
-    A.res[p] = r;
-    return;
-  }
-  if (A.ptb[p].P == 3) return;  // long pair: its own traceback kernel writes the result
-
-  uint32_t *slice = wlds + threadIdx.x * WSTRIDE;
-  if (A.ptb[p].P == 2) walk<2>(A, p, pd, f, r, local, stab, slice);
-  else walk<1>(A, p, pd, f, r, local, stab, slice);
+  walk_pair(A, A.order ? A.order[w] : w, stab, wlds + threadIdx.x * WSTRIDE);
 }
 
 }  // namespace ta
