@@ -85,14 +85,13 @@ struct FillArgs {
   TbArgs wk;             // the walks' arguments (walk != 0)
 };
 
-// Walks inside the fill kernel.  A workgroup is then one fill wave and one
-// 16-lane walker wave; the fill wave hands each finished task's pairs to the
-// walker through an LDS ring, and the walker walks them while the fill goes
-// on.  Both waves sit in one workgroup (one CU), so a workgroup-scope release
-// / acquire orders the fill's plane and result stores before the walker's
-// loads, and neither wave ever waits on a wave outside its workgroup: the
-// walker waits only for its fill wave, the fill wave only for ring space the
-// walker frees.
+// Walks inside the fill kernel.  Each fill wave then has a walker wave; the
+// fill wave hands each finished task's pairs to its walker through an LDS
+// ring, and the walker walks them while the fill goes on.  Both waves sit in
+// one workgroup (one CU), so a workgroup-scope release / acquire orders the
+// fill's plane and result stores before the walker's loads, and neither wave
+// ever waits on a wave outside its workgroup: the walker waits only for its
+// fill wave, the fill wave only for ring space the walker frees.
 constexpr int WQ_CAP = 64;  // ring entries (pair ids)
 struct WalkQ {
   int32_t ids[WQ_CAP];
@@ -101,10 +100,11 @@ struct WalkQ {
   int32_t done;  // fill wave finished (tail is final)
   int32_t pad_;
 };
-constexpr int WALK_FW = 4;  // fill waves per workgroup when walking (one per SIMD)
+constexpr int WALK_FW = 4;      // fill waves per workgroup when walking (one per SIMD)
+constexpr int WALK_LANES = 32;  // walker lanes per fill wave
 // LDS of a walking workgroup: table, WALK_FW fill waves' slots, walker slices, rings.
 __host__ __device__ constexpr int walk_lds_bytes(int P, int Kp) {
-  return 1024 + WALK_FW * (64 * (P * Kp * R + 8) + TB_WALKERS * WSTRIDE * 4 + (int)sizeof(WalkQ));
+  return 1024 + WALK_FW * (64 * (P * Kp * R + 8) + WALK_LANES * WSTRIDE * 4 + (int)sizeof(WalkQ));
 }
 
 // Fill side, at a task's end: append its pairs to the ring.  NG groups
@@ -139,7 +139,7 @@ __device__ __forceinline__ void walk_push(WalkQ *q, int &tail, const int (&sg)[N
 // the ring is empty.
 template <int P>
 __device__ __forceinline__ void walk_loop(const FillArgs &A, WalkQ *q, const int8_t *tab, uint32_t *wlds) {
-  const int wl = threadIdx.x & (TB_WALKERS - 1);
+  const int wl = threadIdx.x & 63;
   uint32_t *slice = wlds + wl * WSTRIDE;
   Walker<P> w;
   bool active = false;
@@ -703,7 +703,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
 
 // Workgroup: one fill wave or, with A.walk, WALK_FW fill waves (waves
 // 0..WALK_FW-1) and WALK_FW walker waves (wave WALK_FW + f serves fill wave
-// f with its first 16 lanes).  Waves of a workgroup are dealt to the SIMDs in
+// f with its first WALK_LANES lanes).  Waves of a workgroup are dealt to the SIMDs in
 // turn, so the fill waves land on distinct SIMDs, each beside one walker.
 template <int P, bool LOCAL, bool TB>
 __global__ __launch_bounds__(TB && LOCAL ? 128 * WALK_FW : 64) void fill_kernel(FillArgs A) {
@@ -722,15 +722,15 @@ __global__ __launch_bounds__(TB && LOCAL ? 128 * WALK_FW : 64) void fill_kernel(
     if (A.walk) {
       const int f = wave & (WALK_FW - 1);
       const int base = 1024 + WALK_FW * 64 * stride;
-      wlds = (uint32_t *)(lds + base) + f * TB_WALKERS * WSTRIDE;
-      wq = (WalkQ *)(lds + base + WALK_FW * TB_WALKERS * WSTRIDE * 4) + f;
+      wlds = (uint32_t *)(lds + base) + f * WALK_LANES * WSTRIDE;
+      wq = (WalkQ *)(lds + base + WALK_FW * WALK_LANES * WSTRIDE * 4) + f;
       if (wave < WALK_FW && lane == 0) { wq->tail = 0; wq->head = 0; wq->done = 0; }
     }
   }
   __syncthreads();
   if constexpr (TB && LOCAL) {
     if (wave >= WALK_FW) {
-      if (lane < TB_WALKERS) walk_loop<P>(A, wq, tab, wlds);
+      if (lane < WALK_LANES) walk_loop<P>(A, wq, tab, wlds);
       return;
     }
   }
