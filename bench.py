@@ -209,13 +209,27 @@ def main():
                        "unit": "Gop/s", "frac": round(ach / VALU_PEAK_GOPS, 4),
                        "traffic": tr["bytes_per_launch"] if tr else None,
                        "ops_per_cell": OPS_PER_CELL[1],
-                       "note": "achieved = fill GCUPS x 10 recurrence ops per local cell over the fill launches' "
-                               "HIP-event time; traffic = HBM bytes per fill launch (rocprofv3 FETCH_SIZE + "
-                               "WRITE_SIZE, copy-kernel calibrated, profiles/)"}
+                       "note": "achieved = cells x 10 recurrence ops per local cell over the HIP-event time of "
+                               "the fill launch, which also walks the tracebacks of the pairs it fills (walker "
+                               "waves beside the fill waves); traffic = HBM bytes per fill launch (rocprofv3 "
+                               "FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated, profiles/)"}
     if tr:
         out["roofline"]["hbm_gbs"] = round(tr["bytes_per_launch"] / (fill_avg / 1e3) / 1e9, 1)
 
     if not args.no_extra:
+        # The same batch with the walks in their own kernel after the fill:
+        # the fill kernel alone against the roofline.
+        os.environ["TWEAKALIGN_WALK_IN_FILL"] = "0"
+        bf = eng.batch(w3.pairs_struct(), w3.params)
+        os.environ.pop("TWEAKALIGN_WALK_IN_FILL")
+        _, ff, fr = timed_steps(bf, max(2, args.steps // 2), 1)
+        bf.close()
+        fo = sum(ff) / len(ff)
+        out["fill_only"] = {"fill_kernel_ms": round(fo, 3), "traceback_after_ms": round(sum(fr) / len(fr) - fo, 3),
+                            "fill_gcups": round(cells / (fo / 1e3) / 1e9, 2),
+                            "frac": round(cells / (fo / 1e3) / 1e9 * OPS_PER_CELL[1] / VALU_PEAK_GOPS, 4)}
+        log("fill only (walks after the fill): fill %.2f ms, %.1f GCUPS, frac %.3f"
+            % (fo, out["fill_only"]["fill_gcups"], out["fill_only"]["frac"]))
         # PCIe-inclusive: ta_align_batch from host strings to host results
         # (encode, upload, run, download, CIGAR text), never the headline value.
         import ctypes
