@@ -51,10 +51,11 @@ def run_batch(eng, pairs, params, walk, graph=False, runs=3):
     return outs
 
 
+@pytest.mark.parametrize("mode", [ta.LOCAL, ta.GLOBAL])
 @pytest.mark.parametrize("lanes", [ta.LANES_AUTO, ta.LANES_I32])
-def test_walk_in_fill_matches_after_fill_and_oracle(eng, lanes):
-    pairs = mixed_pairs(71 + lanes, 1200)
-    params = ta.engine.make_params(mode=ta.LOCAL, lanes=lanes)
+def test_walk_in_fill_matches_after_fill_and_oracle(eng, lanes, mode):
+    pairs = mixed_pairs(71 + lanes + 5 * mode, 1200)
+    params = ta.engine.make_params(mode=mode, lanes=lanes)
     after = run_batch(eng, pairs, params, walk=False)
     inside = run_batch(eng, pairs, params, walk=True)
     for r in after[1:] + inside:

@@ -438,8 +438,23 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
       }
     }
     const uint32_t ccw[4] = {ccodes.x, ccodes.y, ccodes.z, ccodes.w};
-    const uint32_t rhw[8] = {rH0.x, rH0.y, rH0.z, rH0.w, rH1.x, rH1.y, rH1.z, rH1.w};
-    const uint32_t rfw[8] = {rF0.x, rF0.y, rF0.z, rF0.w, rF1.x, rF1.y, rF1.z, rF1.w};
+    // Lane 0's top row of each step: the previous strip's bottom row from the
+    // ring or, in the first strip, the boundary.  One branch per block: s is
+    // the same in every lane of a wave (a whole-wave task, or a quad of one
+    // shape).
+    uint32_t rhw[8], rfw[8];
+    if (__builtin_amdgcn_readfirstlane(s) == 0) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        rhw[q] = (uint32_t)L::bits(L::splat(LOCAL ? 0 : A.open + (8 * b + q) * A.ext));
+        rfw[q] = (uint32_t)L::bits(vneg);
+      }
+    } else {
+      rhw[0] = rH0.x; rhw[1] = rH0.y; rhw[2] = rH0.z; rhw[3] = rH0.w;
+      rhw[4] = rH1.x; rhw[5] = rH1.y; rhw[6] = rH1.z; rhw[7] = rH1.w;
+      rfw[0] = rF0.x; rfw[1] = rF0.y; rfw[2] = rF0.z; rfw[3] = rF0.w;
+      rfw[4] = rF1.x; rfw[5] = rF1.y; rfw[6] = rF1.z; rfw[7] = rF1.w;
+    }
     uint32_t tbw[16];
     int rbH[8], rbF[8];
 
@@ -472,14 +487,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
           pa_n = *(const uint2 *)(slot + (ccur & 0xff) * R);
           if constexpr (P == 2) pb_n = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
         }
-        int topH, topF;
-        if (s == 0) {
-          topH = L::bits(L::splat(LOCAL ? 0 : A.open + t * A.ext));
-          topF = L::bits(vneg);
-        } else {
-          topH = (int)rhw[q];
-          topF = (int)rfw[q];
-        }
+        const int topH = (int)rhw[q], topF = (int)rfw[q];
         const V Hup = L::unbits(shr1<GG>(topH, L::bits(Hbot)));
         const V Fup = L::unbits(shr1<GG>(topF, L::bits(Fbot)));
 
@@ -521,9 +529,15 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
             if constexpr (LOCAL) Hn = L::vmax(Hn, vzero);
             const V Hon = L::add(Hn, vopen);
             if constexpr (TB) {
-              const uint32_t dE = (uint32_t)L::bits(L::sub(HoL[k], Eext));
-              const uint32_t dF = (uint32_t)L::bits(L::sub(HoU, Fext));
-              const uint32_t dD = (uint32_t)L::bits(L::sub(HdS, EF));
+              // Signs of: H+open - E (e: E extended), H+open - F (f), the
+              // diagonal - H (d: H not from the diagonal), E - F (x: F above
+              // E).  Taken against the maxima rather than the candidates so
+              // that only values still live are read.  Where local H is 0
+              // (the floor won) d can differ from the candidates' sign; a
+              // walk stops on entering such a cell and never reads it.
+              const uint32_t dE = (uint32_t)L::bits(L::sub(HoL[k], En));
+              const uint32_t dF = (uint32_t)L::bits(L::sub(HoU, Fn));
+              const uint32_t dD = (uint32_t)L::bits(L::sub(HdS, Hn));
               const uint32_t dX = (uint32_t)L::bits(L::sub(En, Fn));
               if constexpr (P == 2) {
                 // sign bytes (0x00 / 0xFF) of [e lo, e hi, f lo, f hi] and
@@ -706,7 +720,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
 // f with its first WALK_LANES lanes).  Waves of a workgroup are dealt to the SIMDs in
 // turn, so the fill waves land on distinct SIMDs, each beside one walker.
 template <int P, bool LOCAL, bool TB>
-__global__ __launch_bounds__(TB && LOCAL ? 128 * WALK_FW : 64) void fill_kernel(FillArgs A) {
+__global__ __launch_bounds__(TB ? 128 * WALK_FW : 64) void fill_kernel(FillArgs A) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // Substitution table in LDS with 32-byte rows (row r at tab + 32 r).
@@ -718,7 +732,7 @@ __global__ __launch_bounds__(TB && LOCAL ? 128 * WALK_FW : 64) void fill_kernel(
   const int stride = fill_slot_stride(P, A.Kp, P == 2 && !TB);
   WalkQ *wq = nullptr;
   uint32_t *wlds = nullptr;
-  if constexpr (TB && LOCAL) {
+  if constexpr (TB) {
     if (A.walk) {
       const int f = wave & (WALK_FW - 1);
       const int base = 1024 + WALK_FW * 64 * stride;
@@ -728,7 +742,7 @@ __global__ __launch_bounds__(TB && LOCAL ? 128 * WALK_FW : 64) void fill_kernel(
     }
   }
   __syncthreads();
-  if constexpr (TB && LOCAL) {
+  if constexpr (TB) {
     if (wave >= WALK_FW) {
       if (lane < WALK_LANES) walk_loop<P>(A, wq, tab, wlds);
       return;
@@ -739,7 +753,7 @@ __global__ __launch_bounds__(TB && LOCAL ? 128 * WALK_FW : 64) void fill_kernel(
   // Long tasks first, one per wave; then 16-lane groups take the rest.
   if (A.ntasks[0]) fill_phase<GWAVE, P, LOCAL, TB>(A, A.tasks[0], A.ntasks[0], A.ctr, tab, slot, wq, wtail);
   if (A.ntasks[1]) fill_phase<G, P, LOCAL, TB>(A, A.tasks[1], A.ntasks[1], A.ctr + 1, tab, slot, wq, wtail);
-  if constexpr (TB && LOCAL) {
+  if constexpr (TB) {
     if (A.walk && lane == 0) __hip_atomic_store(&wq->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }

@@ -719,11 +719,10 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     work += g16_cost(sh.first, sh.second);
   }
   for (int k : i32) work += g16_cost(B->hp[k].la, B->hp[k].lb);
-  // Walks in the fill kernel: local batches whose int16 fill cannot saturate
-  // (global packed fills hold too many registers to share a SIMD with a
-  // walker wave).
+  // Walks in the fill kernel: batches with direction planes whose int16 fill
+  // cannot saturate (a rerun would replace some pairs' planes after the fill).
   const char *wenv = getenv("TWEAKALIGN_WALK_IN_FILL");  // "0": walks only after the fill
-  B->walk = local && B->trace && !B->may_saturate && !(wenv && wenv[0] == '0') &&
+  B->walk = B->trace && !B->may_saturate && !(wenv && wenv[0] == '0') &&
             walk_lds_bytes(2, B->Kp) <= ctx->lds_max;
   const int wpc = fill_waves_per_cu(2, B->Kp, !B->trace, waves_wanted(B), B->walk);
   const double slots = (double)ctx->cus * wpc * 4;
@@ -904,7 +903,7 @@ int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntas
   a.rowcodes = rows; a.colcodes = cols; a.ring = ring; a.tb = tb; a.out = B->d_fo;
   a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.sat_limit = 32767 - 2 * B->alph.maxS;
-  walk = walk && B->trace && B->p.mode == TA_LOCAL;
+  walk = walk && B->trace;
   a.walk = walk;
   if (walk) a.wk = tb_args(B);
   const int lds = walk ? walk_lds_bytes(P, B->Kp) : fill_lds_bytes(P, B->Kp, nib);
