@@ -138,44 +138,75 @@ std::vector<int8_t> padded_table(const Alphabet &A) {
     }                                                               \
   } while (0)
 
-// Device memory cache: blocks up to 256 MB are kept per power-of-two size
-// class and reused (the serving loop builds many small batches; hipMalloc and
-// hipFree per buffer would dominate them).  Larger blocks go straight to HIP.
+// Device memory cache.  Blocks of every size are kept per size class and
+// reused: hipFree waits for the whole device, so freeing a batch's planes
+// while other batches run would stall them all (the serving loop keeps
+// several batches in flight), and hipMalloc per buffer would dominate small
+// batches.  Classes are powers of two up to 1 MB, then four per doubling
+// (at most 19% rounding).  A request takes a free block of its class or of
+// the next classes up to twice its size.  Free blocks stay cached up to half
+// the device's memory; past that, and when hipMalloc fails, cached blocks
+// are released.
 struct MemCache {
   std::mutex mu;
   std::map<std::pair<int, size_t>, std::vector<void *>> free;  // (device, class) -> blocks
   std::unordered_map<void *, std::pair<int, size_t>> live;
+  std::map<int, size_t> cached, cap;                              // per device: free bytes, their limit
 };
 MemCache &memcache() {
   static MemCache *c = new MemCache();  // lives until the process ends
   return *c;
 }
-constexpr size_t kCacheMax = 256ull << 20;
 
 size_t size_class(size_t n) {
   size_t c = 256;
-  while (c < n) c <<= 1;
-  return c;
+  while (c < n && c < (1u << 20)) c <<= 1;
+  if (c >= n) return c;
+  while (c * 2 < n) c <<= 1;  // c < n <= 2c: classes c * (5, 6, 7, 8) / 4
+  for (int q = 5; q <= 8; ++q)
+    if (c / 4 * q >= n) return c / 4 * q;
+  return 2 * c;
+}
+
+// Releases every cached free block of `dev` (lock held).
+void cache_release(MemCache &mc, int dev) {
+  for (auto it = mc.free.begin(); it != mc.free.end(); ++it) {
+    if (it->first.first != dev) continue;
+    for (void *p : it->second) (void)hipFree(p);
+    it->second.clear();
+  }
+  mc.cached[dev] = 0;
 }
 
 void *cache_alloc(size_t bytes) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   MemCache &mc = memcache();
-  const size_t cls = bytes <= kCacheMax ? size_class(bytes) : bytes;
-  if (cls <= kCacheMax) {
-    std::lock_guard<std::mutex> g(mc.mu);
-    auto &v = mc.free[{dev, cls}];
-    if (!v.empty()) {
-      void *p = v.back();
-      v.pop_back();
-      mc.live[p] = {dev, cls};
-      return p;
-    }
+  const size_t cls = size_class(bytes);
+  std::lock_guard<std::mutex> g(mc.mu);
+  if (!mc.cap.count(dev)) {
+    size_t fr = 0, tot = 0;
+    mc.cap[dev] = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 2 : (size_t)64 << 30;
+  }
+  for (auto it = mc.free.lower_bound({dev, cls}); it != mc.free.end() && it->first.first == dev &&
+                                                  it->first.second <= 2 * cls;
+       ++it) {
+    if (it->second.empty()) continue;
+    void *p = it->second.back();
+    it->second.pop_back();
+    mc.cached[dev] -= it->first.second;
+    mc.live[p] = it->first;
+    return p;
   }
   void *p = nullptr;
-  if (hipMalloc(&p, cls) != hipSuccess) return nullptr;
-  std::lock_guard<std::mutex> g(mc.mu);
+  if (hipMalloc(&p, cls) != hipSuccess) {
+    (void)hipGetLastError();
+    cache_release(mc, dev);
+    if (hipMalloc(&p, cls) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
   mc.live[p] = {dev, cls};
   return p;
 }
@@ -183,19 +214,14 @@ void *cache_alloc(size_t bytes) {
 void cache_free(void *p) {
   if (!p) return;
   MemCache &mc = memcache();
-  std::pair<int, size_t> key{0, 0};
-  {
-    std::lock_guard<std::mutex> g(mc.mu);
-    auto it = mc.live.find(p);
-    if (it == mc.live.end()) return;
-    key = it->second;
-    mc.live.erase(it);
-    if (key.second <= kCacheMax) {
-      mc.free[key].push_back(p);
-      return;
-    }
-  }
-  (void)hipFree(p);
+  std::lock_guard<std::mutex> g(mc.mu);
+  auto it = mc.live.find(p);
+  if (it == mc.live.end()) return;
+  const auto key = it->second;
+  mc.live.erase(it);
+  mc.free[key].push_back(p);
+  mc.cached[key.first] += key.second;
+  if (mc.cached[key.first] > mc.cap[key.first]) cache_release(mc, key.first);
 }
 
 template <typename T>
@@ -1189,28 +1215,86 @@ int64_t format_runs(const uint32_t *runs, int n, char *out) {
 }
 }  // namespace
 
+namespace {
+
+// Gathers every pair's CIGAR runs (written backwards at the end of its
+// region) into one dense array: one wave per pair, dword copies.
+__global__ void gather_runs_kernel(const uint32_t *cig, const ResultDev *res, const int64_t *dst_off, uint32_t *dst,
+                                   int n) {
+  const int p = blockIdx.x;
+  if (p >= n) return;
+  const int nr = res[p].cigar_len / 4;
+  if (nr <= 0) return;
+  const uint32_t *src = cig + res[p].cigar_off / 4;
+  uint32_t *d = dst + dst_off[p] / 4;
+  for (int k = threadIdx.x; k < nr; k += blockDim.x) d[k] = src[k];
+}
+
+// Results and CIGAR runs of a finished batch on the host: `res` per pair,
+// `runs` dense, pair k's runs at runs[off[k] / 4 ...] (cigar_len bytes).
+// Only the runs cross PCIe, not the per-pair worst-case regions.
+int fetch_results(ta_batch *B, std::vector<ResultDev> &res, std::vector<uint32_t> &runs, std::vector<int64_t> &off) {
+  res.resize(B->n);
+  off.assign(B->n, 0);
+  runs.clear();
+  hipStream_t st = B->stream;
+  if (B->n) HIPCHK(hipMemcpyAsync(res.data(), B->d_res, sizeof(ResultDev) * B->n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (!B->p.traceback) return TA_OK;
+  int64_t total = 0;
+  for (int32_t k = 0; k < B->n; ++k) {
+    off[k] = total;
+    if (res[k].cigar_len > 0) total += res[k].cigar_len;
+  }
+  if (!total) return TA_OK;
+  runs.resize((size_t)(total / 4));
+  int64_t *d_off = nullptr;
+  uint32_t *d_runs = nullptr;
+  int rc = upload(&d_off, off, st);
+  rc = rc ? rc : dalloc(&d_runs, (size_t)(total / 4));
+  if (!rc) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(gather_runs_kernel, dim3(B->n), dim3(64), 0, st, (const uint32_t *)B->d_cig, B->d_res,
+                       d_off, d_runs, B->n);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(runs.data(), d_runs, (size_t)total, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = TA_E_DEVICE;
+  }
+  (void)hipStreamSynchronize(st);  // nothing in flight reads the buffers once they return to the cache
+  dfree(d_off);
+  dfree(d_runs);
+  return rc;
+}
+
+void fill_result(ta_result &o, const ResultDev &r) {
+  o.score = r.score; o.a_begin = r.a_begin; o.a_end = r.a_end; o.b_begin = r.b_begin; o.b_end = r.b_end;
+  o.flags = r.flags;
+  o.cigar_off = 0;
+  o.cigar_len = 0;
+}
+
+}  // namespace
+
 int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, int64_t cap, int64_t *used) {
   if (!ctx || !B || (B->n && !out)) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
-  std::vector<ResultDev> res(B->n);
-  std::vector<char> cig(B->p.traceback ? B->cig_bytes : 0);
-  if (B->n) HIPCHK(hipMemcpyAsync(res.data(), B->d_res, sizeof(ResultDev) * B->n, hipMemcpyDeviceToHost, B->stream));
-  if (!cig.empty()) HIPCHK(hipMemcpyAsync(cig.data(), B->d_cig, B->cig_bytes, hipMemcpyDeviceToHost, B->stream));
-  HIPCHK(hipStreamSynchronize(B->stream));
+  std::vector<ResultDev> res;
+  std::vector<uint32_t> runs;
+  std::vector<int64_t> off;
+  int rc = fetch_results(B, res, runs, off);
+  if (rc) return rc;
   int64_t pos = 0;
-  int rc = TA_OK;
   for (int32_t k = 0; k < B->n; ++k) {
     const ResultDev &r = res[k];
     ta_result &o = out[k];
-    o.score = r.score; o.a_begin = r.a_begin; o.a_end = r.a_end; o.b_begin = r.b_begin; o.b_end = r.b_end;
-    o.flags = r.flags;
+    fill_result(o, r);
     o.cigar_off = pos;
-    o.cigar_len = 0;
     if (B->p.traceback && r.cigar_len > 0) {
-      const uint32_t *runs = (const uint32_t *)(cig.data() + r.cigar_off);
+      const uint32_t *rk = runs.data() + off[k] / 4;
       const int nr = r.cigar_len / 4;
-      const int64_t need = format_runs(runs, nr, nullptr);
-      if (cigar_buf && pos + need <= cap) format_runs(runs, nr, cigar_buf + pos);
+      const int64_t need = format_runs(rk, nr, nullptr);
+      if (cigar_buf && pos + need <= cap) format_runs(rk, nr, cigar_buf + pos);
       else rc = TA_E_SPACE;
       o.cigar_len = (int32_t)need;
     }
@@ -1218,6 +1302,35 @@ int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, 
   }
   if (used) *used = pos;
   return rc;
+}
+
+int batch_results_text(ta_ctx *ctx, ta_batch *B, std::vector<ta_result> &out, std::string &text) {
+  if (!ctx || !B) return TA_E_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<ResultDev> res;
+  std::vector<uint32_t> runs;
+  std::vector<int64_t> off;
+  const int rc = fetch_results(B, res, runs, off);
+  if (rc) return rc;
+  out.resize(B->n);
+  text.clear();
+  text.reserve(runs.size() * 3);
+  for (int32_t k = 0; k < B->n; ++k) {
+    const ResultDev &r = res[k];
+    ta_result &o = out[k];
+    fill_result(o, r);
+    o.cigar_off = (int64_t)text.size();
+    if (B->p.traceback && r.cigar_len > 0) {
+      const uint32_t *rk = runs.data() + off[k] / 4;
+      const int nr = r.cigar_len / 4;
+      const int64_t need = format_runs(rk, nr, nullptr);
+      const size_t at = text.size();
+      text.resize(at + (size_t)need);
+      format_runs(rk, nr, &text[at]);
+      o.cigar_len = (int32_t)need;
+    }
+  }
+  return TA_OK;
 }
 
 int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, ta_result *out,

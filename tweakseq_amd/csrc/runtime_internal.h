@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <string>
+#include <vector>
 #include <mutex>
 
 #include "../../include/tweakalign.h"
@@ -32,5 +34,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
                     ta_batch **out, const ta_tuning *force = nullptr);
 // Direction-plane bytes a batch of these pairs would allocate (admission).
 int64_t batch_tb_estimate(const ta_params *p, const ta_pair *pairs, int32_t n);
+// Results of a finished batch with the CIGAR text in one string
+// (ta_result::cigar_off indexes `text`): ta_batch_results in one pass.
+int batch_results_text(ta_ctx *ctx, ta_batch *B, std::vector<ta_result> &out, std::string &text);
 // Stops the serving thread (called by ta_destroy).
 void serve_shutdown(ta_ctx *ctx);

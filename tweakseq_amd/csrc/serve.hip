@@ -5,11 +5,15 @@
 // cancel) with an in-process queue.  ta_submit copies a request and returns a
 // ticket at once.  A scheduler thread per context admits pending requests in
 // arrival order while the traceback memory of everything in flight stays
-// within the budget, groups admitted requests with equal parameters into a
-// batch, and launches it on its own stream, so requests that arrive while
-// others run are picked up as soon as memory and an in-flight slot free up
-// and their kernels overlap on the device.  Results are per pair and do not
-// depend on what else shares the batch (pairs never interact).
+// within the budget and fewer than `max_inflight` batches run, and groups
+// admitted requests with equal parameters into a batch.  Each batch goes to
+// one of `max_inflight` worker threads, which builds it, runs it on the
+// worker's own stream, waits for it and splits the results back per request:
+// the host work of one batch (encoding, planning, result formatting)
+// overlaps the GPU work and the host work of the others, and requests that
+// arrive while others run are picked up as soon as memory and a slot free
+// up.  Results are per pair and do not depend on what else shares the batch
+// (pairs never interact).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -40,13 +44,9 @@ struct Request {
   std::string cigar;
 };
 
-struct Flight {
-  ta_batch *B = nullptr;
-  hipStream_t st = nullptr;
-  hipEvent_t done = nullptr;
+struct Group {
   std::vector<Request *> reqs;
   int64_t tb = 0;
-  int32_t npairs = 0;
 };
 
 bool same_params(const ta_params &x, const ta_params &y) { return memcmp(&x, &y, sizeof(ta_params)) == 0; }
@@ -56,35 +56,42 @@ bool same_params(const ta_params &x, const ta_params &y) { return memcmp(&x, &y,
 struct Serve {
   ta_ctx *ctx = nullptr;
   std::mutex mu;
-  std::condition_variable cv;
+  std::condition_variable cv;   // scheduler: requests, slots, memory
+  std::condition_variable wcv;  // workers: admitted groups
   std::deque<Request *> pending;
   std::unordered_map<uint64_t, Request *> all;
-  std::vector<Flight> flights;
-  std::vector<hipStream_t> free_streams;
+  std::deque<Group> groups;     // admitted, waiting for a worker
   int64_t budget = 64ll << 30, used = 0;
-  int max_inflight = 4;
+  int max_inflight = 4, inflight = 0;
   uint64_t next_ticket = 1;
-  bool stop = false;
+  uint64_t changes = 0;         // bumped whenever admission may change its mind
+  bool stop = false, stop_workers = false;
   std::thread th;
+  std::vector<std::thread> workers;
 
   void loop();
-  void finish(Flight &f);
-  bool launch(std::vector<Request *> &group);
+  void worker();
+  void run_group(Group &g, hipStream_t st);
 };
 
-void Serve::finish(Flight &f) {
-  // Results of the whole batch, split back into the requests.
-  std::vector<ta_result> res(f.npairs);
-  int64_t need = 0;
-  int rc = ta_batch_results(ctx, f.B, res.data(), nullptr, 0, &need);
-  std::vector<char> text;
-  if (rc == TA_OK || rc == TA_E_SPACE) {
-    text.resize((size_t)need + 1);
-    rc = ta_batch_results(ctx, f.B, res.data(), text.data(), need + 1, &need);
-  }
-  std::lock_guard<std::mutex> g(mu);
+// One batch from build to per-request results (a worker, without the lock).
+void Serve::run_group(Group &g, hipStream_t st) {
+  std::vector<ta_pair> pairs;
+  for (Request *r : g.reqs)
+    for (size_t k = 0; k < r->a.size(); ++k)
+      pairs.push_back({r->a[k].data(), (int32_t)r->a[k].size(), r->b[k].data(), (int32_t)r->b[k].size()});
+  const int32_t np = (int32_t)pairs.size();
+  ta_batch *B = nullptr;
+  std::vector<ta_result> res;
+  std::string text;
+  int rc = batch_create_on(ctx, &g.reqs[0]->p, pairs.data(), np, st, &B);
+  if (!rc) rc = ta_batch_run(ctx, B);
+  if (!rc) rc = ta_batch_wait(ctx, B);
+  if (!rc) rc = batch_results_text(ctx, B, res, text);
+  if (B) ta_batch_destroy(B);
+  std::lock_guard<std::mutex> lk(mu);
   int32_t k = 0;
-  for (Request *r : f.reqs) {
+  for (Request *r : g.reqs) {
     const int32_t n = (int32_t)r->a.size();
     if (r->state == RState::Cancelled) {  // cancelled while in flight: retire now
       k += n;
@@ -92,108 +99,81 @@ void Serve::finish(Flight &f) {
       delete r;
       continue;
     }
-    if (r->state == RState::InFlight) {
-      if (rc != TA_OK) {
-        r->state = RState::Failed;
-        r->err = rc;
-      } else {
-        r->res.assign(res.begin() + k, res.begin() + k + n);
-        int64_t base = n ? r->res[0].cigar_off : 0;
-        int64_t len = 0;
-        for (auto &x : r->res) len += x.cigar_len;
-        r->cigar.assign(text.data() + base, (size_t)len);
-        for (auto &x : r->res) x.cigar_off -= base;
-        r->state = RState::Done;
-      }
+    if (rc != TA_OK) {
+      r->state = RState::Failed;
+      r->err = rc;
+    } else {
+      r->res.assign(res.begin() + k, res.begin() + k + n);
+      const int64_t base = n ? r->res[0].cigar_off : 0;
+      int64_t len = 0;
+      for (auto &x : r->res) len += x.cigar_len;
+      r->cigar.assign(text.data() + base, (size_t)len);
+      for (auto &x : r->res) x.cigar_off -= base;
+      r->state = RState::Done;
+      r->a.clear(); r->a.shrink_to_fit();
+      r->b.clear(); r->b.shrink_to_fit();
     }
     k += n;
   }
-  used -= f.tb;
-  ta_batch_destroy(f.B);
-  (void)hipEventDestroy(f.done);
-  free_streams.push_back(f.st);
-  f.B = nullptr;
 }
 
-bool Serve::launch(std::vector<Request *> &group) {
-  Flight f;
-  std::vector<ta_pair> pairs;
-  for (Request *r : group)
-    for (size_t k = 0; k < r->a.size(); ++k)
-      pairs.push_back({r->a[k].data(), (int32_t)r->a[k].size(), r->b[k].data(), (int32_t)r->b[k].size()});
-  {
-    std::lock_guard<std::mutex> g(mu);
-    if (free_streams.empty()) {
-      hipStream_t s;
-      if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return false;
-      free_streams.push_back(s);
+void Serve::worker() {
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = nullptr;
+  const bool have_stream = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+  std::unique_lock<std::mutex> lk(mu);
+  for (;;) {
+    wcv.wait(lk, [&] { return stop_workers || !groups.empty(); });
+    if (groups.empty()) break;  // stopping, nothing left
+    Group g = std::move(groups.front());
+    groups.pop_front();
+    lk.unlock();
+    if (have_stream) {
+      run_group(g, st);
+    } else {
+      std::lock_guard<std::mutex> g2(mu);
+      for (Request *r : g.reqs)
+        if (r->state == RState::InFlight) { r->state = RState::Failed; r->err = TA_E_DEVICE; }
     }
-    f.st = free_streams.back();
-    free_streams.pop_back();
+    lk.lock();
+    --inflight;
+    used -= g.tb;
+    ++changes;
+    cv.notify_all();
   }
-  f.reqs = group;
-  f.npairs = (int32_t)pairs.size();
-  for (Request *r : group) f.tb += r->tb_bytes;
-  int rc = batch_create_on(ctx, &group[0]->p, pairs.data(), f.npairs, f.st, &f.B);
-  if (!rc) rc = ta_batch_run(ctx, f.B);
-  if (!rc && hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess) rc = TA_E_DEVICE;
-  if (!rc && hipEventRecord(f.done, f.st) != hipSuccess) rc = TA_E_DEVICE;
-  std::lock_guard<std::mutex> g(mu);
-  if (rc) {
-    for (Request *r : group)
-      if (r->state == RState::InFlight) { r->state = RState::Failed; r->err = rc; }
-    used -= f.tb;
-    if (f.B) ta_batch_destroy(f.B);
-    free_streams.push_back(f.st);
-    return false;
-  }
-  flights.push_back(f);
-  return true;
+  lk.unlock();
+  if (have_stream) (void)hipStreamDestroy(st);
 }
 
 void Serve::loop() {
-  (void)hipSetDevice(ctx->device);
   std::unique_lock<std::mutex> lk(mu);
   for (;;) {
-    if (stop && pending.empty() && flights.empty()) break;
-    // Completed batches first: they return memory to the budget.
-    for (size_t x = 0; x < flights.size();) {
-      if (hipEventQuery(flights[x].done) == hipSuccess) {
-        Flight f = flights[x];
-        flights.erase(flights.begin() + x);
-        lk.unlock();
-        finish(f);
-        lk.lock();
-        cv.notify_all();
-      } else {
-        ++x;
-      }
-    }
+    if (stop) break;
+    const uint64_t seen = changes;
     // Admission in arrival order while memory and in-flight slots allow;
     // requests with the parameters of the first admitted one form the batch.
-    if (!pending.empty() && (int)flights.size() < max_inflight) {
-      std::vector<Request *> group;
-      int64_t add = 0;
+    if (!pending.empty() && inflight < max_inflight) {
+      Group g;
       for (auto it = pending.begin(); it != pending.end();) {
         Request *r = *it;
         if (r->state == RState::Cancelled) { it = pending.erase(it); continue; }
-        if (!group.empty() && !same_params(r->p, group[0]->p)) { ++it; continue; }
-        if (used + add + r->tb_bytes > budget) break;
-        add += r->tb_bytes;
+        if (!g.reqs.empty() && !same_params(r->p, g.reqs[0]->p)) { ++it; continue; }
+        if (used + g.tb + r->tb_bytes > budget) break;
+        g.tb += r->tb_bytes;
         r->state = RState::InFlight;
-        group.push_back(r);
+        g.reqs.push_back(r);
         it = pending.erase(it);
       }
-      if (!group.empty()) {
-        used += add;
-        lk.unlock();
-        launch(group);
-        lk.lock();
+      if (!g.reqs.empty()) {
+        used += g.tb;
+        ++inflight;
+        while ((int)workers.size() < max_inflight) workers.emplace_back([this] { worker(); });
+        groups.push_back(std::move(g));
+        wcv.notify_one();
         continue;
       }
     }
-    if (flights.empty()) cv.wait(lk, [&] { return stop || !pending.empty(); });
-    else cv.wait_for(lk, std::chrono::microseconds(50));
+    cv.wait(lk, [&] { return stop || changes != seen; });
   }
 }
 
@@ -208,8 +188,13 @@ void serve_shutdown(ta_ctx *ctx) {
   }
   s->cv.notify_all();
   if (s->th.joinable()) s->th.join();
+  {  // the scheduler admits nothing more; workers finish what they hold
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop_workers = true;
+  }
+  s->wcv.notify_all();
+  for (auto &w : s->workers) w.join();
   for (auto &kv : s->all) delete kv.second;
-  for (hipStream_t st : s->free_streams) (void)hipStreamDestroy(st);
   delete s;
   ctx->serve = nullptr;
 }
@@ -232,6 +217,7 @@ int ta_serve_config(ta_ctx *ctx, int64_t tb_budget_bytes, int32_t max_inflight) 
   std::lock_guard<std::mutex> g(s->mu);
   s->budget = tb_budget_bytes;
   if (max_inflight > 0) s->max_inflight = max_inflight;
+  ++s->changes;
   s->cv.notify_all();
   return TA_OK;
 }
@@ -260,6 +246,7 @@ int ta_submit(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, 
   s->all[r->ticket] = r;
   s->pending.push_back(r);
   *ticket = r->ticket;
+  ++s->changes;
   s->cv.notify_all();
   return TA_OK;
 }
