@@ -640,19 +640,40 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   B->wave_tasks = tu.wave_tasks != 0;
   const int lanes = p->lanes != TA_LANES_AUTO ? p->lanes : tu.lanes;
 
-  // Encode (sequences shared between pairs are encoded once).
-  std::unordered_map<const char *, std::pair<uint64_t, int32_t>> seen;
-  auto enc = [&](const char *s, int32_t len, uint64_t *off) -> int {
+  // Encode (sequences shared between pairs are encoded once).  Each
+  // sequence also gets the sum over its residues of the best score the
+  // residue can reach against any letter (0 if none is positive): no local
+  // alignment of it scores more, which decides whether int16 can saturate.
+  int best[32] = {0};
+  for (int r = 0; r < B->alph.K; ++r)
+    for (int c = 0; c < B->alph.K; ++c) best[r] = std::max<int>(best[r], B->alph.S[r * B->alph.K + c]);
+  struct Enc { uint64_t off; int32_t len; int64_t bound; };
+  std::unordered_map<const char *, Enc> seen;
+  {
+    size_t total = 64;
+    for (int32_t k = 0; k < n; ++k) total += (size_t)std::max(pairs[k].la, 0) + std::max(pairs[k].lb, 0) + 32;
+    B->codes.reserve(std::min<size_t>(total, (size_t)1 << 34));
+  }
+  auto enc = [&](const char *s, int32_t len, uint64_t *off, int64_t *bound) -> int {
     auto it = seen.find(s);
-    if (it != seen.end() && it->second.second == len) { *off = it->second.first; return TA_OK; }
-    while (B->codes.size() % 16) B->codes.push_back(0);  // 16-byte aligned starts (walker windows)
-    *off = B->codes.size();
-    for (int32_t i = 0; i < len; ++i) {
-      int c = B->alph.map[(unsigned char)s[i]];
-      if (c < 0) return TA_E_ALPHABET;
-      B->codes.push_back((uint8_t)c);
+    if (it != seen.end() && it->second.len == len) {
+      *off = it->second.off;
+      *bound = it->second.bound;
+      return TA_OK;
     }
-    seen[s] = {*off, len};
+    const size_t at = (B->codes.size() + 15) & ~(size_t)15;  // 16-byte aligned starts (walker windows)
+    B->codes.resize(at + (size_t)len);
+    uint8_t *dst = B->codes.data() + at;
+    int64_t sum = 0;
+    for (int32_t i = 0; i < len; ++i) {
+      const int c = B->alph.map[(unsigned char)s[i]];
+      if (c < 0) return TA_E_ALPHABET;
+      dst[i] = (uint8_t)c;
+      sum += best[c];
+    }
+    *off = at;
+    *bound = sum;
+    seen[s] = Enc{at, len, sum};
     return TA_OK;
   };
   B->hp.resize(n);
@@ -670,8 +691,9 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       return TA_E_INVALID;
     }
     HostPair &h = B->hp[k];
-    int rc = enc(q.a, q.la, &h.a_off);
-    if (!rc) rc = enc(q.b, q.lb, &h.b_off);
+    int64_t bound_a = 0, bound_b = 0;
+    int rc = enc(q.a, q.la, &h.a_off, &bound_a);
+    if (!rc) rc = enc(q.b, q.lb, &h.b_off, &bound_b);
     if (rc) { delete B; return rc; }
     h.la = q.la; h.lb = q.lb;
     B->cells += (int64_t)q.la * q.lb;
@@ -692,7 +714,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       h.path = 1;  // packed end-cell keys hold the column in 15 bits
     } else if (local) {
       h.path = 2;
-      if ((long)std::min(q.la, q.lb) * B->alph.maxS > 32767 - 2L * B->alph.maxS) B->may_saturate = true;
+      if (std::min(bound_a, bound_b) > 32767 - 2L * B->alph.maxS) B->may_saturate = true;
     } else {
       h.path = global_i16_safe(*p, B->alph.maxS, q.la, q.lb) ? 2 : 1;
     }
