@@ -303,8 +303,20 @@ def main():
             % (c5, out["long_dna_ms"], out["long_dna_gcups"], out["long_dna_fill_gcups"], t5[2]))
 
     if not args.no_extra:
-        # Measured VALU throughput (instruction-kind microbenchmark) beside the theoretical peak.
+        # Served form: open-loop Poisson arrivals (2,000 requests/s, 1-256
+        # pairs each, lengths 50-5,000, global/local 50/50) through
+        # ta_submit/ta_poll in a child process; latency = completion minus
+        # scheduled arrival.
         import subprocess
+        sb = os.path.join(ROOT, "tweakseq_amd", "tweakalign-serve-bench")
+        if os.path.exists(sb):
+            r = subprocess.run([sb, "4000"], capture_output=True, text=True, timeout=300)
+            if r.returncode == 0:
+                out["served"] = json.loads(r.stdout.strip().splitlines()[-1])
+                log("served: p50 %.1f ms, p99 %.1f ms, %.1f GCUPS served of %.1f offered"
+                    % (out["served"]["p50_ms"], out["served"]["p99_ms"], out["served"]["served_gcups"],
+                       out["served"]["offered_gcups"]))
+        # Measured VALU throughput (instruction-kind microbenchmark) beside the theoretical peak.
         tool = os.path.join(ROOT, "tweakseq_amd", "tweakalign-valu-peak")
         if os.path.exists(tool):
             r = subprocess.run([tool], capture_output=True, text=True, timeout=120)
