@@ -80,3 +80,40 @@ def test_walk_in_fill_config3_prefix(eng):
     inside = run_batch(eng, pairs, params, walk=True, runs=2)
     for r in inside:
         assert r == after
+
+
+def run_env(eng, pairs, params, env, runs=2):
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        b = eng.batch(pairs, params)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    outs = []
+    for _ in range(runs):
+        b.run(); b.wait()
+        outs.append(b.results())
+    b.close()
+    return outs
+
+
+@pytest.mark.parametrize("mode", [ta.LOCAL, ta.GLOBAL])
+def test_cooperative_long_tasks(eng, mode):
+    # few pairs, several 512-row strips each: a workgroup's fill waves sweep
+    # each pair together (strips pipelined); one short pair rides along
+    rng = random.Random(81 + mode)
+    pairs = []
+    for la in (4100, 2600, 1800, 1030, 700, 513, 90):
+        a = rand_prot(rng, la)
+        pairs.append((a, mutate(rng, a)))
+    pairs.append((rand_prot(rng, 3000), rand_prot(rng, 2500)))  # unrelated: scores near 0
+    params = ta.engine.make_params(mode=mode)
+    coop = run_env(eng, pairs, params, {"TWEAKALIGN_COOP": "1"})
+    plain = run_env(eng, pairs, params, {"TWEAKALIGN_COOP": "0"}, runs=1)[0]
+    for r in coop:
+        assert r == plain
+    compare(coop[0], oracle_results(params, pairs), params, pairs, "cooperative")

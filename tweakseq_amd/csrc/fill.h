@@ -83,6 +83,11 @@ struct FillArgs {
   int32_t bias;          // 4-bit slots: table entries are score + bias (0..15)
   int32_t walk;          // direction-plane kernels: a second wave walks the pairs the first fills
   TbArgs wk;             // the walks' arguments (walk != 0)
+  int32_t ncoop;         // walk != 0: the first ncoop whole-wave tasks are swept by a workgroup's
+                         // fill waves together, strips pipelined (counter coop_ctr)
+  int32_t *coop_ctr;
+  int32_t *err;          // | 4: a cooperative wait gave up
+  int32_t spin_limit;
 };
 
 // Walks inside the fill kernel.  Each fill wave then has a walker wave; the
@@ -102,9 +107,24 @@ struct WalkQ {
 };
 constexpr int WALK_FW = 4;      // fill waves per workgroup when walking (one per SIMD)
 constexpr int WALK_LANES = 32;  // walker lanes per fill wave
+
+// Cooperative whole-wave tasks (batches of few tasks, where one long pair on
+// one wave would set the run time): the WALK_FW fill waves of a workgroup
+// sweep one task together, wave w taking strips w, w + WALK_FW, ...  Strip s
+// reads strip s-1's bottom row from the task's ring once strip s-1 has
+// published that it stored it (LDS counters, workgroup scope: all strips of
+// the task run on one CU).  Wave 0 leads: it starts a task when all waves
+// have arrived from the previous one, then reduces their best cells and
+// hands the pair to its walker.
+struct CoopQ {
+  int32_t task, epoch, arrived, pad_;
+  int32_t prog[8];                 // strip s publishes in prog[s % 8]: ((s + 1) << 20) | blocks stored
+  int32_t best[WALK_FW][2][3];     // per wave and half: H, i, j of its strips' best cell
+};
 // LDS of a walking workgroup: table, WALK_FW fill waves' slots, walker slices, rings.
 __host__ __device__ constexpr int walk_lds_bytes(int P, int Kp) {
-  return 1024 + WALK_FW * (64 * (P * Kp * R + 8) + WALK_LANES * WSTRIDE * 4 + (int)sizeof(WalkQ));
+  return 1024 + WALK_FW * (64 * (P * Kp * R + 8) + WALK_LANES * WSTRIDE * 4 + (int)sizeof(WalkQ)) +
+         (int)sizeof(CoopQ);
 }
 
 // Fill side, at a task's end: append its pairs to the ring.  NG groups
@@ -303,9 +323,27 @@ __device__ __forceinline__ void build_slot_nib(const int8_t *tab, uint32_t *slot
   }
 }
 
-template <int GG, int P, bool LOCAL, bool TB>
+// Progress value of strip s with `blocks` ring blocks stored (grows with s).
+__device__ __forceinline__ int coop_prog(int s, int blocks) { return ((s + 1) << 20) | blocks; }
+
+// Waits until an LDS counter reaches `target` (acquire); gives up after the
+// spin limit, flagging the error, so a wait can never hang the kernel.
+__device__ __forceinline__ void coop_wait(const FillArgs &A, int32_t *c, int target) {
+  int spins = 0;
+  while (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
+    if (++spins > A.spin_limit) {
+      if ((threadIdx.x & 63) == 0) atomicOr(A.err, 4);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+template <int GG, int P, bool LOCAL, bool TB, bool COOP = false>
 __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks, int ntasks, int32_t *ctr,
-                                           const int8_t *tab, int8_t *slot, WalkQ *wq, int &wtail) {
+                                           const int8_t *tab, int8_t *slot, WalkQ *wq, int &wtail,
+                                           CoopQ *cq = nullptr) {
+  static_assert(!COOP || (GG == GWAVE && TB), "cooperative tasks are whole-wave, with walks");
   using L = Lane<P>;
   using V = typename L::V;
   constexpr int ST = GG * R;  // rows per strip
@@ -313,6 +351,9 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
   const int lane = threadIdx.x & 63;
   const int l = lane & (GG - 1);
   const int Kp = A.Kp;
+  const int wv = threadIdx.x >> 6;            // fill wave of the workgroup (cooperative tasks)
+  constexpr int SSTEP = COOP ? WALK_FW : 1;  // strips a wave advances by
+  int cepoch = 0;                            // cooperative tasks started (every wave agrees)
 
   const V vopen = L::splat(A.open);
   const V vext = L::splat(A.ext);
@@ -357,10 +398,84 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
   // 16-lane tasks of one shape (same strips and blocks), so the strips of the
   // four groups start and end together and a block's form (full / masked) is
   // the same for the whole wave.
+  // Cooperative task end, any wave: its strips' best cell per half to LDS,
+  // its stores completed, then its arrival.
+  auto coop_arrive = [&]() {
+    if constexpr (COOP) {
+      if constexpr (LOCAL) {
+#pragma unroll
+        for (int h = 0; h < P; ++h) {
+          int bh = h ? bestH1 : bestH0, bi = h ? besti1 : besti0, bj = h ? bestj1 : bestj0;
+#pragma unroll
+          for (int w = GG / 2; w >= 1; w >>= 1) {
+            const int oh = __shfl_xor(bh, w, 64), oi = __shfl_xor(bi, w, 64), oj = __shfl_xor(bj, w, 64);
+            const bool take = oh > bh || (oh == bh && (oi < bi || (oi == bi && oj < bj)));
+            if (take) { bh = oh; bi = oi; bj = oj; }
+          }
+          if (lane == 0) {
+            cq->best[wv][h][0] = bh; cq->best[wv][h][1] = bi; cq->best[wv][h][2] = bj;
+          }
+        }
+      }
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): planes, ring and results stored
+      if (lane == 0) __hip_atomic_fetch_add(&cq->arrived, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  };
+  // Cooperative task end, leader (every wave has arrived): the best cell over
+  // all waves' strips (H desc, i asc, j asc, as within a wave), the result,
+  // and the pair handed to the leader's walker.
+  auto coop_finish = [&]() {
+    if constexpr (COOP) {
+      if constexpr (LOCAL) {
+#pragma unroll
+        for (int h = 0; h < P; ++h) {
+          if (h == 1 && single) continue;
+          int bh = -1, bi = 0, bj = 0;
+#pragma unroll
+          for (int w = 0; w < WALK_FW; ++w) {
+            const int oh = cq->best[w][h][0], oi = cq->best[w][h][1], oj = cq->best[w][h][2];
+            const bool take = oh > bh || (oh == bh && (oi < bi || (oi == bi && oj < bj)));
+            if (take) { bh = oh; bi = oi; bj = oj; }
+          }
+          if (lane == 0) {
+            const int off = h ? off1 : off0;
+            FillOut o;
+            o.score = bh;
+            o.end_i = bh > 0 ? bi + 1 : 0;
+            o.end_j = bh > 0 ? bj - off + 1 : 0;
+            o.flags = (P == 1 ? 1 : 0) | (P == 2 && bh > A.sat_limit ? 2 : 0) | (bh == 0 ? 8 : 0) | flags_or;
+            A.out[h ? pid1 : pid0] = o;
+          }
+        }
+      }
+      const int sg[1] = {single}, a0[1] = {pid0}, a1[1] = {pid1};
+      walk_push<1>(wq, wtail, sg, a0, a1);
+    }
+  };
+
   auto fetch = [&]() {
     int t = 0;
-    if (lane == 0) t = atomicAdd(ctr, 1);
-    t = __shfl(t, 0, 64);
+    if constexpr (COOP) {
+      if (wv == 0) {
+        if (cepoch > 0) {
+          coop_wait(A, &cq->arrived, WALK_FW * cepoch);
+          coop_finish();
+        }
+        if (lane == 0) t = atomicAdd(ctr, 1);
+        t = __shfl(t, 0, 64);
+        if (lane < 8) cq->prog[lane] = 0;
+        if (lane == 0) cq->task = t;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_store(&cq->epoch, cepoch + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else {
+        coop_wait(A, &cq->epoch, cepoch + 1);
+        t = __hip_atomic_load(&cq->task, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      ++cepoch;
+    } else {
+      if (lane == 0) t = atomicAdd(ctr, 1);
+      t = __shfl(t, 0, 64);
+    }
     if (t >= ntasks) {
       alive = false;
       return;
@@ -374,7 +489,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     row_off = T.row_off; col_off = T.col_off; ring_off = T.ring_off; tb_off = T.tb_off;
     ring_len = T.ring_len;
     flags_or = T.flags_or;
-    s = 0; b = 0;
+    s = COOP ? wv : 0; b = 0;
     bestH0 = 0; besti0 = 0; bestj0 = 0; bestH1 = 0; besti1 = 0; bestj1 = 0;
   };
 
@@ -385,11 +500,19 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
 
   while (__any(alive)) {
     if (!alive) continue;
+    if constexpr (COOP) {
+      if (s >= S) {  // this wave's strips of the task are done
+        coop_arrive();
+        fetch();
+        continue;
+      }
+    }
 
     if (b == 0) {
       // ---- strip start: profile slot, left boundary, first block inputs.
-      const uint4 rc = s == 0 ? *(const uint4 *)(A.rowcodes + row_off + l * R) : rc_next;
-      if (s + 1 < S) rc_next = *(const uint4 *)(A.rowcodes + row_off + (uint64_t)(s + 1) * ST + l * R);
+      const uint4 rc = s < SSTEP ? *(const uint4 *)(A.rowcodes + row_off + (uint64_t)s * ST + l * R) : rc_next;
+      if (s + SSTEP < S)
+        rc_next = *(const uint4 *)(A.rowcodes + row_off + (uint64_t)(s + SSTEP) * ST + l * R);
       if constexpr (NIB) build_slot_nib(tab, (uint32_t *)slot, Kp, rc);
       else build_slot<P>(tab, slot, Kp, rc);
       const int row0 = s * ST + l * R;  // 0-based first row of this lane
@@ -415,6 +538,8 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
         if constexpr (P == 2) pb_n = *(const uint2 *)(slot + Kp * R + ((ccur >> 8) & 0xff) * R);
       }
       if (s > 0) {
+        // ring block 0 = the writer's block GG / 8
+        if constexpr (COOP) coop_wait(A, &cq->prog[(s - 1) & 7], coop_prog(s - 1, min(GG / 8 + 1, NB)));
         const uint32_t *rh = A.ring + ring_off + GG;
         const uint32_t *rf = rh + ring_len;
         nH0 = *(const uint4 *)rh; nH1 = *(const uint4 *)(rh + 4);
@@ -431,6 +556,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     rH0 = nH0; rH1 = nH1; rF0 = nF0; rF1 = nF1;
     if (b + 1 < NB) {  // prefetch the next block's lane-0 ring inputs
       if (s > 0) {
+        if constexpr (COOP) coop_wait(A, &cq->prog[(s - 1) & 7], coop_prog(s - 1, min(b + 1 + GG / 8 + 1, NB)));
         const uint32_t *rh = A.ring + ring_off + GG + 8 * (b + 1);
         const uint32_t *rf = rh + ring_len;
         nH0 = *(const uint4 *)rh; nH1 = *(const uint4 *)(rh + 4);
@@ -622,6 +748,8 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
       *(uint4 *)(rh + 4) = make_uint4(rbH[3], rbH[4], rbH[5], rbH[6]);
       *(uint4 *)rf = make_uint4(carryF, rbF[0], rbF[1], rbF[2]);
       *(uint4 *)(rf + 4) = make_uint4(rbF[3], rbF[4], rbF[5], rbF[6]);
+      if constexpr (COOP)  // the next strip (another wave) may read this block
+        __hip_atomic_store(&cq->prog[s & 7], coop_prog(s, b + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     carryH = rbH[7];
     carryF = rbF[7];
@@ -670,11 +798,11 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
         }
       }
     }
-    ++s;
+    s += SSTEP;
     b = 0;
     carryH = 0;
     carryF = 0;
-    if (s < S) continue;
+    if (COOP || s < S) continue;
 
     // ---- task end (local): reduce the group's best cells per half.
     if constexpr (LOCAL) {
@@ -732,13 +860,17 @@ __global__ __launch_bounds__(TB ? 128 * WALK_FW : 64) void fill_kernel(FillArgs 
   const int stride = fill_slot_stride(P, A.Kp, P == 2 && !TB);
   WalkQ *wq = nullptr;
   uint32_t *wlds = nullptr;
+  CoopQ *cq = nullptr;
   if constexpr (TB) {
     if (A.walk) {
       const int f = wave & (WALK_FW - 1);
       const int base = 1024 + WALK_FW * 64 * stride;
       wlds = (uint32_t *)(lds + base) + f * WALK_LANES * WSTRIDE;
-      wq = (WalkQ *)(lds + base + WALK_FW * WALK_LANES * WSTRIDE * 4) + f;
+      WalkQ *wq0 = (WalkQ *)(lds + base + WALK_FW * WALK_LANES * WSTRIDE * 4);
+      wq = wq0 + f;
+      cq = (CoopQ *)(wq0 + WALK_FW);
       if (wave < WALK_FW && lane == 0) { wq->tail = 0; wq->head = 0; wq->done = 0; }
+      if (threadIdx.x == 0) { cq->task = 0; cq->epoch = 0; cq->arrived = 0; }
     }
   }
   __syncthreads();
@@ -750,8 +882,17 @@ __global__ __launch_bounds__(TB ? 128 * WALK_FW : 64) void fill_kernel(FillArgs 
   }
   int8_t *slot = (int8_t *)lds + 1024 + threadIdx.x * stride;
   int wtail = 0;
-  // Long tasks first, one per wave; then 16-lane groups take the rest.
-  if (A.ntasks[0]) fill_phase<GWAVE, P, LOCAL, TB>(A, A.tasks[0], A.ntasks[0], A.ctr, tab, slot, wq, wtail);
+  // Cooperative tasks first (a workgroup each), then long tasks (a wave
+  // each), then 16-lane groups take the rest.
+  int ncoop = 0;
+  if constexpr (TB) {
+    if (A.walk && A.ncoop) {
+      ncoop = A.ncoop;
+      fill_phase<GWAVE, P, LOCAL, TB, true>(A, A.tasks[0], ncoop, A.coop_ctr, tab, slot, wq, wtail, cq);
+    }
+  }
+  if (A.ntasks[0] > ncoop)
+    fill_phase<GWAVE, P, LOCAL, TB>(A, A.tasks[0] + ncoop, A.ntasks[0] - ncoop, A.ctr, tab, slot, wq, wtail);
   if (A.ntasks[1]) fill_phase<G, P, LOCAL, TB>(A, A.tasks[1], A.ntasks[1], A.ctr + 1, tab, slot, wq, wtail);
   if constexpr (TB) {
     if (A.walk && lane == 0) __hip_atomic_store(&wq->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);

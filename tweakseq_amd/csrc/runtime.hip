@@ -498,6 +498,9 @@ struct ta_batch {
   // Walks inside the fill kernel (direction planes, no int32 rerun possible):
   // the traceback kernel then takes only the pairs no fill task holds.
   bool walk = false;
+  // Walking batches of few tasks: the first ncoop[P-1]... whole-wave tasks of
+  // each lane width are swept by a workgroup's fill waves together.
+  int32_t ncoop2 = 0, ncoop1 = 0;
   int32_t n_rest = 0;
   int32_t *d_order_rest = nullptr;
   std::vector<PairDev> pairs_h;
@@ -797,6 +800,21 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   auto by_cost = [](const Task &x, const Task &y) { return task_cost(x) > task_cost(y); };
   std::stable_sort(pl.tasks2[0].begin(), pl.tasks2[0].end(), by_cost);
   std::stable_sort(pl.tasks1[0].begin(), pl.tasks1[0].end(), by_cost);
+  // With few tasks the run lasts as long as its longest whole-wave task, so
+  // tasks of two or more strips get a workgroup's fill waves each (strips
+  // pipelined across them), as long as there are at most two per CU.
+  const char *cenv = getenv("TWEAKALIGN_COOP");  // "0": never
+  if (B->walk && few && !(cenv && cenv[0] == '0')) {
+    auto multi = [](const Task &t) { return t.S >= 2; };
+    const size_t c2 = std::count_if(pl.tasks2[0].begin(), pl.tasks2[0].end(), multi);
+    const size_t c1 = std::count_if(pl.tasks1[0].begin(), pl.tasks1[0].end(), multi);
+    if (c2 + c1 > 0 && c2 + c1 <= (size_t)(2 * ctx->cus)) {
+      std::stable_partition(pl.tasks2[0].begin(), pl.tasks2[0].end(), multi);
+      std::stable_partition(pl.tasks1[0].begin(), pl.tasks1[0].end(), multi);
+      B->ncoop2 = (int32_t)c2;
+      B->ncoop1 = (int32_t)c1;
+    }
+  }
 
   // Long pairs: strips of all pairs form one queue, largest pairs first.
   {
@@ -954,12 +972,19 @@ int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntas
   walk = walk && B->trace;
   a.walk = walk;
   if (walk) a.wk = tb_args(B);
+  a.ncoop = walk ? (P == 2 ? B->ncoop2 : B->ncoop1) : 0;
+  a.coop_ctr = B->d_ctr + (P == 2 ? 12 : 13);
+  a.err = B->d_ctr + 10;
+  a.spin_limit = B->spin_limit * 16;
   const int lds = walk ? walk_lds_bytes(P, B->Kp) : fill_lds_bytes(P, B->Kp, nib);
   const int per_cu = fill_waves_per_cu(P, B->Kp, nib, waves_wanted(B), walk);
   int groups = ntasks[0] + ntasks[1];  // one wave per whole-wave task or quad
   if (max_groups > 0) groups = std::min(groups, max_groups);
   groups = std::max(1, std::min(groups, ctx->cus * per_cu));
-  if (walk) groups = (groups + WALK_FW - 1) / WALK_FW;  // WALK_FW fill waves per workgroup
+  if (walk) {  // WALK_FW fill waves per workgroup; a cooperative task takes one
+    groups = a.ncoop + (groups - std::min(groups, a.ncoop) + WALK_FW - 1) / WALK_FW;
+    groups = std::max(1, std::min(groups, ctx->cus));
+  }
   const bool local = B->p.mode == TA_LOCAL;
   void (*k)(FillArgs) = nullptr;
   if (local) k = B->trace ? fill_kernel<P, true, true> : fill_kernel<P, true, false>;
@@ -1198,11 +1223,12 @@ int ta_batch_wait(ta_ctx *ctx, ta_batch *B) {
     if (hipEventElapsedTime(&t, B->ev[2], B->ev[3]) == hipSuccess) B->ms[2] = t;
   }
   if (hipEventElapsedTime(&t, B->ev[0], B->ev[3]) == hipSuccess) B->ms[3] = t;
-  if (!B->longs.empty()) {
+  if (!B->longs.empty() || B->ncoop2 || B->ncoop1) {
     int32_t err = 0;
-    HIPCHK(hipMemcpy(&err, B->d_ctr + 10, sizeof(err), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(&err, B->d_ctr + 10, sizeof(err), hipMemcpyDeviceToHost, B->stream));
+    HIPCHK(hipStreamSynchronize(B->stream));
     if (err) {
-      fprintf(stderr, "tweakalign: long-pair pipeline wait gave up (spin limit %d)\n", B->spin_limit);
+      fprintf(stderr, "tweakalign: a wait on another wave's progress gave up (code %d)\n", err);
       return TA_E_DEVICE;
     }
   }
