@@ -1520,6 +1520,7 @@ int ta_autotune(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n
   for (int32_t lanes : lanes_c) {
     const int P = lanes == TA_LANES_I32 ? 1 : 2;
     int last_eff = -1;
+    bool walk_done[2] = {false, false};  // walking fills ignore the wave count: measure one
     for (int waves : {4, 8, 12}) {
       const int eff = fill_waves_per_cu(P, Kp, P == 2 && !(p->mode == TA_LOCAL || p->traceback), waves);
       if (eff == last_eff) continue;
@@ -1529,6 +1530,11 @@ int ta_autotune(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n
         ta_batch *B = nullptr;
         int rc = batch_create_on(ctx, p, pairs, n, ctx->stream, &B, &cand);
         if (rc) return rc;
+        if (B->walk && walk_done[wt]) {
+          ta_batch_destroy(B);
+          continue;
+        }
+        if (B->walk) walk_done[wt] = true;
         double ms = 1e300;
         for (int r = 0; r <= repeats && !rc; ++r) {  // first run warms up
           rc = ta_batch_run(ctx, B);
