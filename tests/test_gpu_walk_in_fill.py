@@ -101,8 +101,9 @@ def run_env(eng, pairs, params, env, runs=2):
     return outs
 
 
+@pytest.mark.parametrize("lanes", [ta.LANES_AUTO, ta.LANES_I32])
 @pytest.mark.parametrize("mode", [ta.LOCAL, ta.GLOBAL])
-def test_cooperative_long_tasks(eng, mode):
+def test_cooperative_long_tasks(eng, mode, lanes):
     # few pairs, several 512-row strips each: a workgroup's fill waves sweep
     # each pair together (strips pipelined); one short pair rides along
     rng = random.Random(81 + mode)
@@ -111,7 +112,7 @@ def test_cooperative_long_tasks(eng, mode):
         a = rand_prot(rng, la)
         pairs.append((a, mutate(rng, a)))
     pairs.append((rand_prot(rng, 3000), rand_prot(rng, 2500)))  # unrelated: scores near 0
-    params = ta.engine.make_params(mode=mode)
+    params = ta.engine.make_params(mode=mode, lanes=lanes)
     coop = run_env(eng, pairs, params, {"TWEAKALIGN_COOP": "1"})
     plain = run_env(eng, pairs, params, {"TWEAKALIGN_COOP": "0"}, runs=1)[0]
     for r in coop:
