@@ -87,7 +87,8 @@ def oracle_inputs(workload, first, count):
 
 
 def cpu_single_pair(workload, repeats=1000):
-    """Config 1 on one core: median of `repeats` alignments with traceback."""
+    """The 300 x 320 global pair (workload 1) on one core: median of `repeats`
+    alignments with traceback."""
     from oracle import pyoracle as po
     p = workload.params
     op = po.params(mode=p.mode, alphabet=p.alphabet, gap_open=p.gap_open, gap_extend=p.gap_extend,
@@ -123,8 +124,9 @@ def cpu_long_dna(workload, npairs=4):
 
 
 def fill_traffic():
-    """HBM bytes per launch of the config-3 fill kernel, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
+    """HBM bytes per launch of the fill kernel on the 65,536-pair local batch
+    (workload 3), from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (None when absent)."""
     path = os.path.join(ROOT, "profiles", "fill_traffic_local_65536.json")
     try:
         with open(path) as f:
@@ -167,7 +169,7 @@ def main():
     t0 = time.perf_counter()
     b3 = eng.batch(w3.pairs_struct(), w3.params)
     info = b3.info()
-    log("config3: %d pairs, %.4g cells, %d tasks, %.3g GB direction planes, setup %.1f s"
+    log("local batch: %d pairs, %.4g cells, %d tasks, %.3g GB direction planes, setup %.1f s"
         % (w3.npairs, info["cells"], info["tasks"], info["tb_bytes"] / 1e9, time.perf_counter() - t0))
     wall, fill_ms, run_ms = timed_steps(b3, args.steps, args.warmup)
     cells = info["cells"]

@@ -124,7 +124,7 @@ def test_long_dna_workload_prefix(eng):
         b_off.append(pos); lb.append(len(eb)); codes.append(eb); pos += len(eb)
     res, cigs = po.align_batch(p, np.concatenate(codes), a_off, la, b_off, lb)
     want = [(r.score, r.a_begin, r.a_end, r.b_begin, r.b_end, c) for r, c in zip(res, cigs)]
-    compare(got, want, params, pairs, "config5 prefix")
+    compare(got, want, params, pairs, "long DNA pairs, prefix")
     assert all(g.flags & ta.F_LONG for g in got)
 
 

@@ -161,13 +161,13 @@ def test_batched_equals_plain(eng):
         assert batched == plain
 
 
-def test_workload_config1(eng):
+def test_workload_single_pair_300x320(eng):
     w = ta.Workload(1)
     a, b = w.pair_bytes(0)
-    check(eng, [(a, b)], "config1", mode=w.params.mode, gap_open=w.params.gap_open)
+    check(eng, [(a, b)], "300 x 320 global pair", mode=w.params.mode, gap_open=w.params.gap_open)
 
 
-def test_workload_config3_prefix(eng):
+def test_workload_local_batch_prefix(eng):
     w = ta.Workload(3, size=1500)
     pairs = [w.pair_bytes(k) for k in range(w.npairs)]
-    check(eng, pairs, "config3", mode=ta.LOCAL, gap_open=-11, gap_extend=-1)
+    check(eng, pairs, "65,536-pair local batch, first 1,500", mode=ta.LOCAL, gap_open=-11, gap_extend=-1)

@@ -72,7 +72,7 @@ def test_walk_in_fill_few_pairs_and_graph(eng):
         compare(r, ref, params, pairs, "walk in fill, graph replay")
 
 
-def test_walk_in_fill_config3_prefix(eng):
+def test_walk_in_fill_local_batch_prefix(eng):
     w = ta.Workload(3, size=4000)
     pairs = [w.pair_bytes(k) for k in range(w.npairs)]
     params = w.params
