@@ -225,7 +225,8 @@ struct Walker {
       rb1 = *(const uint4 *)(pb + bbase + 16);
     }
     // ---- up to EPOCH steps inside the two windows
-    for (int e = 0; e < EPOCH && guard > 0; ++e, --guard) {
+    int e = 0;
+    for (; e < EPOCH && guard > 0; ++e, --guard) {
       if (!local && (i == 0 || j == 0)) break;  // boundary handled by the epoch head
       const int t = j - 1 + off + l;
       const int lane_now = s * GG + l;
@@ -268,6 +269,10 @@ struct Walker {
         if (local && g == 0 && (h == 0 || i == 0 || j == 0)) { done = true; break; }
       }
     }
+    // An epoch starts inside its own window, so it always takes a step; one
+    // that could not (a start cell outside the planes) ends the walk,
+    // flagged, rather than looping.
+    if (e == 0 && !done) guard = 0;
     if (done || guard <= 0) {
       finish(A);
       return true;
