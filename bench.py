@@ -305,6 +305,30 @@ def main():
             % (c5, out["long_dna_ms"], out["long_dna_gcups"], out["long_dna_fill_gcups"], t5[2]))
 
     if not args.no_extra:
+        # Column consensus (Consensus::calculate) of a 512 x 2,000 alignment:
+        # ta_consensus end to end (rows from host memory, consensus back),
+        # and the oracle's O(cols x nseq^2) restatement on one core.
+        import random as _r
+        rng = _r.Random(5)
+        base = [rng.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(2000)]
+        rows = []
+        for _ in range(512):
+            r = [c if rng.random() > 0.25 else rng.choice("ARNDCQEGHILKMFPSTWYV-") for c in base]
+            rows.append("".join(r).encode())
+        eng.consensus(rows)
+        ct = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            got = eng.consensus(rows)
+            ct.append(time.perf_counter() - t0)
+        out["consensus_512x2000_ms"] = round(sorted(ct)[2] * 1e3, 3)
+        if not args.no_cpu:
+            from oracle import pyoracle as po
+            t0 = time.perf_counter()
+            want = po.consensus(rows)
+            out["consensus_512x2000_cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+            out["consensus_matches_oracle"] = got == want
+        log("consensus 512 x 2000: %.3f ms on the GPU (end to end)" % out["consensus_512x2000_ms"])
         # Served form: open-loop Poisson arrivals (2,000 requests/s, 1-256
         # pairs each, lengths 50-5,000, global/local 50/50) through
         # ta_submit/ta_poll in a child process; latency = completion minus

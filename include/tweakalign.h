@@ -170,6 +170,17 @@ int ta_matrix(int32_t alphabet, int32_t match, int32_t mismatch, int8_t *out, in
  * residues tweakseq exports keep them, tweakseq/Core/Sequence.cpp:57-69). */
 int ta_encode(int32_t alphabet, const char *s, int32_t len, uint8_t *out);
 
+/* Column consensus of an alignment on the GPU: replaces
+ * Consensus::calculate (tweakseq/Core/Annotations/Consensus.cpp:80-161).
+ * rows[0..nseq) are the aligned sequences as tweakseq holds them (gaps
+ * kept, latin1), each at least len characters; out[c] gets the character of
+ * the first sequence whose residue scores best against the column
+ * (BLOSUM62 for two letters 'A'..'Z', +1 for two non-letters, -4 for one of
+ * each, summed over the other sequences), or '?' when that residue's
+ * positive pairings number fewer than `plurality` (< 0: nseq / 2, the
+ * reference's default with unit weights, Consensus.cpp:163-173). */
+int ta_consensus(ta_ctx *ctx, const char *const *rows, int32_t nseq, int32_t len, double plurality, char *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,14 @@ int or_rescore(const or_params *p, const int8_t *smat, int32_t K,
                const uint8_t *a, const uint8_t *b, const or_result *res,
                const char *cigar, int32_t *score);
 
+/* Column consensus of an alignment of nseq rows of len characters (msa is
+ * row-major, row s at msa + s * len), restating Consensus::calculate
+ * (tweakseq/Core/Annotations/Consensus.cpp:80-161) with the 23 x 23 BLOSUM62
+ * table and 26-letter map passed in; plurality < 0 means nseq / 2.  Writes
+ * out[0..len).  Returns 0, -1 on bad arguments, -5 on allocation failure. */
+int or_consensus(const int8_t *smat23, const int32_t *map26, const char *msa, int32_t nseq, int32_t len,
+                 double plurality, char *out);
+
 #ifdef __cplusplus
 }
 #endif

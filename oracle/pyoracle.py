@@ -47,6 +47,8 @@ def lib():
         L.or_rescore.argtypes = [P(Params), P(ctypes.c_int8), ctypes.c_int32,
                                  P(ctypes.c_uint8), P(ctypes.c_uint8), P(Result),
                                  ctypes.c_char_p, P(ctypes.c_int32)]
+        L.or_consensus.argtypes = [P(ctypes.c_int8), P(ctypes.c_int32), ctypes.c_char_p, ctypes.c_int32,
+                                   ctypes.c_int32, ctypes.c_double, ctypes.c_char_p]
         _LIB = L
     return _LIB
 
@@ -162,3 +164,18 @@ def rescore(p, a, b, result_tuple, smat=None):
     if rc:
         raise ValueError("CIGAR does not consume the reported ranges")
     return out.value
+
+
+def consensus(rows, plurality=-1.0):
+    """Consensus::calculate on equal-length rows (bytes or str); returns bytes."""
+    rows = [r.encode() if isinstance(r, str) else bytes(r) for r in rows]
+    n, L = len(rows), len(rows[0]) if rows else 0
+    assert all(len(r) == L for r in rows)
+    smat, lmap = protein_matrix()
+    out = ctypes.create_string_buffer(max(L, 1))
+    rc = lib().or_consensus(smat.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                            lmap.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), b"".join(rows), n, L,
+                            float(plurality), out)
+    if rc:
+        raise RuntimeError("or_consensus: %d" % rc)
+    return out.raw[:L]

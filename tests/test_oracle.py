@@ -138,3 +138,76 @@ def test_batch_matches_single():
     for k, (i, j) in enumerate(pairs):
         one = po.align(p, seqs[i], seqs[j])
         assert (res[k].score, res[k].a_begin, res[k].a_end, res[k].b_begin, res[k].b_end, cig[k]) == one
+
+
+# ---- Consensus::calculate (Consensus.cpp:80-161)
+
+CONS_CHARS = "ACDEFGHIKLMNPQRSTVWYBZXJOU-.acdefgw?*1"
+
+
+def consensus_by_counts(rows, plurality=-1.0):
+    """Per-column class counts, then one score per class present (an
+    algorithm of its own: the oracle loops over all pairs of sequences)."""
+    d = po.blosum62_fixture()
+    S, lmap = d["matrix"], d["map"]
+    n = len(rows)
+    if plurality < 0:
+        plurality = n / 2.0
+
+    def cls(ch):
+        x = ord(ch) - 65
+        return lmap[x] if 0 <= x <= 25 else 99
+
+    def pair(a, b):
+        if a == 99 and b == 99:
+            return 1, 1
+        if a == 99 or b == 99:
+            return -4, 0
+        return S[a][b], int(S[a][b] > 0)
+
+    out = []
+    for c in range(len(rows[0])):
+        col = [cls(r[c]) for r in rows]
+        counts = {}
+        for k in col:
+            counts[k] = counts.get(k, 0) + 1
+        per = {}
+        for k in counts:
+            sc = mt = 0
+            for j, cj in counts.items():
+                s, m = pair(k, j)
+                sc += cj * s
+                mt += cj * m
+            s, m = pair(k, k)
+            per[k] = (sc - s, mt - m)
+        best = max(per[k][0] for k in per)
+        ri = next(i for i, k in enumerate(col) if per[k][0] == best)
+        out.append(rows[ri][c] if per[col[ri]][1] >= plurality else "?")
+    return "".join(out)
+
+
+def test_consensus_closed_forms():
+    # identical rows: every pairing is the diagonal score, positive
+    assert po.consensus(["ACDEW", "ACDEW", "ACDEW"]) == b"ACDEW"
+    # one row: no pairings, no matches, below the default plurality 0.5
+    assert po.consensus(["ACD"]) == b"???"
+    # gaps pair +1 with gaps: the gap wins a gap column
+    assert po.consensus(["-", "-", "-"]) == b"-"
+    # W W A: W scores 11 - 3 = 8 (twice), A scores -6; W's matches: 1 < 1.5
+    assert po.consensus(["W", "W", "A"]) == b"?"
+    assert po.consensus(["W", "W", "A"], plurality=1.0) == b"W"
+    # ties go to the first sequence: A/A/S/S, A and S both score 4 + 1 + 1 = 6
+    assert po.consensus(["A", "S", "A", "S"]) == b"A"
+    assert po.consensus(["S", "A", "A", "S"]) == b"S"
+    # lowercase and other bytes are the non-letter class; the winner keeps its byte
+    assert po.consensus(["a", "-", "."]) == b"a"
+
+
+def test_consensus_matches_counting():
+    rng = random.Random(31)
+    for _ in range(60):
+        n, L = rng.randint(1, 40), rng.randint(1, 60)
+        alpha = CONS_CHARS[: rng.randint(3, len(CONS_CHARS))]
+        rows = ["".join(rng.choice(alpha) for _ in range(L)) for _ in range(n)]
+        pl = rng.choice([-1.0, 0.0, 1.0, n / 3.0, float(n)])
+        assert po.consensus(rows, pl).decode("latin-1") == consensus_by_counts(rows, pl)

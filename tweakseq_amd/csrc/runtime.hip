@@ -248,6 +248,10 @@ void dfree(T *&p) {
 
 }  // namespace
 
+// The device-memory cache for the library's other sources (runtime_internal.h).
+void *device_alloc(size_t bytes) { return cache_alloc(bytes); }
+void device_free(void *p) { cache_free(p); }
+
 // ---------------------------------------------------------------------------
 const char *ta_strerror(int code) {
   switch (code) {

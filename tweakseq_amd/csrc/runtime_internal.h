@@ -37,5 +37,9 @@ int64_t batch_tb_estimate(const ta_params *p, const ta_pair *pairs, int32_t n);
 // Results of a finished batch with the CIGAR text in one string
 // (ta_result::cigar_off indexes `text`): ta_batch_results in one pass.
 int batch_results_text(ta_ctx *ctx, ta_batch *B, std::vector<ta_result> &out, std::string &text);
+// Device memory from the library's cache (blocks are reused, never handed to
+// hipFree while work runs; device_free returns them to the cache).
+void *device_alloc(size_t bytes);
+void device_free(void *p);
 // Stops the serving thread (called by ta_destroy).
 void serve_shutdown(ta_ctx *ctx);

@@ -75,6 +75,7 @@ def lib():
         "ta_cancel": ([vp, ctypes.c_uint64], ctypes.c_int),
         "ta_matrix": ([i32, i32, i32, P(ctypes.c_int8), P(i32)], ctypes.c_int),
         "ta_encode": ([i32, ctypes.c_char_p, i32, P(ctypes.c_uint8)], ctypes.c_int),
+        "ta_consensus": ([vp, P(ctypes.c_char_p), i32, i32, ctypes.c_double, ctypes.c_char_p], ctypes.c_int),
         "ta_workload_create": ([i32, ctypes.c_uint64, i32, i32, P(vp)], ctypes.c_int),
         "ta_workload_destroy": ([vp], None),
         "ta_workload_npairs": ([vp], i32),
@@ -224,6 +225,19 @@ class Engine:
             return b.results()
         finally:
             b.close()
+
+    def consensus(self, rows, plurality=-1.0):
+        """Column consensus of aligned rows (equal lengths; bytes or str) on
+        the GPU, as Consensus::calculate computes it; returns bytes."""
+        rows = [r.encode("latin-1") if isinstance(r, str) else bytes(r) for r in rows]
+        n = len(rows)
+        L = len(rows[0]) if rows else 0
+        if any(len(r) != L for r in rows):
+            raise ValueError("rows of unequal length")
+        arr = (ctypes.c_char_p * max(n, 1))(*rows)
+        out = ctypes.create_string_buffer(max(L, 1))
+        _check(lib().ta_consensus(self.ctx, arr, n, L, float(plurality), out), "ta_consensus")
+        return out.raw[:L]
 
     # autotuning
     def autotune(self, pairs, params, repeats=2):
