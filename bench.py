@@ -263,6 +263,7 @@ def main():
         bp.close()
         plain = pcells / pdt / 1e9
         out["plain_gcups"] = round(plain, 3)
+        out["plain_frac_of_roofline"] = round(plain * OPS_PER_CELL[1] / VALU_PEAK_GOPS, 6)
         out["batched_vs_plain"] = round(gcups / plain, 1)
         out["plain_sample"] = "first %d pairs, one fill launch + one traceback launch per pair" % npl
         log("plain: %.3f GCUPS on %d pairs -> batched/plain %.1fx" % (plain, npl, gcups / plain))
@@ -339,6 +340,10 @@ def main():
             r = subprocess.run([sb, "4000"], capture_output=True, text=True, timeout=300)
             if r.returncode == 0:
                 out["served"] = json.loads(r.stdout.strip().splitlines()[-1])
+                # served GCUPS follow the offered load (2,000 requests/s); the
+                # fraction is of the GPU's int32 roofline at 9.5 ops per cell
+                # (half global, half local)
+                out["served"]["frac_of_roofline"] = round(out["served"]["served_gcups"] * 9.5 / VALU_PEAK_GOPS, 4)
                 log("served: p50 %.1f ms, p99 %.1f ms, %.1f GCUPS served of %.1f offered"
                     % (out["served"]["p50_ms"], out["served"]["p99_ms"], out["served"]["served_gcups"],
                        out["served"]["offered_gcups"]))
