@@ -193,11 +193,51 @@ struct Walker {
     A.res[p] = r;
   }
 
-  // One epoch; true when the walk has ended and its result is written.
-  __device__ __forceinline__ bool epoch(const TbArgs &A, const int8_t *stab, uint32_t *slice) {
-    const bool local = A.local != 0;
-    const int open = A.open, ext = A.ext;
-    if (!local) {
+  // Direction code (e f d x in bits 0..3) of row k in a lane's block entry.
+  __device__ __forceinline__ int code_at(const uint32_t *entry) const {
+    if constexpr (p2) {
+      const uint2 v = *(const uint2 *)entry;
+      const int sh = hsh + k;
+      return (int)(((v.x >> sh) & 1) | (((v.x >> (16 + sh)) & 1) << 1) | (((v.y >> sh) & 1) << 2) |
+                   (((v.y >> (16 + sh)) & 1) << 3));
+    } else {
+      const uint32_t acc = *entry;
+      const int pbit = 31 - 4 * k;  // pushes e, f, d, x of row k land at pbit .. pbit-3
+      return (int)(((acc >> pbit) & 1) | (((acc >> (pbit - 1)) & 1) << 1) | (((acc >> (pbit - 2)) & 1) << 2) |
+                   (((acc >> (pbit - 3)) & 1) << 3));
+    }
+  }
+
+  // One step from the current cell with its direction code: the op (M 0,
+  // D 1, I 2; from state H a gap source is taken at once with the same
+  // cell's extend bit), the carried score (local; `diag` gives the cell's
+  // substitution score), the run, the move.  True when a local walk has
+  // entered a cell whose H is 0 (or a border): the walk is over.
+  template <class Diag>
+  __device__ __forceinline__ bool take(int code, bool local, int open, int ext, Diag diag) {
+    const int g = state ? state : ((code & 4) ? ((code & 8) ? 2 : 1) : 0);
+    const int xb = g == 1 ? (code & 1) : (g == 2 ? ((code >> 1) & 1) : 0);
+    if (local) h -= g == 0 ? diag() : (xb ? ext : open);
+    cw.emit(g, 1);
+    state = xb ? g : 0;
+    if (g != 2) --j;
+    if (g != 1) {
+      --i;
+      if (k > 0) --k;
+      else {
+        k = 7;
+        if (l > 0) --l;
+        else { l = GG - 1; --s; }
+      }
+      if (local && g == 0 && (h == 0 || i == 0 || j == 0)) return true;
+    }
+    return false;
+  }
+
+  // The walk's head: borders that end it.  True when it is over (result
+  // written).
+  __device__ __forceinline__ bool at_border(const TbArgs &A) {
+    if (!A.local) {
       if (i == 0) { if (j) cw.emit(1, j); finish(A); return true; }
       if (j == 0) { cw.emit(2, i); finish(A); return true; }
     } else if (i <= 0 || j <= 0) {
@@ -205,6 +245,14 @@ struct Walker {
       finish(A);
       return true;
     }
+    return false;
+  }
+
+  // One epoch; true when the walk has ended and its result is written.
+  __device__ __forceinline__ bool epoch(const TbArgs &A, const int8_t *stab, uint32_t *slice) {
+    const bool local = A.local != 0;
+    const int open = A.open, ext = A.ext;
+    if (at_border(A)) return true;
     // ---- epoch loads
     const int t0 = j - 1 + off + l;
     const int fA = first_block(t0);
@@ -234,39 +282,12 @@ struct Walker {
       if (lane_now == laneA && t >= 8 * fA && t < 8 * fA + 16) { wsel = 0; wi = t - 8 * fA; }
       else if (hasB && lane_now == laneB && t >= 8 * fB && t < 8 * fB + 16) { wsel = 32; wi = t - 8 * fB; }
       else break;  // left both windows: next epoch
-      int code;
-      if constexpr (p2) {
-        const uint2 v = *(const uint2 *)(slice + wsel + (wi >> 3) * 16 + (wi & 7) * 2);
-        const int sh = hsh + k;
-        code = (int)(((v.x >> sh) & 1) | (((v.x >> (16 + sh)) & 1) << 1) | (((v.y >> sh) & 1) << 2) |
-                     (((v.y >> (16 + sh)) & 1) << 3));
-      } else {
-        const uint32_t acc = slice[wsel + (wi >> 3) * 8 + (wi & 7)];
-        const int pbit = 31 - 4 * k;  // pushes e, f, d, x of row k land at pbit .. pbit-3
-        code = (int)(((acc >> pbit) & 1) | (((acc >> (pbit - 1)) & 1) << 1) | (((acc >> (pbit - 2)) & 1) << 2) |
-                     (((acc >> (pbit - 3)) & 1) << 3));
-      }
-      // Op of this step: M (0), D (1) or I (2); from state H a gap source is
-      // taken at once with the same cell's extend bit.
-      const int g = state ? state : ((code & 4) ? ((code & 8) ? 2 : 1) : 0);
-      const int xb = g == 1 ? (code & 1) : (g == 2 ? ((code >> 1) & 1) : 0);
-      if (local) {
-        const int sc = g == 0 ? (int)stab[win32_byte(ra0, ra1, i - 1 - abase) * 32 + win32_byte(rb0, rb1, j - 1 - bbase)]
-                              : (xb ? ext : open);
-        h -= sc;
-      }
-      cw.emit(g, 1);
-      state = xb ? g : 0;
-      if (g != 2) --j;
-      if (g != 1) {
-        --i;
-        if (k > 0) --k;
-        else {
-          k = 7;
-          if (l > 0) --l;
-          else { l = GG - 1; --s; }
-        }
-        if (local && g == 0 && (h == 0 || i == 0 || j == 0)) { done = true; break; }
+      const int code = code_at(slice + wsel + (wi >> 3) * (p2 ? 16 : 8) + (wi & 7) * (p2 ? 2 : 1));
+      if (take(code, local, open, ext, [&] {
+            return (int)stab[win32_byte(ra0, ra1, i - 1 - abase) * 32 + win32_byte(rb0, rb1, j - 1 - bbase)];
+          })) {
+        done = true;
+        break;
       }
     }
     // An epoch starts inside its own window, so it always takes a step; one
