@@ -359,6 +359,26 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
   const V vext = L::splat(A.ext);
   const V vzero = L::splat(0);
   const V vneg = L::splat(L::NEG);
+  // Local fills keep E and F floored at 0: the gap terms are unsigned
+  // saturating subtractions of |open| and |ext|, which gives E' = max(E, 0),
+  // F' = max(F, 0) for the same H (H = max(HdS, E, F, 0) = max(HdS, E', F')),
+  // so H needs no floor of its own.  The direction bits are those of the
+  // exact values wherever a walk reads them: every E (F) on a local path is
+  // > 0, and a difference of two values differs from the floored one's only
+  // where both are <= 0.
+  constexpr bool FLOOR = LOCAL;
+  const V vgo = FLOOR ? L::splat(-A.open) : vopen;  // gap terms as the gap op takes them
+  const V vge = FLOOR ? L::splat(-A.ext) : vext;
+  const V vgap0 = FLOOR ? vzero : vneg;             // E and F where no gap can be yet
+  auto gap = [](V x, V g) -> V {
+    if constexpr (FLOOR) {
+      typedef unsigned short u2v __attribute__((ext_vector_type(2)));
+      using U = std::conditional_t<P == 2, u2v, unsigned>;
+      return __builtin_bit_cast(V, __builtin_elementwise_sub_sat(__builtin_bit_cast(U, x), __builtin_bit_cast(U, g)));
+    } else {
+      return L::add(x, g);
+    }
+  };
 
   // Group state (identical in the 16 lanes of a group).
   bool alive = true;
@@ -389,7 +409,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
   int carryH = 0, carryF = 0;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    H[k] = vzero; E[k] = vneg; HoL[k] = vzero;
+    H[k] = vzero; E[k] = vgap0; HoL[k] = vzero;
     rmax[k] = 0; rcol[k] = 0;
     if (k < R / 2) { kp0[k] = 0; kp1[k] = 0; }
   }
@@ -519,8 +539,8 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         H[k] = L::splat(left(row0 + k + 1));
-        E[k] = vneg;
-        HoL[k] = L::add(H[k], vopen);
+        E[k] = vgap0;
+        HoL[k] = gap(H[k], vgo);
         rmax[k] = 0; rcol[k] = 0;
         if (k < R / 2) { kp0[k] = 0; kp1[k] = 0; }
       }
@@ -573,7 +593,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         rhw[q] = (uint32_t)L::bits(L::splat(LOCAL ? 0 : A.open + (8 * b + q) * A.ext));
-        rfw[q] = (uint32_t)L::bits(vneg);
+        rfw[q] = (uint32_t)L::bits(vgap0);
       }
     } else {
       rhw[0] = rH0.x; rhw[1] = rH0.y; rhw[2] = rH0.z; rhw[3] = rH0.w;
@@ -627,7 +647,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
           uint32_t ke0 = 0, ke1 = 0;
 
           V hd = hd0;
-          V HoU = L::add(Hup, vopen);
+          V HoU = gap(Hup, vgo);
           V F = Fup;
           uint32_t xc = 0, yc = 0, acc = 0;
 #pragma unroll
@@ -646,14 +666,14 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
               Sk = L::unbits(prof1(pa, k));
             }
             const V HdS = L::add(hd, Sk);
-            const V Eext = L::add(E[k], vext);
+            const V Eext = gap(E[k], vge);
             const V En = L::vmax(HoL[k], Eext);
-            const V Fext = L::add(F, vext);
+            const V Fext = gap(F, vge);
             const V Fn = L::vmax(HoU, Fext);
             const V EF = L::vmax(En, Fn);
             V Hn = L::vmax(HdS, EF);
-            if constexpr (LOCAL) Hn = L::vmax(Hn, vzero);
-            const V Hon = L::add(Hn, vopen);
+            if constexpr (LOCAL && !FLOOR) Hn = L::vmax(Hn, vzero);
+            const V Hon = gap(Hn, vgo);
             if constexpr (TB) {
               // Signs of: H+open - E (e: E extended), H+open - F (f), the
               // diagonal - H (d: H not from the diagonal), E - F (x: F above
