@@ -19,7 +19,7 @@ def eng():
 
 
 def mixed_pairs(seed, n):
-    # many short pairs (16-lane quads) and some long ones (whole-wave tasks)
+    # many short pairs (G-lane groups) and some long ones (whole-wave tasks)
     rng = random.Random(seed)
     out = []
     for k in range(n):

@@ -3,12 +3,15 @@
 // Layout of the work (no counterpart in the reference, which spawns an
 // external aligner; recurrence, boundaries and tie-breaks follow the
 // specification the CPU oracle implements, oracle/gotoh.c):
-//  * a wave64 = 4 DP groups of 16 lanes (one DPP row each); a group owns a
-//    task and sweeps it strip by strip;
-//  * a strip is 16 lanes x R query rows held in registers; lane l works on
-//    column j = t - l at step t, so a group covers an anti-diagonal band and
-//    H/F of a lane's bottom row reach the next lane one step later through
-//    DPP row_shr:1 (lane 0 takes the strip's top boundary instead);
+//  * a wave64 = NGW DP groups of G lanes (16: one DPP row each; 8: two per
+//    row, interleaved) or, for long tasks, one whole-wave group; a group owns
+//    a task and sweeps it strip by strip;
+//  * a strip is G lanes x R query rows held in registers (64 lanes for a
+//    whole-wave group); lane l works on column j = t - l at step t, so a
+//    group covers an anti-diagonal band and H/F of a lane's bottom row reach
+//    the next lane one step later through DPP row_shr:1 (row_shr:2 for 8-lane
+//    groups, wave_shr:1 for the whole wave; lane 0 takes the strip's top
+//    boundary instead);
 //  * the bottom row of a strip goes to a ring in global memory and is read
 //    back as the top boundary of the next strip;
 //  * P == 2 packs two pairs ("a couple") into the int16 halves of every
@@ -68,8 +71,8 @@ struct Lane<1> {
 };
 
 struct FillArgs {
-  const Task *tasks[2];  // [0] whole-wave tasks (GG = 64), [1] quads of 16-lane tasks
-  int32_t ntasks[2];     // tasks, quads
+  const Task *tasks[2];  // [0] whole-wave tasks (GG = 64), [1] wave units of NGW G-lane tasks
+  int32_t ntasks[2];     // tasks, wave units
   int32_t *ctr;          // work-queue heads ctr[0], ctr[1]
   const int8_t *table;   // Kp x Kp, row/column Kp-1 = pad
   int32_t Kp;
@@ -128,7 +131,7 @@ __host__ __device__ constexpr int walk_lds_bytes(int P, int Kp) {
 }
 
 // Fill side, at a task's end: append its pairs to the ring.  NG groups
-// (1 for a whole-wave task, 4 for a quad) with their `single` codes and pair
+// (1 for a whole-wave task, NGW for a wave unit) with their `single` codes and pair
 // ids as seen by every lane; a filler (single 2) has no pair, a single
 // (1) only its lo pair.
 template <int NG>
@@ -204,11 +207,34 @@ __device__ __forceinline__ void walk_loop(const FillArgs &A, WalkQ *q, const int
   }
 }
 
-// Shift one lane up inside a group: DPP row_shr:1 (16-lane rows) or
-// wave_shr:1 (whole wave); the group's lane 0 takes `old`.
+// Lanes of a group.  A 16-lane group is a DPP row.  An 8-lane group takes
+// every other lane of a row (two groups per row, interleaved: group lane l at
+// row lane 2l or 2l + 1), so that one row_shr:2 moves every lane of both
+// groups up one and leaves both groups' lane 0 to `old`.  A whole-wave group
+// is the wave.
+template <int GG>
+__device__ __forceinline__ int group_lane(int lane) {
+  static_assert(GG == 8 || GG == 16 || GG == GWAVE, "group width");
+  return GG == 8 ? (lane & 15) >> 1 : lane & (GG - 1);
+}
+// Group of a lane, and the wave lane of lane 0 of group g.
+template <int GG>
+__device__ __forceinline__ int group_of(int lane) {
+  return GG == 8 ? ((lane >> 4) << 1) | (lane & 1) : lane / GG;
+}
+template <int GG>
+__device__ __forceinline__ int group_leader(int g) {
+  return GG == 8 ? ((g >> 1) << 4) | (g & 1) : g * GG;
+}
+// Wave-lane distance between group lanes l and l ^ w.
+template <int GG>
+__device__ __forceinline__ int group_xor(int w) { return GG == 8 ? 2 * w : w; }
+
+// Shift one lane up inside a group (DPP row_shr:2 / row_shr:1 / wave_shr:1);
+// the group's lane 0 takes `old`.
 template <int GG>
 __device__ __forceinline__ int shr1(int old, int v) {
-  return __builtin_amdgcn_update_dpp(old, v, GG == 16 ? 0x111 : 0x138, 0xf, 0xf, false);
+  return __builtin_amdgcn_update_dpp(old, v, GG == GWAVE ? 0x138 : (GG == 16 ? 0x111 : 0x112), 0xf, 0xf, false);
 }
 
 // LDS bytes the fill kernel needs for a table of Kp codes: per lane, R byte
@@ -349,7 +375,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
   constexpr int ST = GG * R;  // rows per strip
   constexpr bool NIB = P == 2 && !TB;  // 4-bit profile slot
   const int lane = threadIdx.x & 63;
-  const int l = lane & (GG - 1);
+  const int l = group_lane<GG>(lane);
   const int Kp = A.Kp;
   const int wv = threadIdx.x >> 6;            // fill wave of the workgroup (cooperative tasks)
   constexpr int SSTEP = COOP ? WALK_FW : 1;  // strips a wave advances by
@@ -380,7 +406,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     }
   };
 
-  // Group state (identical in the 16 lanes of a group).
+  // Group state (identical in the lanes of a group).
   bool alive = true;
   int task = -1, s = 0, b = 0;
   int n = 0, S = 0, NB = 0, single = 0;
@@ -414,10 +440,10 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     if (k < R / 2) { kp0[k] = 0; kp1[k] = 0; }
   }
 
-  // A wave takes one unit of work: a whole-wave task, or a quad of four
-  // 16-lane tasks of one shape (same strips and blocks), so the strips of the
-  // four groups start and end together and a block's form (full / masked) is
-  // the same for the whole wave.
+  // A wave takes one unit of work: a whole-wave task, or NGW G-lane tasks of
+  // one shape (same strips and blocks), so the strips of the groups start and
+  // end together and a block's form (full / masked) is the same for the
+  // whole wave.
   // Cooperative task end, any wave: its strips' best cell per half to LDS,
   // its stores completed, then its arrival.
   auto coop_arrive = [&]() {
@@ -500,7 +526,7 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
       alive = false;
       return;
     }
-    if constexpr (GG == G) t = 4 * t + (lane >> 4);
+    if constexpr (GG != GWAVE) t = (GWAVE / GG) * t + group_of<GG>(lane);
     task = t;
     const Task &T = tasks[t];
     n = T.n; S = T.S; NB = T.NB; single = T.single;
@@ -586,8 +612,8 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     const uint32_t ccw[4] = {ccodes.x, ccodes.y, ccodes.z, ccodes.w};
     // Lane 0's top row of each step: the previous strip's bottom row from the
     // ring or, in the first strip, the boundary.  One branch per block: s is
-    // the same in every lane of a wave (a whole-wave task, or a quad of one
-    // shape).
+    // the same in every lane of a wave (a whole-wave task, or a unit of
+    // tasks of one shape).
     uint32_t rhw[8], rfw[8];
     if (__builtin_amdgcn_readfirstlane(s) == 0) {
 #pragma unroll
@@ -743,8 +769,9 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     if (__all(inside)) steps(std::true_type{});
     else steps(std::false_type{});
 
-    // ---- block outputs: direction planes (all lanes), ring (lane 15); a
-    // quad's filler group writes nothing (it shares the offsets of a task).
+    // ---- block outputs: direction planes (all lanes), ring (the group's
+    // last lane); a filler group writes nothing (it shares the offsets of a
+    // task).
     if (TB && single != 2) {
       if constexpr (P == 2) {
         uint4 *dst = (uint4 *)(A.tb + tb_off + ((((uint64_t)s * NB + b) * GG + l) * 64));
@@ -831,7 +858,8 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
         int bh = h ? bestH1 : bestH0, bi = h ? besti1 : besti0, bj = h ? bestj1 : bestj0;
 #pragma unroll
         for (int w = GG / 2; w >= 1; w >>= 1) {
-          const int oh = __shfl_xor(bh, w, 64), oi = __shfl_xor(bi, w, 64), oj = __shfl_xor(bj, w, 64);
+          const int x = group_xor<GG>(w);
+          const int oh = __shfl_xor(bh, x, 64), oi = __shfl_xor(bi, x, 64), oj = __shfl_xor(bj, x, 64);
           const bool take = oh > bh || (oh == bh && (oi < bi || (oi == bi && oj < bj)));
           if (take) { bh = oh; bi = oi; bj = oj; }
         }
@@ -848,13 +876,13 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
     }
     if constexpr (TB) {
       if (A.walk) {  // hand the task's pairs to the walker wave
-        constexpr int NG = GG == GWAVE ? 1 : 4;
+        constexpr int NG = GWAVE / GG;
         int sg[NG], a0[NG], a1[NG];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-          sg[g] = NG == 1 ? single : __shfl(single, 16 * g, 64);
-          a0[g] = NG == 1 ? pid0 : __shfl(pid0, 16 * g, 64);
-          a1[g] = NG == 1 ? pid1 : __shfl(pid1, 16 * g, 64);
+          sg[g] = NG == 1 ? single : __shfl(single, group_leader<GG>(g), 64);
+          a0[g] = NG == 1 ? pid0 : __shfl(pid0, group_leader<GG>(g), 64);
+          a1[g] = NG == 1 ? pid1 : __shfl(pid1, group_leader<GG>(g), 64);
         }
         walk_push<NG>(wq, wtail, sg, a0, a1);
       }
@@ -903,7 +931,7 @@ __global__ __launch_bounds__(TB ? 128 * WALK_FW : 64) void fill_kernel(FillArgs 
   int8_t *slot = (int8_t *)lds + 1024 + threadIdx.x * stride;
   int wtail = 0;
   // Cooperative tasks first (a workgroup each), then long tasks (a wave
-  // each), then 16-lane groups take the rest.
+  // each), then G-lane groups take the rest.
   int ncoop = 0;
   if constexpr (TB) {
     if (A.walk && A.ncoop) {

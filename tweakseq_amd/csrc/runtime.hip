@@ -343,7 +343,7 @@ bool long_pair(const ta_params &p, int la, int lb) {
 }
 
 struct Plan {
-  std::vector<Task> tasks2[2], tasks1[2];  // [0] whole-wave groups, [1] 16-lane groups
+  std::vector<Task> tasks2[2], tasks1[2];  // [0] whole-wave groups, [1] G-lane groups (NGW per wave unit)
   std::vector<uint16_t> rowcodes, colcodes;
   std::vector<PairTB> ptb;
   uint64_t ring_elems = 0, tb_bytes = 0;
@@ -365,7 +365,7 @@ int fill_waves_per_cu(int P, int Kp, bool nib, int want, bool walk = false) {
 
 // Append one task (pair lo / hi; hi == lo for a single) to the plan, swept
 // by groups of GG lanes.
-// n_force / S_force (quads): columns and strips of the task's quad; extra
+// n_force / S_force (wave units): columns and strips of the task's unit; extra
 // columns lead (local couples only), extra strips are pad rows.
 void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const std::vector<HostPair> &hp,
               const std::vector<uint8_t> &codes, int lo, int hi, bool single, int Kp, int n_force = 0,
@@ -410,20 +410,21 @@ void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const
   dst.push_back(T);
 }
 
-// A couple (or single) waiting to be grouped into a quad of 16-lane tasks.
+// A couple (or single) waiting to be grouped into a wave unit of G-lane tasks.
 struct Unit {
   int lo, hi;
   bool single;
   int m, n;  // rows and columns (maxima over the two halves)
 };
 
-// Quads: four 16-lane tasks of one shape that one wave runs in lockstep.
-// Local units are ordered by (strips, columns) and take their quad's column
-// count as leading pad columns; global units need equal column counts
-// (H(m, n) is read at the last column), so they are grouped by columns.
-// A quad short of four is filled with copies of its first task that write
-// nothing.  Quads go out largest first, flattened (4 tasks per quad).
-void add_quads(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, bool local, bool trace,
+// Wave units: NGW G-lane tasks of one shape that one wave runs in lockstep.
+// Local couples are ordered by (strips, columns) and take their unit's
+// column count as leading pad columns; global couples need equal column
+// counts (H(m, n) is read at the last column), so they are grouped by
+// columns.  A unit short of NGW tasks is filled with copies of its first task
+// that write nothing.  Units go out largest first, flattened (NGW tasks per
+// unit).
+void add_units(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, bool local, bool trace,
                const std::vector<HostPair> &hp, const std::vector<uint8_t> &codes, int Kp) {
   auto strips = [](int m) { return ceil_div(std::max(m, 1), G * R); };
   std::stable_sort(u.begin(), u.end(), [&](const Unit &x, const Unit &y) {
@@ -431,45 +432,46 @@ void add_quads(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, boo
     if (strips(x.m) != strips(y.m)) return strips(x.m) > strips(y.m);
     return x.n > y.n;
   });
-  std::vector<std::vector<Task>> quads;
+  std::vector<std::vector<Task>> units;
   for (size_t k = 0; k < u.size();) {
     size_t e = k;
-    while (e < u.size() && e - k < 4 && (local || u[e].n == u[k].n)) ++e;
+    while (e < u.size() && e - k < NGW && (local || u[e].n == u[k].n)) ++e;
     int nq = 0, mq = 0;
     for (size_t i = k; i < e; ++i) { nq = std::max(nq, u[i].n); mq = std::max(mq, u[i].m); }
     std::vector<Task> q;
     for (size_t i = k; i < e; ++i)
       add_task(pl, q, P, G, trace, hp, codes, u[i].lo, u[i].hi, u[i].single, Kp, nq, strips(mq));
-    while (q.size() < 4) {
+    while (q.size() < (size_t)NGW) {
       Task f = q[0];
       f.single = 2;
       q.push_back(f);
     }
-    quads.push_back(std::move(q));
+    units.push_back(std::move(q));
     k = e;
   }
-  std::stable_sort(quads.begin(), quads.end(), [](const std::vector<Task> &x, const std::vector<Task> &y) {
+  std::stable_sort(units.begin(), units.end(), [](const std::vector<Task> &x, const std::vector<Task> &y) {
     return (double)x[0].S * x[0].NB > (double)y[0].S * y[0].NB;
   });
-  for (auto &q : quads) dst.insert(dst.end(), q.begin(), q.end());
+  for (auto &q : units) dst.insert(dst.end(), q.begin(), q.end());
 }
 
-// Steps a 16-lane group needs for a task of these shapes, in 8-step blocks.
-double g16_cost(int mmax, int n) { return (double)ceil_div(mmax, G * R) * ceil_div(n + G, 8); }
+// Steps a G-lane group needs for a task of these shapes, in 8-step blocks.
+double group_cost(int mmax, int n) { return (double)ceil_div(mmax, G * R) * ceil_div(n + G, 8); }
 
 double task_cost(const Task &T) { return (double)T.S * T.NB; }
 
-// Whole-wave groups for tasks long enough to set the makespan of a 16-lane
-// sweep: more than 0.75x the mean work of a 16-lane slot (a list-scheduling
+// Whole-wave groups for tasks long enough to set the makespan of a group
+// sweep: more than 0.75x the mean work of a group slot (a list-scheduling
 // simulation of the 65,536-pair log-normal batch gives 99% slot efficiency
 // there, 68% at 1.5x), and tall enough (>= 384 rows) not to waste most of a
 // 512-row strip.  With fewer tasks than resident waves (`few`) the run takes
-// as long as one task's sweep, so every task taller than one 16-lane strip
-// gets a whole wave (a lone 300 x 320 pair: 384 steps instead of 1,008).
+// as long as one task's sweep, so every task taller than one group strip
+// gets a whole wave (a lone 300 x 320 pair: 384 steps instead of 1,008 in
+// 16-lane strips).
 int pick_gg(int mmax, int n, double mean_slot, bool few) {
   if (mmax <= G * R) return G;
   if (few) return GWAVE;
-  return (mmax >= 3 * G * R && g16_cost(mmax, n) > 0.75 * mean_slot) ? GWAVE : G;
+  return (mmax >= 3 * GWAVE * R / 4 && group_cost(mmax, n) > 0.75 * mean_slot) ? GWAVE : G;
 }
 
 // Start of a run: work-queue counters to 0, fill outputs to their initial
@@ -764,23 +766,23 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       for (size_t t = 0; t < v.size(); t += 2) couples.push_back({v[t], t + 1 < v.size() ? v[t + 1] : -1});
     }
   }
-  // Mean 16-lane-slot work decides which tasks get whole-wave groups.
+  // Mean group-slot work decides which tasks get whole-wave groups.
   double work = 0;
   auto shape = [&](int lo, int hi) {
     return std::make_pair(std::max(B->hp[lo].la, B->hp[hi].la), std::max(B->hp[lo].lb, B->hp[hi].lb));
   };
   for (auto &c : couples) {
     auto sh = shape(c.first, c.second < 0 ? c.first : c.second);
-    work += g16_cost(sh.first, sh.second);
+    work += group_cost(sh.first, sh.second);
   }
-  for (int k : i32) work += g16_cost(B->hp[k].la, B->hp[k].lb);
+  for (int k : i32) work += group_cost(B->hp[k].la, B->hp[k].lb);
   // Walks in the fill kernel: batches with direction planes whose int16 fill
   // cannot saturate (a rerun would replace some pairs' planes after the fill).
   const char *wenv = getenv("TWEAKALIGN_WALK_IN_FILL");  // "0": walks only after the fill
   B->walk = B->trace && !B->may_saturate && !(wenv && wenv[0] == '0') &&
             walk_lds_bytes(2, B->Kp) <= ctx->lds_max;
   const int wpc = fill_waves_per_cu(2, B->Kp, !B->trace, waves_wanted(B), B->walk);
-  const double slots = (double)ctx->cus * wpc * 4;
+  const double slots = (double)ctx->cus * wpc * NGW;
   const double mean_slot = work / slots;
   const bool few = B->wave_tasks || couples.size() + i32.size() <= (size_t)ctx->cus * wpc;
   std::vector<Unit> u2, u1;
@@ -799,8 +801,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     else
       u1.push_back(Unit{k, k, true, B->hp[k].la, B->hp[k].lb});
   }
-  add_quads(pl, pl.tasks2[1], 2, u2, local, B->trace, B->hp, B->codes, B->Kp);
-  add_quads(pl, pl.tasks1[1], 1, u1, local, B->trace, B->hp, B->codes, B->Kp);
+  add_units(pl, pl.tasks2[1], 2, u2, local, B->trace, B->hp, B->codes, B->Kp);
+  add_units(pl, pl.tasks1[1], 1, u1, local, B->trace, B->hp, B->codes, B->Kp);
   auto by_cost = [](const Task &x, const Task &y) { return task_cost(x) > task_cost(y); };
   std::stable_sort(pl.tasks2[0].begin(), pl.tasks2[0].end(), by_cost);
   std::stable_sort(pl.tasks1[0].begin(), pl.tasks1[0].end(), by_cost);
@@ -925,7 +927,7 @@ int64_t batch_tb_estimate(const ta_params *p, const ta_pair *pairs, int32_t n) {
       t += (S - 1) / KC * 8 * rl + (int64_t)((lb - 1) / CKX) * S * LST * 8 +
            (int64_t)KC * GWAVE * ((std::min(lb, LREG) + GWAVE + 7) / 8) * 32;
     } else {
-      t += (int64_t)ceil_div(la, STRIP) * ceil_div(lb + G, 8) * 512;
+      t += (int64_t)ceil_div(la, STRIP) * ceil_div(lb + G, 8) * (G * R * 4);
     }
   }
   return t;
@@ -982,7 +984,7 @@ int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntas
   a.spin_limit = B->spin_limit * 16;
   const int lds = walk ? walk_lds_bytes(P, B->Kp) : fill_lds_bytes(P, B->Kp, nib);
   const int per_cu = fill_waves_per_cu(P, B->Kp, nib, waves_wanted(B), walk);
-  int groups = ntasks[0] + ntasks[1];  // one wave per whole-wave task or quad
+  int groups = ntasks[0] + ntasks[1];  // one wave per whole-wave task or wave unit
   if (max_groups > 0) groups = std::min(groups, max_groups);
   groups = std::max(1, std::min(groups, ctx->cus * per_cu));
   if (walk) {  // WALK_FW fill waves per workgroup; a cooperative task takes one
@@ -1090,7 +1092,7 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
       u.push_back(Unit{k, k, true, B->hp[k].la, B->hp[k].lb});
     }
   }
-  add_quads(pl, pl.tasks1[1], 1, u, B->p.mode == TA_LOCAL, B->trace, B->hp, B->codes, B->Kp);
+  add_units(pl, pl.tasks1[1], 1, u, B->p.mode == TA_LOCAL, B->trace, B->hp, B->codes, B->Kp);
   for (Task &t : pl.tasks1[1]) t.flags_or = TA_F_RERUN;
   std::vector<Task> all(pl.tasks1[0]);
   all.insert(all.end(), pl.tasks1[1].begin(), pl.tasks1[1].end());
@@ -1113,7 +1115,7 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   HIPCHK(hipMemcpyAsync(B->d_ptb, ptb.data(), sizeof(PairTB) * B->n, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(B->d_ctr + 4, 0, 2 * sizeof(int32_t), st));
   Task *const tk[2] = {B->d_tasks_r, B->d_tasks_r + pl.tasks1[0].size()};
-  const int nt[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / 4};
+  const int nt[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / NGW};
   return launch_fill<1>(ctx, B, tk, nt, B->d_rowcodes_r, B->d_colcodes_r, B->d_ring_r, B->d_tb_r, B->d_ctr + 4, 0);
 }
 
@@ -1131,8 +1133,8 @@ int launch_init(ta_batch *B) {
 
 int launch_fills(ta_ctx *ctx, ta_batch *B) {
   const Plan &pl = B->plan;
-  const int n2[2] = {(int)pl.tasks2[0].size(), (int)pl.tasks2[1].size() / 4};
-  const int n1[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / 4};
+  const int n2[2] = {(int)pl.tasks2[0].size(), (int)pl.tasks2[1].size() / NGW};
+  const int n1[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / NGW};
   int rc = launch_fill<2>(ctx, B, B->d_tasks2, n2, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb, B->d_ctr, 0,
                           B->walk);
   if (!rc) rc = launch_fill<1>(ctx, B, B->d_tasks1, n1, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb,
@@ -1409,13 +1411,14 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     Plan pl;
     pl.ptb.assign(B->n, PairTB{0, 0, 0, 0, 0, 0, 0});
     uint64_t max_tb = 0, max_ring = 0;
-    // Four entries per pair: the task, then three quad fillers (a 16-lane
-    // task runs as a quad; fillers write nothing).
+    // NGW entries per pair: the task, then fillers for the other groups of
+    // its wave (a group task runs in a wave of NGW groups; fillers write
+    // nothing).
     std::vector<Task> tasks;
-    auto fill_quad = [&]() {
+    auto fill_unit = [&]() {
       Task f = tasks.back();
       f.single = 2;
-      for (int q = 0; q < 3; ++q) tasks.push_back(f);
+      for (int q = 1; q < NGW; ++q) tasks.push_back(f);
     };
     for (int k = 0; k < B->n; ++k) {
       const HostPair &h = B->hp[k];
@@ -1424,7 +1427,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
       if (h.path == 0 || h.path == 3) {
         if (h.path == 3) pl.ptb[k].P = 3;  // the long-pair kernels write its result
         tasks.push_back(Task{});
-        fill_quad();
+        fill_unit();
         continue;
       }
       const int GG = pick_gg(h.la, h.lb, 0.0, true);
@@ -1436,7 +1439,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
       T.tb_off = 0; T.ring_off = 0;
       pl.ptb[k].tb_off = 0;
       pl.tb_bytes = tb0; pl.ring_elems = r0;
-      fill_quad();
+      fill_unit();
     }
     int rc = TA_OK;
     rc = rc ? rc : upload(&B->d_plain_tasks, tasks, st);
@@ -1466,8 +1469,8 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     }
     if (h.path) {
       HIPCHK(hipMemsetAsync(B->d_ctr + 6, 0, 2 * sizeof(int32_t), st));
-      const bool wave = B->plain_tasks[4 * k].S && pick_gg(h.la, h.lb, 0.0, true) == GWAVE;
-      Task *const tk[2] = {B->d_plain_tasks + 4 * k, B->d_plain_tasks + 4 * k};
+      const bool wave = B->plain_tasks[NGW * k].S && pick_gg(h.la, h.lb, 0.0, true) == GWAVE;
+      Task *const tk[2] = {B->d_plain_tasks + NGW * k, B->d_plain_tasks + NGW * k};
       const int nt[2] = {wave ? 1 : 0, wave ? 0 : 1};
       int rc = h.path == 1 ? launch_fill<1>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1)
                            : launch_fill<2>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1);

@@ -147,7 +147,7 @@ struct Walker {
     s = 0; l = 0; k = 0;  // strip, lane, row-in-lane of row i
     if (i > 0) {
       const int r0 = i - 1;
-      s = r0 >> ((GG == 64 ? 6 : 4) + 3);  // strips of GG * R rows
+      s = r0 >> (__builtin_ctz(GG) + 3);  // strips of GG * R rows (GG a power of two)
       l = (r0 >> 3) & (GG - 1);
       k = r0 & 7;
     }
