@@ -123,6 +123,7 @@ int ta_batch_graph(ta_batch *b, int32_t on);
 int ta_batch_timings(ta_batch *b, double *ms, int32_t cap);
 /* Cells (sum of la*lb) and direction-plane bytes of the batch. */
 int ta_batch_info(ta_batch *b, int64_t *cells, int64_t *tb_bytes, int32_t *tasks);
+/* Waits for the batch's last run, then frees it. */
 void ta_batch_destroy(ta_batch *b);
 
 /* Serving: continuous batching.  ta_submit copies the pairs and returns at

@@ -86,6 +86,41 @@ def test_budget_rejects_oversized_request():
         eng.close()
 
 
+def test_lowered_budget_fails_queued_request_and_serves_the_rest():
+    eng = ta.Engine(0)
+    try:
+        eng.serve_config(64 << 30, 1)
+        params = ta.engine.make_params(mode=ta.LOCAL)
+        big = [(rand_prot(random.Random(3), 3000), rand_prot(random.Random(4), 3000))]
+        small = [(rand_prot(random.Random(5), 120), rand_prot(random.Random(6), 130))]
+        # One in-flight slot, taken by a request of other parameters (so that it
+        # forms a batch of its own) submitted just before the others.
+        rng = random.Random(7)
+        first_pairs = [(rand_prot(rng, 1500), rand_prot(rng, 1500)) for _ in range(200)]
+        first, n_first = eng.submit(first_pairs, ta.engine.make_params(mode=ta.GLOBAL))
+        t_big, _ = eng.submit(big, params)
+        t_small, _ = eng.submit(small, params)
+        eng.serve_config(1 << 20, 0)  # now below the queued big request's planes
+        t0 = time.time()
+        r = None
+        while r is None:
+            r = eng.poll(t_small, 1)
+            assert time.time() - t0 < 60
+        assert r == eng.align(small, params)
+        with pytest.raises(RuntimeError, match="budget"):
+            while eng.poll(t_big, 1) is None:
+                assert time.time() - t0 < 60
+        # The first request ran, or (if the scheduler first saw it after the
+        # change) failed the same way; either way it left the queue.
+        try:
+            while eng.poll(first, n_first) is None:
+                assert time.time() - t0 < 60
+        except RuntimeError as e:
+            assert "budget" in str(e)
+    finally:
+        eng.close()
+
+
 def test_cancel():
     eng = ta.Engine(0)
     try:

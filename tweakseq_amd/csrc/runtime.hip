@@ -589,6 +589,11 @@ void free_rerun(ta_batch *B) {
 
 void ta_batch_destroy(ta_batch *B) {
   if (!B) return;
+  // The blocks go back to the cache, where a batch on another stream may take
+  // them at once: the batch's last run (every run form ends by recording
+  // ev[3]) must have finished first.  The event is the batch's own, so this
+  // holds even where the run's stream is gone.
+  if (B->ev[3]) (void)hipEventSynchronize(B->ev[3]);
   dfree(B->d_codes); dfree(B->d_table); dfree(B->d_table_nib);
   for (int g = 0; g < 2; ++g) { dfree(B->d_tasks2[g]); dfree(B->d_tasks1[g]); }
   dfree(B->d_rowcodes); dfree(B->d_colcodes); dfree(B->d_ring); dfree(B->d_tb);

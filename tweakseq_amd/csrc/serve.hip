@@ -132,8 +132,15 @@ void Serve::worker() {
       run_group(g, st);
     } else {
       std::lock_guard<std::mutex> g2(mu);
-      for (Request *r : g.reqs)
-        if (r->state == RState::InFlight) { r->state = RState::Failed; r->err = TA_E_DEVICE; }
+      for (Request *r : g.reqs) {
+        if (r->state == RState::InFlight) {
+          r->state = RState::Failed;
+          r->err = TA_E_DEVICE;
+        } else if (r->state == RState::Cancelled) {  // cancelled while in flight: retire now
+          all.erase(r->ticket);
+          delete r;
+        }
+      }
     }
     lk.lock();
     --inflight;
@@ -157,6 +164,12 @@ void Serve::loop() {
       for (auto it = pending.begin(); it != pending.end();) {
         Request *r = *it;
         if (r->state == RState::Cancelled) { it = pending.erase(it); continue; }
+        if (r->tb_bytes > budget) {  // the budget was lowered below it: it can never run
+          r->state = RState::Failed;
+          r->err = TA_E_BUDGET;
+          it = pending.erase(it);
+          continue;
+        }
         if (!g.reqs.empty() && !same_params(r->p, g.reqs[0]->p)) { ++it; continue; }
         if (used + g.tb + r->tb_bytes > budget) break;
         g.tb += r->tb_bytes;
