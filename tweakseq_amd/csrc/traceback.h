@@ -12,9 +12,10 @@
 // steps][lane]).  While the walk stays in one lane (8 rows) it needs at most
 // two consecutive blocks of that lane, so each lane visit is one two-block
 // window (two loads), kept in a padded per-thread LDS slice (dynamic indexing
-// stays out of scratch).  Residue codes come
-// through 16-byte register windows read backwards.  One iteration emits one
-// CIGAR op without branching on the state; runs leave as binary dwords.  The
+// stays out of scratch).  Residue codes come through 8-byte register
+// windows (the current residue in the top byte, shifted a byte per move).
+// One iteration emits one CIGAR op and moves with selects, without
+// branching on the state; runs leave as binary dwords.  The
 // walk is latency-bound, so a wave carries few walkers (more waves per SIMD
 // overlap their loads); walkers run longest-path first.
 #pragma once
@@ -63,13 +64,21 @@ struct RunWriter {
   __device__ __forceinline__ void finish() { store(); len = 0; }
 };
 
-// Byte o (0..31) of a 32-byte window held as two uint4.
-__device__ __forceinline__ int win32_byte(uint4 lo, uint4 hi, int o) {
-  const bool up = (o & 16) != 0, mid = (o & 8) != 0;
-  const uint32_t w0 = mid ? (up ? hi.z : lo.z) : (up ? hi.x : lo.x);
-  const uint32_t w1 = mid ? (up ? hi.w : lo.w) : (up ? hi.y : lo.y);
-  const uint64_t v = ((uint64_t)w1 << 32) | w0;
-  return (int)((v >> ((o & 7) * 8)) & 0xff);
+// The 8 residue codes of a sequence at positions i-8 .. i-1, position i-1 in
+// the top byte (positions below 0 read as 0), from one dword-aligned 12-byte
+// load.  A walk steps up one row by shifting the window left a byte, so the
+// current residue is always the top byte.
+__device__ __forceinline__ uint64_t tail8(const uint8_t *seq, int i) {
+  if (i <= 0) return 0;
+  const int lo = i - 8;
+  const int base = (lo < 0 ? 0 : lo) & ~3;
+  const uint32_t *w = (const uint32_t *)(seq + base);
+  const uint32_t d0 = w[0], d1 = w[1], d2 = w[2];
+  if (lo >= 0) {
+    const uint32_t sh = (uint32_t)(lo - base);  // 0..3
+    return ((uint64_t)__builtin_amdgcn_alignbyte(d2, d1, sh) << 32) | __builtin_amdgcn_alignbyte(d1, d0, sh);
+  }
+  return (((uint64_t)d1 << 32) | d0) << (8 * (8 - i));  // positions 0..7, i in 1..7
 }
 
 constexpr int WSTRIDE = 68;     // dwords per walker LDS slice: two 32-dword windows + pad (16-byte aligned)
@@ -220,18 +229,16 @@ struct Walker {
     if (local) h -= g == 0 ? diag() : (xb ? ext : open);
     cw.emit(g, 1);
     state = xb ? g : 0;
-    if (g != 2) --j;
-    if (g != 1) {
-      --i;
-      if (k > 0) --k;
-      else {
-        k = 7;
-        if (l > 0) --l;
-        else { l = GG - 1; --s; }
-      }
-      if (local && g == 0 && (h == 0 || i == 0 || j == 0)) return true;
-    }
-    return false;
+    // the move, in selects: a row move steps k down, borrowing from the lane
+    // (l), which borrows from the strip (s); GG is a power of two
+    const bool up = g != 1;
+    j -= g != 2;
+    i -= up;
+    const bool bk = up && k == 0, bl = bk && l == 0;
+    k = up ? ((k - 1) & 7) : k;
+    l = bk ? ((l - 1) & (GG - 1)) : l;
+    s -= bl;
+    return local && g == 0 && (h == 0 || i == 0 || j == 0);
   }
 
   // The walk's head: borders that end it.  True when it is over (result
@@ -260,35 +267,33 @@ struct Walker {
     if (lB < 0) { lB = GG - 1; sB = s - 1; tB += GG; }
     const bool hasB = sB >= 0;
     const int fB = hasB ? first_block(tB) : 0;
-    const int laneA = s * GG + l, laneB = sB * GG + lB;
+    const int sA = s, lA = l;
     copy_window(slice, block_at(s, l, fA), block_at(s, l, fA + 1));
     if (hasB) copy_window(slice + 32, block_at(sB, lB, fB), block_at(sB, lB, fB + 1));
-    uint4 ra0 = {0, 0, 0, 0}, ra1 = ra0, rb0 = ra0, rb1 = ra0;
-    const int abase = i - 1 >= 0 ? ((i - 1 - EPOCH) < 0 ? 0 : (i - 1 - EPOCH)) & ~15 : 0;
-    const int bbase = j - 1 >= 0 ? ((j - 1 - EPOCH) < 0 ? 0 : (j - 1 - EPOCH)) & ~15 : 0;
+    uint64_t ra = 0, rb = 0;  // local: residues of the rows / columns ahead, current one on top
     if (local) {
-      ra0 = *(const uint4 *)(pa + abase);
-      ra1 = *(const uint4 *)(pa + abase + 16);
-      rb0 = *(const uint4 *)(pb + bbase);
-      rb1 = *(const uint4 *)(pb + bbase + 16);
+      ra = tail8(pa, i);
+      rb = tail8(pb, j);
     }
-    // ---- up to EPOCH steps inside the two windows
+    // ---- up to EPOCH steps inside the two windows.  In EPOCH steps a walk
+    // climbs at most one lane boundary, so it is in the first window's lane
+    // or in the lane above (the second window).
     int e = 0;
+#pragma unroll 1
     for (; e < EPOCH && guard > 0; ++e, --guard) {
       if (!local && (i == 0 || j == 0)) break;  // boundary handled by the epoch head
       const int t = j - 1 + off + l;
-      const int lane_now = s * GG + l;
-      int wsel, wi;
-      if (lane_now == laneA && t >= 8 * fA && t < 8 * fA + 16) { wsel = 0; wi = t - 8 * fA; }
-      else if (hasB && lane_now == laneB && t >= 8 * fB && t < 8 * fB + 16) { wsel = 32; wi = t - 8 * fB; }
-      else break;  // left both windows: next epoch
-      const int code = code_at(slice + wsel + (wi >> 3) * (p2 ? 16 : 8) + (wi & 7) * (p2 ? 2 : 1));
-      if (take(code, local, open, ext, [&] {
-            return (int)stab[win32_byte(ra0, ra1, i - 1 - abase) * 32 + win32_byte(rb0, rb1, j - 1 - bbase)];
-          })) {
+      const bool inA = s == sA && l == lA;
+      const int wi = t - 8 * (inA ? fA : fB);
+      if (!(inA || hasB) || wi < 0 || wi >= 16) break;  // left both windows: next epoch
+      const int code = code_at(slice + (inA ? 0 : 32) + (wi >> 3) * (p2 ? 16 : 8) + (wi & 7) * (p2 ? 2 : 1));
+      const int i0 = i, j0 = j;
+      if (take(code, local, open, ext, [&] { return (int)stab[((int)(ra >> 56) << 5) | (int)(rb >> 56)]; })) {
         done = true;
         break;
       }
+      ra = i != i0 ? ra << 8 : ra;
+      rb = j != j0 ? rb << 8 : rb;
     }
     // An epoch starts inside its own window, so it always takes a step; one
     // that could not (a start cell outside the planes) ends the walk,
