@@ -84,6 +84,17 @@ int or_rescore(const or_params *p, const int8_t *smat, int32_t K,
  * out[0..len).  Returns 0, -1 on bad arguments, -5 on allocation failure. */
 int or_consensus(const int8_t *smat23, const int32_t *map26, const char *msa, int32_t nseq, int32_t len,
                  double plurality, char *out);
+/* Progressive multiple alignment (msa.c states the specification): global
+ * pair scores, UPGMA on normalised similarities, profile joins by global
+ * Gotoh over columns.  Sequence s is codes[off[s] .. off[s] + len[s]).
+ * Writes the alignment length to *alen, the column of every residue to
+ * col[off[s] + k], and (joins != NULL) the two clusters of join t to
+ * joins[2t], joins[2t + 1] (leaves 0..nseq-1, join t creates nseq + t).
+ * Returns 0, -1 on bad arguments, -2 for a code >= K, -5 on allocation
+ * failure. */
+int or_msa(const int8_t *smat, int32_t K, int32_t gap_open, int32_t gap_extend, const uint8_t *codes,
+           const int64_t *off, const int32_t *len, int32_t nseq, int32_t *joins, int32_t *alen,
+           int32_t *col);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,9 @@ def lib():
                                  ctypes.c_char_p, P(ctypes.c_int32)]
         L.or_consensus.argtypes = [P(ctypes.c_int8), P(ctypes.c_int32), ctypes.c_char_p, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_double, ctypes.c_char_p]
+        L.or_msa.argtypes = [P(ctypes.c_int8), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                             P(ctypes.c_uint8), P(ctypes.c_int64), P(ctypes.c_int32), ctypes.c_int32,
+                             P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32)]
         _LIB = L
     return _LIB
 
@@ -179,3 +182,42 @@ def consensus(rows, plurality=-1.0):
     if rc:
         raise RuntimeError("or_consensus: %d" % rc)
     return out.raw[:L]
+
+
+def msa(seqs, smat=None, gap_open=-10, gap_extend=-1):
+    """Progressive MSA (or_msa) of code arrays; returns (alen, cols, joins):
+    cols[s] = column of each residue of sequence s."""
+    if smat is None:
+        smat, _ = protein_matrix()
+    smat = np.ascontiguousarray(smat, dtype=np.int8)
+    K = smat.shape[0]
+    seqs = [np.ascontiguousarray(x, dtype=np.uint8) for x in seqs]
+    n = len(seqs)
+    lens = np.array([len(x) for x in seqs], dtype=np.int32)
+    off = np.zeros(n, dtype=np.int64)
+    if n > 1:
+        off[1:] = np.cumsum(lens[:-1])
+    codes = np.concatenate(seqs + [np.zeros(1, dtype=np.uint8)]).astype(np.uint8)
+    col = np.zeros(max(int(lens.sum()), 1), dtype=np.int32)
+    joins = np.zeros(max(2 * (n - 1), 1), dtype=np.int32)
+    alen = ctypes.c_int32()
+    P = ctypes.POINTER
+    rc = lib().or_msa(smat.ctypes.data_as(P(ctypes.c_int8)), K, gap_open, gap_extend,
+                      codes.ctypes.data_as(P(ctypes.c_uint8)), off.ctypes.data_as(P(ctypes.c_int64)),
+                      lens.ctypes.data_as(P(ctypes.c_int32)), n, joins.ctypes.data_as(P(ctypes.c_int32)),
+                      ctypes.byref(alen), col.ctypes.data_as(P(ctypes.c_int32)))
+    if rc:
+        raise RuntimeError("or_msa: %d" % rc)
+    cols = [col[off[s]:off[s] + lens[s]].copy() for s in range(n)]
+    return alen.value, cols, [tuple(joins[2 * t:2 * t + 2]) for t in range(n - 1)]
+
+
+def msa_rows(seqs_text, alen, cols):
+    """Aligned rows ('-' for gaps) from residue columns."""
+    rows = []
+    for s, c in zip(seqs_text, cols):
+        r = ["-"] * alen
+        for k, q in enumerate(c):
+            r[q] = s[k]
+        rows.append("".join(r))
+    return rows
