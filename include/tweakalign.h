@@ -182,6 +182,21 @@ int ta_encode(int32_t alphabet, const char *s, int32_t len, uint8_t *out);
  * reference's default with unit weights, Consensus.cpp:163-173). */
 int ta_consensus(ta_ctx *ctx, const char *const *rows, int32_t nseq, int32_t len, double plurality, char *out);
 
+/* Progressive multiple alignment of nseq sequences (seqs[s][0..lens[s]),
+ * lens[s] >= 1, letters as ta_encode takes them; p->mode TA_GLOBAL, gaps and
+ * alphabet as for pairs): global pair scores on the GPU, UPGMA on normalised
+ * scores, then the profile joins level by level on the GPU.  Replaces the
+ * external MSA run behind "Align all" (tweakseq/UI/SeqEditMainWin.cpp
+ * :1647-1659, result read by Project::readNewAlignment,
+ * tweakseq/Core/Project.cpp:908-1032).  Writes the alignment length to
+ * *alen and the column of residue k of sequence s to cols[coff_s + k]
+ * (coff_s = lens[0] + ... + lens[s-1]); joins (may be NULL) gets the two
+ * clusters of each of the nseq - 1 joins (leaves 0..nseq-1, join t makes
+ * cluster nseq + t).  Profiles taking the DP rows (the lower cluster id of a
+ * join) are limited to 2,048 columns (TA_E_INVALID beyond). */
+int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32_t *lens, int32_t nseq,
+           int32_t *alen, int32_t *cols, int32_t *joins);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,9 +17,8 @@
 // are concatenated, output is wrapped at 80 columns.  Gaps ('-', '.') left by
 // an earlier alignment are removed first (Sequence::filter keeps them,
 // Sequence.cpp:57-69).  Two sequences: Gotoh global alignment on the GPU.
-// More: center-star multiple alignment -- all-vs-all global scores on the GPU
-// pick the sequence with the highest score sum, every other sequence is
-// aligned to it, and the pairwise alignments are merged on the center.
+// More: progressive multiple alignment (ta_msa) -- all-vs-all global scores
+// on the GPU, a UPGMA guide tree, and profile joins on the GPU.
 #include <algorithm>
 #include <cctype>
 #include <cstdio>
@@ -197,71 +196,36 @@ int main(int argc, char **argv) {
     int rc = ta_create(&al.ctx, 0);
     if (rc) return fail(std::string("GPU: ") + ta_strerror(rc));
     const size_t N = recs.size();
-    size_t center = 0;
-    std::vector<ta_result> res;
-    std::vector<char> cig;
-    if (N > 2) {
-      // All-vs-all global scores (score only) choose the center.
-      std::vector<std::pair<const std::string *, const std::string *>> pairs;
-      for (size_t x = 0; x < N; ++x)
-        for (size_t y = x + 1; y < N; ++y) pairs.push_back({&recs[x].seq, &recs[y].seq});
-      al.p.traceback = 0;
+    if (N == 2) {
+      std::vector<std::pair<const std::string *, const std::string *>> pairs{{&recs[0].seq, &recs[1].seq}};
+      std::vector<ta_result> res;
+      std::vector<char> cig;
+      al.p.traceback = 1;
       rc = al.run(pairs, res, cig);
       if (rc) return fail(std::string("alignment failed: ") + ta_strerror(rc));
-      std::vector<long long> sum(N, 0);
-      size_t q = 0;
-      for (size_t x = 0; x < N; ++x)
-        for (size_t y = x + 1; y < N; ++y, ++q) { sum[x] += res[q].score; sum[y] += res[q].score; }
-      center = (size_t)(std::max_element(sum.begin(), sum.end()) - sum.begin());
-    }
-    // Every other sequence against the center (center = query rows).
-    std::vector<std::pair<const std::string *, const std::string *>> pairs;
-    std::vector<size_t> who;
-    for (size_t x = 0; x < N; ++x)
-      if (x != center) { pairs.push_back({&recs[center].seq, &recs[x].seq}); who.push_back(x); }
-    al.p.traceback = 1;
-    rc = al.run(pairs, res, cig);
-    if (rc) return fail(std::string("alignment failed: ") + ta_strerror(rc));
-    const std::string &c = recs[center].seq;
-    std::vector<std::string> ops(pairs.size());
-    for (size_t q = 0; q < pairs.size(); ++q) ops[q] = expand(cig.data() + res[q].cigar_off, res[q].cigar_len);
-    if (N == 2) {
       std::string ra, rb;
-      render(c, recs[who[0]].seq, ops[0], ra, rb);
-      outrecs[center].seq = ra;
-      outrecs[who[0]].seq = rb;
+      render(recs[0].seq, recs[1].seq, expand(cig.data() + res[0].cigar_off, res[0].cigar_len), ra, rb);
+      outrecs[0].seq = ra;
+      outrecs[1].seq = rb;
     } else {
-      // Insertion slots between center residues: the widest 'D' run there.
-      std::vector<size_t> slot(c.size() + 1, 0);
-      for (const std::string &o : ops) {
-        size_t i = 0, run = 0;
-        for (char x : o) {
-          if (x == 'D') { ++run; continue; }
-          slot[i] = std::max(slot[i], run);
-          run = 0;
-          ++i;  // M and I both consume a center residue
-        }
-        slot[i] = std::max(slot[i], run);
+      std::vector<const char *> seqs(N);
+      std::vector<int32_t> lens(N);
+      size_t total = 0;
+      for (size_t x = 0; x < N; ++x) {
+        seqs[x] = recs[x].seq.data();
+        lens[x] = (int32_t)recs[x].seq.size();
+        total += recs[x].seq.size();
       }
-      std::string cr;
-      for (size_t i = 0; i <= c.size(); ++i) {
-        cr.append(slot[i], '-');
-        if (i < c.size()) cr += c[i];
-      }
-      outrecs[center].seq = cr;
-      for (size_t q = 0; q < ops.size(); ++q) {
-        const std::string &s = recs[who[q]].seq;
-        std::string row;
-        size_t i = 0, j = 0, run = 0;
-        auto close = [&](size_t at) { row.append(slot[at] - run, '-'); run = 0; };
-        for (char x : ops[q]) {
-          if (x == 'D') { row += s[j++]; ++run; continue; }
-          close(i);
-          row += x == 'M' ? s[j++] : '-';
-          ++i;
-        }
-        close(i);
-        outrecs[who[q]].seq = row;
+      std::vector<int32_t> cols(std::max<size_t>(total, 1));
+      int32_t alen = 0;
+      al.p.traceback = 0;
+      rc = ta_msa(al.ctx, &al.p, seqs.data(), lens.data(), (int32_t)N, &alen, cols.data(), nullptr);
+      if (rc) return fail(std::string("alignment failed: ") + ta_strerror(rc));
+      size_t k = 0;
+      for (size_t x = 0; x < N; ++x) {
+        std::string row((size_t)alen, '-');
+        for (char ch : recs[x].seq) row[(size_t)cols[k++]] = ch;
+        outrecs[x].seq = row;
       }
     }
   }

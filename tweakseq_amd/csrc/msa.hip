@@ -1,0 +1,342 @@
+// Progressive multiple alignment on the GPU (ta_msa).
+//
+// tweakseq aligns N > 2 sequences by handing them to an external MSA program
+// (tweakseq/UI/SeqEditMainWin.cpp:1647-1659) and reading the aligned FASTA
+// back (tweakseq/Core/Project.cpp:908-1032).  This builds the alignment in
+// process, to the specification the CPU oracle restates (oracle/msa.c):
+//
+//  1. global Gotoh scores of all pairs -- one batch of the pairwise engine
+//     (n(n-1)/2 pairs, score only: 130,816 pairs for 512 sequences);
+//  2. similarities s_ij * 1e6 / max(1, min(self_i, self_j)) (integer), and
+//     UPGMA joins on the host with exact integer comparisons (ties to the
+//     smallest pair of cluster ids);
+//  3. the joins, tree level by tree level: every join of a level is one
+//     workgroup of msa_join_kernel, which aligns the columns of the lower
+//     cluster's profile (DP rows) with the higher one's (DP columns) by
+//     global Gotoh -- match score M(i,j) = cntA[i] . PB[j] with
+//     PB[j][x] = sum_y S(x,y) cntB[j][y], gap costs times nA * nB, the
+//     pairwise ties (diag > E > F, E / F prefer opening) -- then walks the
+//     direction bytes back from (la, lb) to ops; the host applies the ops to
+//     the profiles' rows.
+//
+// The join DP sweeps anti-diagonals: thread t owns rows t+1, t+1+256, ...
+// (up to 8 each, so profiles of up to 2,048 columns take the DP rows), keeps
+// H and E of its rows' last cells in registers, and reads the row above
+// (H, F at the previous diagonal, H at the one before) from a three-slot LDS
+// ring of int64 values, one barrier per diagonal.  Values are int64: a join
+// of two 256-sequence profiles scores up to 11 * 65,536 per column pair.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "runtime_internal.h"
+
+namespace {
+
+constexpr int JT = 256;                  // threads per join
+constexpr int JR = 8;                    // DP rows per thread
+constexpr int JMAX = JT * JR;            // longest row profile
+constexpr int64_t JNEG = INT64_MIN / 4;  // -inf
+
+struct JoinJob {
+  int32_t la, lb;
+  int64_t w;        // nA * nB: gap costs scale
+  int64_t cnt_off;  // la x K: residues of each code in A's columns
+  int64_t pb_off;   // lb x K: PB[j][x] = sum_y S(x, y) cntB[j][y]
+  int64_t tb_off;   // (la + 1) x (lb + 1) direction bytes: src | e << 2 | f << 3
+  int64_t ops_off;  // la + lb ops, walk order (last column first)
+};
+
+struct JoinArgs {
+  const JoinJob *jobs;
+  const int32_t *cnt, *pb;
+  uint8_t *tb;
+  uint8_t *ops;
+  int32_t *nops;
+  int32_t K, open, ext;
+};
+
+__global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
+  extern __shared__ int64_t ring[];  // H[3][la + 1], then F[3][la + 1]
+  const JoinJob J = A.jobs[blockIdx.x];
+  const int la = J.la, lb = J.lb, K = A.K, tid = threadIdx.x;
+  const int64_t go = (int64_t)A.open * J.w, ge = (int64_t)A.ext * J.w;
+  const int rs = la + 1;
+  int64_t *Hr = ring, *Fr = ring + 3 * rs;
+  const int32_t *cnt = A.cnt + J.cnt_off, *pb = A.pb + J.pb_off;
+  uint8_t *tb = A.tb + J.tb_off;
+  const int64_t stride = lb + 1;
+  int64_t Hl[JR], El[JR];  // H(i, j-1), E(i, j-1) of each owned row
+#pragma unroll
+  for (int r = 0; r < JR; ++r) {
+    const int i = tid + 1 + r * JT;
+    Hl[r] = go + (int64_t)(i - 1) * ge;  // H(i, 0)
+    El[r] = JNEG;
+  }
+  for (int d = 2; d <= la + lb; ++d) {
+    const int cur = d % 3, prv = (d + 2) % 3, pp = (d + 1) % 3;  // diagonals d, d-1, d-2
+#pragma unroll
+    for (int r = 0; r < JR; ++r) {
+      const int i = tid + 1 + r * JT;
+      const int j = d - i;
+      if (i > la || j < 1 || j > lb) continue;
+      int64_t Hup, Fup, Hdg;
+      if (i == 1) {  // row 0 is the boundary
+        Hup = go + (int64_t)(j - 1) * ge;
+        Fup = JNEG;
+        Hdg = j == 1 ? 0 : go + (int64_t)(j - 2) * ge;
+      } else {
+        Hup = Hr[prv * rs + i - 1];
+        Fup = Fr[prv * rs + i - 1];
+        Hdg = j == 1 ? go + (int64_t)(i - 2) * ge : Hr[pp * rs + i - 1];
+      }
+      const int32_t *ca = cnt + (int64_t)(i - 1) * K, *pj = pb + (int64_t)(j - 1) * K;
+      int64_t m = 0;
+      for (int x = 0; x < K; ++x) m += (int64_t)ca[x] * pj[x];
+      const int64_t eo = Hl[r] + go, ee = El[r] + ge;
+      const bool eb = ee > eo;
+      const int64_t E = eb ? ee : eo;
+      const int64_t fo = Hup + go, fe = Fup + ge;
+      const bool fb = fe > fo;
+      const int64_t F = fb ? fe : fo;
+      int64_t h = Hdg + m;
+      int src = 0;
+      if (E > h) { h = E; src = 1; }
+      if (F > h) { h = F; src = 2; }
+      Hr[cur * rs + i] = h;
+      Fr[cur * rs + i] = F;
+      Hl[r] = h;
+      El[r] = E;
+      tb[(int64_t)i * stride + j] = (uint8_t)(src | ((int)eb << 2) | ((int)fb << 3));
+    }
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  // Walk back from (la, lb): the pairwise state machine (H, E 'D', F 'I').
+  uint8_t *ops = A.ops + J.ops_off;
+  int i = la, j = lb, k = 0, state = 0;
+  for (;;) {
+    if (i == 0) { while (j-- > 0) ops[k++] = 'D'; break; }
+    if (j == 0) { while (i-- > 0) ops[k++] = 'I'; break; }
+    const int code = tb[(int64_t)i * stride + j];
+    if (state == 0) {
+      const int s = code & 3;
+      if (s == 0) { ops[k++] = 'M'; --i; --j; }
+      else state = s;
+    } else if (state == 1) {
+      ops[k++] = 'D';
+      if (!(code & 4)) state = 0;
+      --j;
+    } else {
+      ops[k++] = 'I';
+      if (!(code & 8)) state = 0;
+      --i;
+    }
+  }
+  A.nops[blockIdx.x] = k;
+}
+
+struct Profile {
+  std::vector<int32_t> rows;  // sequence ids
+  int32_t len = 0;
+  std::vector<int16_t> cell;  // rows x len: code, or -1 for a gap
+};
+
+}  // namespace
+
+int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32_t *lens, int32_t nseq,
+           int32_t *alen, int32_t *cols, int32_t *joins) {
+  if (!ctx || !p || !seqs || !lens || nseq <= 0 || !alen || !cols) return TA_E_INVALID;
+  if (p->mode != TA_GLOBAL || p->gap_open > 0 || p->gap_extend > 0) return TA_E_INVALID;
+  std::vector<int64_t> off(nseq + 1, 0);
+  for (int32_t s = 0; s < nseq; ++s) {
+    if (lens[s] <= 0 || !seqs[s]) return TA_E_INVALID;
+    off[s + 1] = off[s] + lens[s];
+  }
+  int8_t S[32 * 32];
+  int32_t K = 0;
+  int rc = ta_matrix(p->alphabet, p->match, p->mismatch, S, &K);
+  if (rc) return rc;
+  std::vector<uint8_t> codes((size_t)off[nseq]);
+  for (int32_t s = 0; s < nseq; ++s)
+    if ((rc = ta_encode(p->alphabet, seqs[s], lens[s], codes.data() + off[s]))) return rc;
+  if (hipSetDevice(ctx->device) != hipSuccess) return TA_E_DEVICE;
+
+  // 1. pair scores on the GPU
+  const int32_t nc = 2 * nseq - 1;
+  std::vector<int64_t> sum((size_t)nc * nc, 0);
+  std::vector<int64_t> self(nseq, 0);
+  for (int32_t s = 0; s < nseq; ++s)
+    for (int64_t q = off[s]; q < off[s + 1]; ++q) self[s] += S[codes[q] * K + codes[q]];
+  if (nseq > 1) {
+    std::vector<ta_pair> pairs;
+    pairs.reserve((size_t)nseq * (nseq - 1) / 2);
+    for (int32_t a = 0; a < nseq; ++a)
+      for (int32_t b = a + 1; b < nseq; ++b) pairs.push_back({seqs[a], lens[a], seqs[b], lens[b]});
+    ta_params q = *p;
+    q.traceback = 0;
+    ta_batch *B = nullptr;
+    std::vector<ta_result> res(pairs.size());
+    rc = batch_create_on(ctx, &q, pairs.data(), (int32_t)pairs.size(), ctx->stream, &B);
+    if (!rc) rc = ta_batch_run(ctx, B);
+    if (!rc) rc = ta_batch_wait(ctx, B);
+    if (!rc) rc = ta_batch_results(ctx, B, res.data(), nullptr, 0, nullptr);
+    if (B) ta_batch_destroy(B);
+    if (rc) return rc;
+    size_t k = 0;
+    for (int32_t a = 0; a < nseq; ++a)
+      for (int32_t b = a + 1; b < nseq; ++b, ++k) {
+        const int64_t d = std::max<int64_t>(1, std::min(self[a], self[b]));
+        sum[(size_t)a * nc + b] = sum[(size_t)b * nc + a] = (int64_t)res[k].score * 1000000 / d;
+      }
+  }
+
+  // 2. UPGMA joins (exact), and each join's tree level
+  std::vector<int32_t> size(nc, 0), level(nc, 0), ja(nseq > 1 ? nseq - 1 : 0), jb(ja.size());
+  std::vector<char> live(nc, 0);
+  for (int32_t s = 0; s < nseq; ++s) { size[s] = 1; live[s] = 1; }
+  int32_t top = 0;
+  for (int32_t t = 0; t + 1 < nseq; ++t) {
+    int32_t ba = -1, bb = -1;
+    for (int32_t a = 0; a < nseq + t; ++a) {
+      if (!live[a]) continue;
+      for (int32_t b = a + 1; b < nseq + t; ++b) {
+        if (!live[b]) continue;
+        if (ba < 0) { ba = a; bb = b; continue; }
+        const __int128 lhs = (__int128)sum[(size_t)a * nc + b] * size[ba] * size[bb];
+        const __int128 rhs = (__int128)sum[(size_t)ba * nc + bb] * size[a] * size[b];
+        if (lhs > rhs) { ba = a; bb = b; }
+      }
+    }
+    const int32_t c = nseq + t;
+    ja[t] = ba; jb[t] = bb;
+    if (joins) { joins[2 * t] = ba; joins[2 * t + 1] = bb; }
+    size[c] = size[ba] + size[bb];
+    level[c] = 1 + std::max(level[ba], level[bb]);
+    top = std::max(top, level[c]);
+    live[ba] = live[bb] = 0;
+    live[c] = 1;
+    for (int32_t x = 0; x < c; ++x)
+      if (live[x]) sum[(size_t)c * nc + x] = sum[(size_t)x * nc + c] = sum[(size_t)ba * nc + x] + sum[(size_t)bb * nc + x];
+  }
+
+  // 3. joins level by level
+  std::vector<Profile> prof(nc);
+  for (int32_t s = 0; s < nseq; ++s) {
+    prof[s].rows = {s};
+    prof[s].len = lens[s];
+    prof[s].cell.assign(codes.begin() + off[s], codes.begin() + off[s + 1]);
+  }
+  hipStream_t st = ctx->stream;
+  for (int32_t L = 1; L <= top; ++L) {
+    std::vector<int32_t> ts;
+    for (int32_t t = 0; t + 1 < nseq; ++t)
+      if (level[nseq + t] == L) ts.push_back(t);
+    std::vector<JoinJob> jobs(ts.size());
+    std::vector<int32_t> hcnt, hpb;
+    int64_t tb_total = 0, ops_total = 0;
+    int max_la = 0;
+    for (size_t q = 0; q < ts.size(); ++q) {
+      const Profile &PA = prof[ja[ts[q]]], &PB = prof[jb[ts[q]]];
+      if (PA.len > JMAX) return TA_E_INVALID;  // longer row profiles are not supported
+      JoinJob &J = jobs[q];
+      J.la = PA.len; J.lb = PB.len;
+      J.w = (int64_t)PA.rows.size() * (int64_t)PB.rows.size();
+      J.cnt_off = (int64_t)hcnt.size();
+      hcnt.resize(hcnt.size() + (size_t)PA.len * K, 0);
+      int32_t *ca = hcnt.data() + J.cnt_off;
+      for (size_t r = 0; r < PA.rows.size(); ++r)
+        for (int32_t i = 0; i < PA.len; ++i) {
+          const int c = PA.cell[r * PA.len + i];
+          if (c >= 0) ca[(size_t)i * K + c] += 1;
+        }
+      J.pb_off = (int64_t)hpb.size();
+      hpb.resize(hpb.size() + (size_t)PB.len * K, 0);
+      int32_t *pbj = hpb.data() + J.pb_off;
+      for (size_t r = 0; r < PB.rows.size(); ++r)
+        for (int32_t j = 0; j < PB.len; ++j) {
+          const int c = PB.cell[r * PB.len + j];
+          if (c < 0) continue;
+          for (int32_t x = 0; x < K; ++x) pbj[(size_t)j * K + x] += S[x * K + c];
+        }
+      J.tb_off = tb_total;
+      tb_total += (int64_t)(PA.len + 1) * (PB.len + 1);
+      J.ops_off = ops_total;
+      ops_total += PA.len + PB.len;
+      max_la = std::max(max_la, PA.len);
+    }
+    const size_t lds = sizeof(int64_t) * 6 * (size_t)(max_la + 1);
+    JoinJob *d_jobs = (JoinJob *)device_alloc(sizeof(JoinJob) * jobs.size());
+    int32_t *d_cnt = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hcnt.size()));
+    int32_t *d_pb = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hpb.size()));
+    uint8_t *d_tb = (uint8_t *)device_alloc((size_t)std::max<int64_t>(1, tb_total));
+    uint8_t *d_ops = (uint8_t *)device_alloc((size_t)std::max<int64_t>(1, ops_total));
+    int32_t *d_nops = (int32_t *)device_alloc(sizeof(int32_t) * jobs.size());
+    std::vector<uint8_t> hops((size_t)std::max<int64_t>(1, ops_total));
+    std::vector<int32_t> hnops(jobs.size());
+    rc = (d_jobs && d_cnt && d_pb && d_tb && d_ops && d_nops) ? TA_OK : TA_E_NOMEM;
+    if (!rc && (hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JoinJob) * jobs.size(), hipMemcpyHostToDevice, st) ||
+                (!hcnt.empty() &&
+                 hipMemcpyAsync(d_cnt, hcnt.data(), sizeof(int32_t) * hcnt.size(), hipMemcpyHostToDevice, st)) ||
+                (!hpb.empty() &&
+                 hipMemcpyAsync(d_pb, hpb.data(), sizeof(int32_t) * hpb.size(), hipMemcpyHostToDevice, st))))
+      rc = TA_E_DEVICE;
+    if (!rc) {
+      JoinArgs a;
+      a.jobs = d_jobs; a.cnt = d_cnt; a.pb = d_pb; a.tb = d_tb; a.ops = d_ops; a.nops = d_nops;
+      a.K = K; a.open = p->gap_open; a.ext = p->gap_extend;
+      (void)hipGetLastError();
+      hipLaunchKernelGGL(msa_join_kernel, dim3((unsigned)jobs.size()), dim3(JT), lds, st, a);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(hops.data(), d_ops, (size_t)ops_total, hipMemcpyDeviceToHost, st) ||
+          hipMemcpyAsync(hnops.data(), d_nops, sizeof(int32_t) * jobs.size(), hipMemcpyDeviceToHost, st) ||
+          hipStreamSynchronize(st))
+        rc = TA_E_DEVICE;
+    }
+    device_free(d_jobs); device_free(d_cnt); device_free(d_pb);
+    device_free(d_tb); device_free(d_ops); device_free(d_nops);
+    if (rc) return rc;
+    // apply each join's ops (walk order: read backwards)
+    for (size_t q = 0; q < ts.size(); ++q) {
+      const int32_t t = ts[q], c = nseq + t;
+      Profile &PA = prof[ja[t]], &PB = prof[jb[t]];
+      const int32_t n = hnops[q];
+      const uint8_t *o = hops.data() + jobs[q].ops_off;
+      Profile C;
+      C.rows = PA.rows;
+      C.rows.insert(C.rows.end(), PB.rows.begin(), PB.rows.end());
+      C.len = n;
+      C.cell.assign(C.rows.size() * (size_t)n, -1);
+      const size_t na = PA.rows.size(), nb = PB.rows.size();
+      int32_t ia = 0, jbb = 0;
+      for (int32_t x = 0; x < n; ++x) {
+        const uint8_t op = o[n - 1 - x];
+        if (op != 'D') {
+          for (size_t r = 0; r < na; ++r) C.cell[r * n + x] = PA.cell[r * PA.len + ia];
+          ++ia;
+        }
+        if (op != 'I') {
+          for (size_t r = 0; r < nb; ++r) C.cell[(na + r) * n + x] = PB.cell[r * PB.len + jbb];
+          ++jbb;
+        }
+      }
+      prof[c] = std::move(C);
+      PA = Profile();
+      PB = Profile();
+    }
+  }
+
+  // 4. every residue's column
+  const Profile &F = prof[nc - 1];
+  *alen = F.len;
+  for (size_t r = 0; r < F.rows.size(); ++r) {
+    const int32_t s = F.rows[r];
+    int64_t k = off[s];
+    for (int32_t x = 0; x < F.len; ++x)
+      if (F.cell[r * F.len + x] >= 0) cols[k++] = x;
+  }
+  return TA_OK;
+}

@@ -76,6 +76,7 @@ def lib():
         "ta_matrix": ([i32, i32, i32, P(ctypes.c_int8), P(i32)], ctypes.c_int),
         "ta_encode": ([i32, ctypes.c_char_p, i32, P(ctypes.c_uint8)], ctypes.c_int),
         "ta_consensus": ([vp, P(ctypes.c_char_p), i32, i32, ctypes.c_double, ctypes.c_char_p], ctypes.c_int),
+        "ta_msa": ([vp, P(Params), P(ctypes.c_char_p), P(i32), i32, P(i32), P(i32), P(i32)], ctypes.c_int),
         "ta_workload_create": ([i32, ctypes.c_uint64, i32, i32, P(vp)], ctypes.c_int),
         "ta_workload_destroy": ([vp], None),
         "ta_workload_npairs": ([vp], i32),
@@ -238,6 +239,30 @@ class Engine:
         out = ctypes.create_string_buffer(max(L, 1))
         _check(lib().ta_consensus(self.ctx, arr, n, L, float(plurality), out), "ta_consensus")
         return out.raw[:L]
+
+    def msa(self, seqs, params=None):
+        """Progressive multiple alignment on the GPU.  Returns (rows, joins):
+        the aligned rows ('-' for gaps) in input order and the UPGMA joins."""
+        raw = [s.encode("latin-1") if isinstance(s, str) else bytes(s) for s in seqs]
+        n = len(raw)
+        if params is None:
+            params = make_params(mode=GLOBAL)
+        lens = (ctypes.c_int32 * max(n, 1))(*[len(s) for s in raw])
+        arr = (ctypes.c_char_p * max(n, 1))(*raw)
+        total = sum(len(s) for s in raw)
+        cols = (ctypes.c_int32 * max(total, 1))()
+        joins = (ctypes.c_int32 * max(2 * (n - 1), 1))()
+        alen = ctypes.c_int32()
+        _check(lib().ta_msa(self.ctx, ctypes.byref(params), arr, lens, n, ctypes.byref(alen), cols, joins),
+               "ta_msa")
+        rows, k = [], 0
+        for s in raw:
+            r = bytearray(b"-" * alen.value)
+            for ch in s:
+                r[cols[k]] = ch
+                k += 1
+            rows.append(r.decode("latin-1"))
+        return rows, [(joins[2 * t], joins[2 * t + 1]) for t in range(n - 1)]
 
     # autotuning
     def autotune(self, pairs, params, repeats=2):
