@@ -19,12 +19,15 @@
 //     direction bytes back from (la, lb) to ops; the host applies the ops to
 //     the profiles' rows.
 //
-// The join DP sweeps anti-diagonals: thread t owns rows t+1, t+1+256, ...
-// (up to 8 each, so profiles of up to 2,048 columns take the DP rows), keeps
-// H and E of its rows' last cells in registers, and reads the row above
-// (H, F at the previous diagonal, H at the one before) from a three-slot LDS
-// ring of int64 values, one barrier per diagonal.  Values are int64: a join
-// of two 256-sequence profiles scores up to 11 * 65,536 per column pair.
+// A join first computes every column pair's match score at once (int32:
+// nA * nB is bounded so that 24 codes x 11 x nA * nB fits), then sweeps
+// anti-diagonals: thread t owns rows t+1, t+1+256, ... (up to 8 each, so
+// profiles of up to 2,048 columns take the DP rows), keeps H and E of its
+// rows' last cells in registers, and reads the row above (H, F at the
+// previous diagonal, H at the one before) from a three-slot LDS ring of int64
+// values, one barrier per diagonal.  The walk back runs on thread 0 over
+// 64 x 64 tiles of direction bytes that the workgroup stages in LDS, one
+// barrier pair per tile instead of a global load per step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,13 +42,16 @@ namespace {
 constexpr int JT = 256;                  // threads per join
 constexpr int JR = 8;                    // DP rows per thread
 constexpr int JMAX = JT * JR;            // longest row profile
+constexpr int TW = 64;                   // walk tile: TW x TW direction bytes in LDS
 constexpr int64_t JNEG = INT64_MIN / 4;  // -inf
+constexpr int64_t WMAX = 8000000;        // nA * nB bound: 24 codes x 11 x nA * nB fits int32
 
 struct JoinJob {
   int32_t la, lb;
   int64_t w;        // nA * nB: gap costs scale
   int64_t cnt_off;  // la x K: residues of each code in A's columns
   int64_t pb_off;   // lb x K: PB[j][x] = sum_y S(x, y) cntB[j][y]
+  int64_t m_off;    // la x lb match scores M(i, j) (scratch)
   int64_t tb_off;   // (la + 1) x (lb + 1) direction bytes: src | e << 2 | f << 3
   int64_t ops_off;  // la + lb ops, walk order (last column first)
 };
@@ -53,6 +59,7 @@ struct JoinJob {
 struct JoinArgs {
   const JoinJob *jobs;
   const int32_t *cnt, *pb;
+  int32_t *mbuf;
   uint8_t *tb;
   uint8_t *ops;
   int32_t *nops;
@@ -61,14 +68,29 @@ struct JoinArgs {
 
 __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
   extern __shared__ int64_t ring[];  // H[3][la + 1], then F[3][la + 1]
+  __shared__ uint8_t tile[TW * TW];
+  __shared__ int32_t walk[4];        // i, j, state, finished
   const JoinJob J = A.jobs[blockIdx.x];
   const int la = J.la, lb = J.lb, K = A.K, tid = threadIdx.x;
   const int64_t go = (int64_t)A.open * J.w, ge = (int64_t)A.ext * J.w;
   const int rs = la + 1;
   int64_t *Hr = ring, *Fr = ring + 3 * rs;
   const int32_t *cnt = A.cnt + J.cnt_off, *pb = A.pb + J.pb_off;
+  int32_t *mm = A.mbuf + J.m_off;
   uint8_t *tb = A.tb + J.tb_off;
   const int64_t stride = lb + 1;
+
+  // ---- match scores, all cells at once
+  for (int c = tid; c < la * lb; c += JT) {
+    const int i = c / lb, j = c - i * lb;
+    const int32_t *ca = cnt + (int64_t)i * K, *pj = pb + (int64_t)j * K;
+    int32_t m = 0;
+    for (int x = 0; x < K; ++x) m += ca[x] * pj[x];
+    mm[c] = m;
+  }
+  __syncthreads();
+
+  // ---- the DP, one anti-diagonal d = i + j per barrier
   int64_t Hl[JR], El[JR];  // H(i, j-1), E(i, j-1) of each owned row
 #pragma unroll
   for (int r = 0; r < JR; ++r) {
@@ -93,16 +115,13 @@ __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
         Fup = Fr[prv * rs + i - 1];
         Hdg = j == 1 ? go + (int64_t)(i - 2) * ge : Hr[pp * rs + i - 1];
       }
-      const int32_t *ca = cnt + (int64_t)(i - 1) * K, *pj = pb + (int64_t)(j - 1) * K;
-      int64_t m = 0;
-      for (int x = 0; x < K; ++x) m += (int64_t)ca[x] * pj[x];
       const int64_t eo = Hl[r] + go, ee = El[r] + ge;
       const bool eb = ee > eo;
       const int64_t E = eb ? ee : eo;
       const int64_t fo = Hup + go, fe = Fup + ge;
       const bool fb = fe > fo;
       const int64_t F = fb ? fe : fo;
-      int64_t h = Hdg + m;
+      int64_t h = Hdg + mm[(int64_t)(i - 1) * lb + (j - 1)];
       int src = 0;
       if (E > h) { h = E; src = 1; }
       if (F > h) { h = F; src = 2; }
@@ -114,29 +133,51 @@ __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
     }
     __syncthreads();
   }
-  if (tid != 0) return;
-  // Walk back from (la, lb): the pairwise state machine (H, E 'D', F 'I').
+
+  // ---- walk back from (la, lb): the pairwise state machine (H, E 'D',
+  // F 'I').  The workgroup stages the TW x TW tile of direction bytes ending
+  // at the walk's cell in LDS; thread 0 walks it until the walk leaves it.
   uint8_t *ops = A.ops + J.ops_off;
-  int i = la, j = lb, k = 0, state = 0;
+  if (tid == 0) { walk[0] = la; walk[1] = lb; walk[2] = 0; walk[3] = 0; }
+  int k = 0;  // ops written (thread 0)
+  __syncthreads();
   for (;;) {
-    if (i == 0) { while (j-- > 0) ops[k++] = 'D'; break; }
-    if (j == 0) { while (i-- > 0) ops[k++] = 'I'; break; }
-    const int code = tb[(int64_t)i * stride + j];
-    if (state == 0) {
-      const int s = code & 3;
-      if (s == 0) { ops[k++] = 'M'; --i; --j; }
-      else state = s;
-    } else if (state == 1) {
-      ops[k++] = 'D';
-      if (!(code & 4)) state = 0;
-      --j;
-    } else {
-      ops[k++] = 'I';
-      if (!(code & 8)) state = 0;
-      --i;
+    const int wi = walk[0], wj = walk[1];
+    if (walk[3]) break;
+    const int i0 = wi - (TW - 1) > 0 ? wi - (TW - 1) : 0, j0 = wj - (TW - 1) > 0 ? wj - (TW - 1) : 0;
+    const int ni = wi - i0 + 1, nj = wj - j0 + 1;
+    for (int c = tid; c < ni * nj; c += JT) {
+      const int r = c / nj, q = c - r * nj;
+      tile[r * TW + q] = tb[(int64_t)(i0 + r) * stride + (j0 + q)];
     }
+    __syncthreads();
+    if (tid == 0) {
+      int i = wi, j = wj, state = walk[2];
+      bool fin = false;
+      for (;;) {
+        if (i == 0) { while (j-- > 0) ops[k++] = 'D'; fin = true; break; }
+        if (j == 0) { while (i-- > 0) ops[k++] = 'I'; fin = true; break; }
+        if (i < i0 || j < j0) break;  // left the tile
+        const int code = tile[(i - i0) * TW + (j - j0)];
+        if (state == 0) {
+          const int s = code & 3;
+          if (s == 0) { ops[k++] = 'M'; --i; --j; }
+          else state = s;
+        } else if (state == 1) {
+          ops[k++] = 'D';
+          if (!(code & 4)) state = 0;
+          --j;
+        } else {
+          ops[k++] = 'I';
+          if (!(code & 8)) state = 0;
+          --i;
+        }
+      }
+      walk[0] = i; walk[1] = j; walk[2] = state; walk[3] = fin;
+    }
+    __syncthreads();
   }
-  A.nops[blockIdx.x] = k;
+  if (tid == 0) A.nops[blockIdx.x] = k;
 }
 
 struct Profile {
@@ -194,24 +235,31 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       }
   }
 
-  // 2. UPGMA joins (exact), and each join's tree level
+  // 2. UPGMA joins (exact), and each join's tree level.  Row a keeps its
+  // best partner b > a (largest mean, ties to the smaller b); the join is the
+  // best row's (ties to the smaller a) -- the lexicographically first pair of
+  // largest mean.  After a join only rows whose partner died are rescanned;
+  // every other row compares its partner with the new cluster.
   std::vector<int32_t> size(nc, 0), level(nc, 0), ja(nseq > 1 ? nseq - 1 : 0), jb(ja.size());
   std::vector<char> live(nc, 0);
+  std::vector<int32_t> best(nc, -1);
   for (int32_t s = 0; s < nseq; ++s) { size[s] = 1; live[s] = 1; }
+  // mean(a, b) > mean(x, y), exactly
+  auto above = [&](int32_t a, int32_t b, int32_t x, int32_t y) {
+    return (__int128)sum[(size_t)a * nc + b] * size[x] * size[y] > (__int128)sum[(size_t)x * nc + y] * size[a] * size[b];
+  };
+  auto rescan = [&](int32_t a, int32_t end) {
+    best[a] = -1;
+    for (int32_t b = a + 1; b < end; ++b)
+      if (live[b] && (best[a] < 0 || above(a, b, a, best[a]))) best[a] = b;
+  };
+  for (int32_t a = 0; a < nseq; ++a) rescan(a, nseq);
   int32_t top = 0;
   for (int32_t t = 0; t + 1 < nseq; ++t) {
-    int32_t ba = -1, bb = -1;
-    for (int32_t a = 0; a < nseq + t; ++a) {
-      if (!live[a]) continue;
-      for (int32_t b = a + 1; b < nseq + t; ++b) {
-        if (!live[b]) continue;
-        if (ba < 0) { ba = a; bb = b; continue; }
-        const __int128 lhs = (__int128)sum[(size_t)a * nc + b] * size[ba] * size[bb];
-        const __int128 rhs = (__int128)sum[(size_t)ba * nc + bb] * size[a] * size[b];
-        if (lhs > rhs) { ba = a; bb = b; }
-      }
-    }
-    const int32_t c = nseq + t;
+    int32_t ba = -1;
+    for (int32_t a = 0; a < nseq + t; ++a)
+      if (live[a] && best[a] >= 0 && (ba < 0 || above(a, best[a], ba, best[ba]))) ba = a;
+    const int32_t bb = best[ba], c = nseq + t;
     ja[t] = ba; jb[t] = bb;
     if (joins) { joins[2 * t] = ba; joins[2 * t + 1] = bb; }
     size[c] = size[ba] + size[bb];
@@ -221,6 +269,11 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
     live[c] = 1;
     for (int32_t x = 0; x < c; ++x)
       if (live[x]) sum[(size_t)c * nc + x] = sum[(size_t)x * nc + c] = sum[(size_t)ba * nc + x] + sum[(size_t)bb * nc + x];
+    for (int32_t a = 0; a < c; ++a) {
+      if (!live[a]) continue;
+      if (best[a] == ba || best[a] == bb || best[a] < 0) rescan(a, c + 1);
+      else if (above(a, c, a, best[a])) best[a] = c;
+    }
   }
 
   // 3. joins level by level
@@ -237,14 +290,15 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       if (level[nseq + t] == L) ts.push_back(t);
     std::vector<JoinJob> jobs(ts.size());
     std::vector<int32_t> hcnt, hpb;
-    int64_t tb_total = 0, ops_total = 0;
+    int64_t tb_total = 0, ops_total = 0, m_total = 0;
     int max_la = 0;
     for (size_t q = 0; q < ts.size(); ++q) {
       const Profile &PA = prof[ja[ts[q]]], &PB = prof[jb[ts[q]]];
-      if (PA.len > JMAX) return TA_E_INVALID;  // longer row profiles are not supported
       JoinJob &J = jobs[q];
       J.la = PA.len; J.lb = PB.len;
       J.w = (int64_t)PA.rows.size() * (int64_t)PB.rows.size();
+      // longer row profiles, and joins whose column scores could leave int32
+      if (PA.len > JMAX || J.w > WMAX || (int64_t)PA.len * PB.len > INT32_MAX) return TA_E_INVALID;
       J.cnt_off = (int64_t)hcnt.size();
       hcnt.resize(hcnt.size() + (size_t)PA.len * K, 0);
       int32_t *ca = hcnt.data() + J.cnt_off;
@@ -262,6 +316,8 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
           if (c < 0) continue;
           for (int32_t x = 0; x < K; ++x) pbj[(size_t)j * K + x] += S[x * K + c];
         }
+      J.m_off = m_total;
+      m_total += (int64_t)PA.len * PB.len;
       J.tb_off = tb_total;
       tb_total += (int64_t)(PA.len + 1) * (PB.len + 1);
       J.ops_off = ops_total;
@@ -272,12 +328,13 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
     JoinJob *d_jobs = (JoinJob *)device_alloc(sizeof(JoinJob) * jobs.size());
     int32_t *d_cnt = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hcnt.size()));
     int32_t *d_pb = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hpb.size()));
+    int32_t *d_m = (int32_t *)device_alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(1, m_total));
     uint8_t *d_tb = (uint8_t *)device_alloc((size_t)std::max<int64_t>(1, tb_total));
     uint8_t *d_ops = (uint8_t *)device_alloc((size_t)std::max<int64_t>(1, ops_total));
     int32_t *d_nops = (int32_t *)device_alloc(sizeof(int32_t) * jobs.size());
     std::vector<uint8_t> hops((size_t)std::max<int64_t>(1, ops_total));
     std::vector<int32_t> hnops(jobs.size());
-    rc = (d_jobs && d_cnt && d_pb && d_tb && d_ops && d_nops) ? TA_OK : TA_E_NOMEM;
+    rc = (d_jobs && d_cnt && d_pb && d_m && d_tb && d_ops && d_nops) ? TA_OK : TA_E_NOMEM;
     if (!rc && (hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JoinJob) * jobs.size(), hipMemcpyHostToDevice, st) ||
                 (!hcnt.empty() &&
                  hipMemcpyAsync(d_cnt, hcnt.data(), sizeof(int32_t) * hcnt.size(), hipMemcpyHostToDevice, st)) ||
@@ -286,7 +343,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       rc = TA_E_DEVICE;
     if (!rc) {
       JoinArgs a;
-      a.jobs = d_jobs; a.cnt = d_cnt; a.pb = d_pb; a.tb = d_tb; a.ops = d_ops; a.nops = d_nops;
+      a.jobs = d_jobs; a.cnt = d_cnt; a.pb = d_pb; a.mbuf = d_m; a.tb = d_tb; a.ops = d_ops; a.nops = d_nops;
       a.K = K; a.open = p->gap_open; a.ext = p->gap_extend;
       (void)hipGetLastError();
       hipLaunchKernelGGL(msa_join_kernel, dim3((unsigned)jobs.size()), dim3(JT), lds, st, a);
@@ -296,7 +353,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
           hipStreamSynchronize(st))
         rc = TA_E_DEVICE;
     }
-    device_free(d_jobs); device_free(d_cnt); device_free(d_pb);
+    device_free(d_jobs); device_free(d_cnt); device_free(d_pb); device_free(d_m);
     device_free(d_tb); device_free(d_ops); device_free(d_nops);
     if (rc) return rc;
     // apply each join's ops (walk order: read backwards)
