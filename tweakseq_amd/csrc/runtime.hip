@@ -12,7 +12,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -342,9 +344,17 @@ bool long_pair(const ta_params &p, int la, int lb) {
   return p.lanes == TA_LANES_AUTO && (int64_t)la * lb >= kLongCells;
 }
 
+// Residue codes one task's strips and columns read (filled by fill_codes).
+struct CodeJob {
+  uint64_t row_off, col_off;
+  int32_t nrows, ncol, lo, hi, off0, off1;
+};
+
 struct Plan {
   std::vector<Task> tasks2[2], tasks1[2];  // [0] whole-wave groups, [1] G-lane groups (NGW per wave unit)
   std::vector<uint16_t> rowcodes, colcodes;
+  std::vector<CodeJob> cjobs;
+  uint64_t rows_n = 0, cols_n = 0;        // rowcodes / colcodes entries planned
   std::vector<PairTB> ptb;
   uint64_t ring_elems = 0, tb_bytes = 0;
 };
@@ -367,6 +377,51 @@ int fill_waves_per_cu(int P, int Kp, bool nib, int want, bool walk = false) {
 // by groups of GG lanes.
 // n_force / S_force (wave units): columns and strips of the task's unit; extra
 // columns lead (local couples only), extra strips are pad rows.
+// Runs f(i) for i in [0, n) on up to 16 host threads in contiguous chunks
+// (f must not depend on the order).
+template <class F>
+void parallel_for(size_t n, F f) {
+  const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t nt = std::min<size_t>(hw, (n + 255) / 256);
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (size_t t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      for (size_t i = n * t / nt; i < n * (t + 1) / nt; ++i) f(i);
+    });
+  for (auto &x : th) x.join();
+}
+
+// The row and column codes of every planned task: row r of a half is its
+// pair's query residue r (pad past m), column j its reference residue
+// j - off (pad before the leading columns and past n).
+void fill_codes(Plan &pl, const std::vector<HostPair> &hp, const std::vector<uint8_t> &codes, int Kp) {
+  const uint16_t pad = (uint16_t)(Kp - 1);
+  pl.rowcodes.resize(pl.rows_n);
+  pl.colcodes.resize(pl.cols_n);
+  parallel_for(pl.cjobs.size(), [&](size_t q) {
+    const CodeJob &c = pl.cjobs[q];
+    const HostPair &A = hp[c.lo], &B = hp[c.hi];
+    uint16_t *rows = pl.rowcodes.data() + c.row_off;
+    for (int r = 0; r < c.nrows; ++r) {
+      const uint16_t ca = r < A.la ? codes[A.a_off + r] : pad;
+      const uint16_t cb = r < B.la ? codes[B.a_off + r] : pad;
+      rows[r] = (uint16_t)(ca | (cb << 8));
+    }
+    uint16_t *cols = pl.colcodes.data() + c.col_off;
+    for (int j = 0; j < c.ncol; ++j) {
+      const int ja = j - c.off0, jb = j - c.off1;
+      const uint16_t ca = (ja >= 0 && ja < A.lb) ? codes[A.b_off + ja] : pad;
+      const uint16_t cb = (jb >= 0 && jb < B.lb) ? codes[B.b_off + jb] : pad;
+      cols[j] = (uint16_t)(ca | (cb << 8));
+    }
+  });
+}
+
 void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const std::vector<HostPair> &hp,
               const std::vector<uint8_t> &codes, int lo, int hi, bool single, int Kp, int n_force = 0,
               int S_force = 0) {
@@ -380,24 +435,14 @@ void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const
   T.S = S_force ? S_force : ceil_div(std::max(A.la, B.la), ST);
   T.NB = ceil_div(T.n + GG, 8);
   T.single = single ? 1 : 0;
-  const uint16_t pad = (uint16_t)(Kp - 1);
-  // Row codes: S * ST entries, pad rows past each pair's m.
-  T.row_off = pl.rowcodes.size();
-  for (int r = 0; r < T.S * ST; ++r) {
-    uint16_t ca = r < A.la ? codes[A.a_off + r] : pad;
-    uint16_t cb = r < B.la ? codes[B.a_off + r] : pad;
-    pl.rowcodes.push_back((uint16_t)(ca | (cb << 8)));
-  }
-  // Column codes: leading pad columns for the shorter pair, pad slack after n.
-  T.col_off = pl.colcodes.size();
+  // Row codes: S * ST entries; column codes: 8 NB + 8 (a multiple of 8, so
+  // every task's columns start 16-byte aligned).  Filled by fill_codes.
+  T.row_off = pl.rows_n;
+  T.col_off = pl.cols_n;
   const int ncol = 8 * T.NB + 8;
-  for (int j = 0; j < ncol; ++j) {
-    int ja = j - T.off[0], jb = j - T.off[1];
-    uint16_t ca = (ja >= 0 && ja < A.lb) ? codes[A.b_off + ja] : pad;
-    uint16_t cb = (jb >= 0 && jb < B.lb) ? codes[B.b_off + jb] : pad;
-    pl.colcodes.push_back((uint16_t)(ca | (cb << 8)));
-  }
-  while (pl.colcodes.size() % 8) pl.colcodes.push_back(pad | (pad << 8));
+  pl.cjobs.push_back(CodeJob{T.row_off, T.col_off, T.S * ST, ncol, lo, hi, T.off[0], T.off[1]});
+  pl.rows_n += (uint64_t)T.S * ST;
+  pl.cols_n += (uint64_t)ncol;
   T.ring_len = T.S > 1 ? 8 * T.NB + 2 * GG + 16 : 0;
   T.ring_off = pl.ring_elems;
   pl.ring_elems += 2ull * T.ring_len;
@@ -661,38 +706,23 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   int best[32] = {0};
   for (int r = 0; r < B->alph.K; ++r)
     for (int c = 0; c < B->alph.K; ++c) best[r] = std::max<int>(best[r], B->alph.S[r * B->alph.K + c]);
-  struct Enc { uint64_t off; int32_t len; int64_t bound; };
-  std::unordered_map<const char *, Enc> seen;
-  {
-    size_t total = 64;
-    for (int32_t k = 0; k < n; ++k) total += (size_t)std::max(pairs[k].la, 0) + std::max(pairs[k].lb, 0) + 32;
-    B->codes.reserve(std::min<size_t>(total, (size_t)1 << 34));
-  }
-  auto enc = [&](const char *s, int32_t len, uint64_t *off, int64_t *bound) -> int {
+  // Unique sequences first (by pointer and length), their offsets in the
+  // code array, then their codes and bounds on the host threads.
+  struct Uniq { const char *s; int32_t len; uint64_t off; int64_t bound; };
+  std::vector<Uniq> uq;
+  std::unordered_map<const char *, int32_t> seen;
+  seen.reserve(2 * (size_t)n);
+  std::vector<int32_t> ua((size_t)n), ub((size_t)n);
+  size_t ctotal = 0;
+  auto uniq = [&](const char *s, int32_t len) -> int32_t {
     auto it = seen.find(s);
-    if (it != seen.end() && it->second.len == len) {
-      *off = it->second.off;
-      *bound = it->second.bound;
-      return TA_OK;
-    }
-    const size_t at = (B->codes.size() + 15) & ~(size_t)15;  // 16-byte aligned starts (walker windows)
-    B->codes.resize(at + (size_t)len);
-    uint8_t *dst = B->codes.data() + at;
-    int64_t sum = 0;
-    for (int32_t i = 0; i < len; ++i) {
-      const int c = B->alph.map[(unsigned char)s[i]];
-      if (c < 0) return TA_E_ALPHABET;
-      dst[i] = (uint8_t)c;
-      sum += best[c];
-    }
-    *off = at;
-    *bound = sum;
-    seen[s] = Enc{at, len, sum};
-    return TA_OK;
+    if (it != seen.end() && uq[it->second].len == len) return it->second;
+    const size_t at = (ctotal + 15) & ~(size_t)15;  // 16-byte aligned starts (walker windows)
+    uq.push_back(Uniq{s, len, at, 0});
+    ctotal = at + (size_t)len;
+    seen[s] = (int32_t)uq.size() - 1;
+    return (int32_t)uq.size() - 1;
   };
-  B->hp.resize(n);
-  B->fo_init.assign(n, FillOut{0, 0, 0, 0});
-  B->pairs_h.resize(n);
   for (int32_t k = 0; k < n; ++k) {
     const ta_pair &q = pairs[k];
     if (q.la < 0 || q.lb < 0 || q.la > 1000000 || q.lb > 1000000 || (q.la && !q.a) || (q.lb && !q.b)) {
@@ -704,11 +734,37 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       delete B;
       return TA_E_INVALID;
     }
+    ua[k] = uniq(q.a, q.la);
+    ub[k] = uniq(q.b, q.lb);
+  }
+  // the walker reads 32-byte windows: 32 bytes of slack after the last one
+  B->codes.assign(((ctotal + 15) & ~(size_t)15) + 32, 0);
+  std::atomic<int> bad_letter{0};
+  parallel_for(uq.size(), [&](size_t u) {
+    Uniq &e = uq[u];
+    uint8_t *dst = B->codes.data() + e.off;
+    int64_t sum = 0;
+    for (int32_t i = 0; i < e.len; ++i) {
+      const int c = B->alph.map[(unsigned char)e.s[i]];
+      if (c < 0) { bad_letter = 1; return; }
+      dst[i] = (uint8_t)c;
+      sum += best[c];
+    }
+    e.bound = sum;
+  });
+  if (bad_letter) {
+    delete B;
+    return TA_E_ALPHABET;
+  }
+  B->hp.resize(n);
+  B->fo_init.assign(n, FillOut{0, 0, 0, 0});
+  B->pairs_h.resize(n);
+  for (int32_t k = 0; k < n; ++k) {
+    const ta_pair &q = pairs[k];
     HostPair &h = B->hp[k];
-    int64_t bound_a = 0, bound_b = 0;
-    int rc = enc(q.a, q.la, &h.a_off, &bound_a);
-    if (!rc) rc = enc(q.b, q.lb, &h.b_off, &bound_b);
-    if (rc) { delete B; return rc; }
+    h.a_off = uq[ua[k]].off;
+    h.b_off = uq[ub[k]].off;
+    const int64_t bound_a = uq[ua[k]].bound, bound_b = uq[ub[k]].bound;
     h.la = q.la; h.lb = q.lb;
     B->cells += (int64_t)q.la * q.lb;
     if (q.la == 0 || q.lb == 0) {
@@ -738,8 +794,6 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     d.cig_off = B->cig_bytes;
     B->cig_bytes += (uint64_t)d.cig_cap;
   }
-  while (B->codes.size() % 16) B->codes.push_back(0);
-  B->codes.insert(B->codes.end(), 32, 0);  // the walker reads 32-byte windows
 
   // Couples: local pairs pad to the longer column count; global couples
   // need equal column counts (so H(m, n) of both sits in the last column).
@@ -876,6 +930,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     rc = rc ? rc : upload(&B->d_tasks2[g], pl.tasks2[g], st);
     rc = rc ? rc : upload(&B->d_tasks1[g], pl.tasks1[g], st);
   }
+  fill_codes(pl, B->hp, B->codes, B->Kp);
   rc = rc ? rc : upload(&B->d_rowcodes, pl.rowcodes, st);
   rc = rc ? rc : upload(&B->d_colcodes, pl.colcodes, st);
   rc = rc ? rc : upload(&B->d_ptb, pl.ptb, st);
@@ -1104,6 +1159,7 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   hipStream_t st = B->stream;
   int rc = TA_OK;
   rc = rc ? rc : upload(&B->d_tasks_r, all, st);
+  fill_codes(pl, B->hp, B->codes, B->Kp);
   rc = rc ? rc : upload(&B->d_rowcodes_r, pl.rowcodes, st);
   rc = rc ? rc : upload(&B->d_colcodes_r, pl.colcodes, st);
   rc = rc ? rc : dalloc(&B->d_ring_r, pl.ring_elems);
@@ -1343,23 +1399,25 @@ int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, 
   std::vector<int64_t> off;
   int rc = fetch_results(B, res, runs, off);
   if (rc) return rc;
+  // CIGAR text lengths, their offsets, then the text, the first and last
+  // passes on the host threads.
+  const bool tb = B->p.traceback != 0;
+  parallel_for((size_t)B->n, [&](size_t k) {
+    const ResultDev &r = res[k];
+    fill_result(out[k], r);
+    if (tb && r.cigar_len > 0) out[k].cigar_len = (int32_t)format_runs(runs.data() + off[k] / 4, r.cigar_len / 4, nullptr);
+  });
   int64_t pos = 0;
   for (int32_t k = 0; k < B->n; ++k) {
-    const ResultDev &r = res[k];
-    ta_result &o = out[k];
-    fill_result(o, r);
-    o.cigar_off = pos;
-    if (B->p.traceback && r.cigar_len > 0) {
-      const uint32_t *rk = runs.data() + off[k] / 4;
-      const int nr = r.cigar_len / 4;
-      const int64_t need = format_runs(rk, nr, nullptr);
-      if (cigar_buf && pos + need <= cap) format_runs(rk, nr, cigar_buf + pos);
-      else rc = TA_E_SPACE;
-      o.cigar_len = (int32_t)need;
-    }
-    pos += o.cigar_len;
+    out[k].cigar_off = pos;
+    pos += out[k].cigar_len;
   }
   if (used) *used = pos;
+  if (pos > 0 && (!cigar_buf || pos > cap)) return TA_E_SPACE;
+  parallel_for((size_t)B->n, [&](size_t k) {
+    if (out[k].cigar_len > 0)
+      format_runs(runs.data() + off[k] / 4, res[k].cigar_len / 4, cigar_buf + out[k].cigar_off);
+  });
   return rc;
 }
 
@@ -1448,6 +1506,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     }
     int rc = TA_OK;
     rc = rc ? rc : upload(&B->d_plain_tasks, tasks, st);
+    fill_codes(pl, B->hp, B->codes, B->Kp);
     rc = rc ? rc : upload(&B->d_prow, pl.rowcodes, st);
     rc = rc ? rc : upload(&B->d_pcol, pl.colcodes, st);
     rc = rc ? rc : upload(&B->d_pptb, pl.ptb, st);
