@@ -248,6 +248,7 @@ void dfree(T *&p) {
   p = nullptr;
 }
 
+
 }  // namespace
 
 // The device-memory cache for the library's other sources (runtime_internal.h).
@@ -344,7 +345,7 @@ bool long_pair(const ta_params &p, int la, int lb) {
   return p.lanes == TA_LANES_AUTO && (int64_t)la * lb >= kLongCells;
 }
 
-// Residue codes one task's strips and columns read (filled by fill_codes).
+// Residue codes one task's strips and columns read (expand_codes_kernel).
 struct CodeJob {
   uint64_t row_off, col_off;
   int32_t nrows, ncol, lo, hi, off0, off1;
@@ -352,7 +353,6 @@ struct CodeJob {
 
 struct Plan {
   std::vector<Task> tasks2[2], tasks1[2];  // [0] whole-wave groups, [1] G-lane groups (NGW per wave unit)
-  std::vector<uint16_t> rowcodes, colcodes;
   std::vector<CodeJob> cjobs;
   uint64_t rows_n = 0, cols_n = 0;        // rowcodes / colcodes entries planned
   std::vector<PairTB> ptb;
@@ -396,30 +396,47 @@ void parallel_for(size_t n, F f) {
   for (auto &x : th) x.join();
 }
 
-// The row and column codes of every planned task: row r of a half is its
-// pair's query residue r (pad past m), column j its reference residue
-// j - off (pad before the leading columns and past n).
-void fill_codes(Plan &pl, const std::vector<HostPair> &hp, const std::vector<uint8_t> &codes, int Kp) {
-  const uint16_t pad = (uint16_t)(Kp - 1);
-  pl.rowcodes.resize(pl.rows_n);
-  pl.colcodes.resize(pl.cols_n);
-  parallel_for(pl.cjobs.size(), [&](size_t q) {
-    const CodeJob &c = pl.cjobs[q];
-    const HostPair &A = hp[c.lo], &B = hp[c.hi];
-    uint16_t *rows = pl.rowcodes.data() + c.row_off;
-    for (int r = 0; r < c.nrows; ++r) {
-      const uint16_t ca = r < A.la ? codes[A.a_off + r] : pad;
-      const uint16_t cb = r < B.la ? codes[B.a_off + r] : pad;
-      rows[r] = (uint16_t)(ca | (cb << 8));
+// The row and column codes of every planned task, expanded on the GPU from
+// the uploaded residues: row r of a half is its pair's query residue r (pad
+// past m), column j its reference residue j - off (pad before the leading
+// columns and past n).  One workgroup per task.
+__global__ void expand_codes_kernel(const CodeJob *jobs, int njobs, const PairDev *pairs, const uint8_t *codes,
+                                    int pad, uint16_t *rowcodes, uint16_t *colcodes) {
+  for (int q = blockIdx.x; q < njobs; q += gridDim.x) {
+    const CodeJob c = jobs[q];
+    const PairDev A = pairs[c.lo], B = pairs[c.hi];
+    for (int r = threadIdx.x; r < c.nrows; r += blockDim.x) {
+      const int ca = r < A.la ? codes[A.a_off + r] : pad;
+      const int cb = r < B.la ? codes[B.a_off + r] : pad;
+      rowcodes[c.row_off + r] = (uint16_t)(ca | (cb << 8));
     }
-    uint16_t *cols = pl.colcodes.data() + c.col_off;
-    for (int j = 0; j < c.ncol; ++j) {
+    for (int j = threadIdx.x; j < c.ncol; j += blockDim.x) {
       const int ja = j - c.off0, jb = j - c.off1;
-      const uint16_t ca = (ja >= 0 && ja < A.lb) ? codes[A.b_off + ja] : pad;
-      const uint16_t cb = (jb >= 0 && jb < B.lb) ? codes[B.b_off + jb] : pad;
-      cols[j] = (uint16_t)(ca | (cb << 8));
+      const int ca = (ja >= 0 && ja < A.lb) ? codes[A.b_off + ja] : pad;
+      const int cb = (jb >= 0 && jb < B.lb) ? codes[B.b_off + jb] : pad;
+      colcodes[c.col_off + j] = (uint16_t)(ca | (cb << 8));
     }
-  });
+  }
+}
+
+// Allocates a plan's row and column codes on the device and expands them
+// there (the residues and pair descriptors are already uploaded).
+int expand_codes(const std::vector<CodeJob> &jobs, uint64_t rows_n, uint64_t cols_n, const PairDev *d_pairs,
+                 const uint8_t *d_codes, int Kp, uint16_t **d_rows, uint16_t **d_cols, hipStream_t st) {
+  int rc = dalloc(d_rows, (size_t)rows_n);
+  rc = rc ? rc : dalloc(d_cols, (size_t)cols_n);
+  if (rc || jobs.empty()) return rc;
+  CodeJob *d_jobs = nullptr;
+  rc = upload(&d_jobs, jobs, st);
+  if (rc) return rc;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(expand_codes_kernel, dim3((unsigned)std::min<size_t>(jobs.size(), 65536)), dim3(256), 0, st,
+                     d_jobs, (int)jobs.size(), d_pairs, d_codes, Kp - 1, *d_rows, *d_cols);
+  if (hipGetLastError() != hipSuccess) rc = TA_E_DEVICE;
+  // the job list goes back to the cache once the expansion has read it
+  if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = TA_E_DEVICE;
+  dfree(d_jobs);
+  return rc;
 }
 
 void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const std::vector<HostPair> &hp,
@@ -436,7 +453,7 @@ void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const
   T.NB = ceil_div(T.n + GG, 8);
   T.single = single ? 1 : 0;
   // Row codes: S * ST entries; column codes: 8 NB + 8 (a multiple of 8, so
-  // every task's columns start 16-byte aligned).  Filled by fill_codes.
+  // every task's columns start 16-byte aligned).  Expanded on the GPU.
   T.row_off = pl.rows_n;
   T.col_off = pl.cols_n;
   const int ncol = 8 * T.NB + 8;
@@ -930,11 +947,10 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     rc = rc ? rc : upload(&B->d_tasks2[g], pl.tasks2[g], st);
     rc = rc ? rc : upload(&B->d_tasks1[g], pl.tasks1[g], st);
   }
-  fill_codes(pl, B->hp, B->codes, B->Kp);
-  rc = rc ? rc : upload(&B->d_rowcodes, pl.rowcodes, st);
-  rc = rc ? rc : upload(&B->d_colcodes, pl.colcodes, st);
   rc = rc ? rc : upload(&B->d_ptb, pl.ptb, st);
   rc = rc ? rc : upload(&B->d_pairs, B->pairs_h, st);
+  rc = rc ? rc : expand_codes(pl.cjobs, pl.rows_n, pl.cols_n, B->d_pairs, B->d_codes, B->Kp, &B->d_rowcodes,
+                              &B->d_colcodes, st);
   rc = rc ? rc : upload(&B->d_fo_init, B->fo_init, st);
   {
     std::vector<int32_t> order(n);
@@ -1159,9 +1175,8 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   hipStream_t st = B->stream;
   int rc = TA_OK;
   rc = rc ? rc : upload(&B->d_tasks_r, all, st);
-  fill_codes(pl, B->hp, B->codes, B->Kp);
-  rc = rc ? rc : upload(&B->d_rowcodes_r, pl.rowcodes, st);
-  rc = rc ? rc : upload(&B->d_colcodes_r, pl.colcodes, st);
+  rc = rc ? rc : expand_codes(pl.cjobs, pl.rows_n, pl.cols_n, B->d_pairs, B->d_codes, B->Kp, &B->d_rowcodes_r,
+                              &B->d_colcodes_r, st);
   rc = rc ? rc : dalloc(&B->d_ring_r, pl.ring_elems);
   rc = rc ? rc : dalloc(&B->d_tb_r, pl.tb_bytes);
   if (rc) return rc;
@@ -1506,9 +1521,8 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     }
     int rc = TA_OK;
     rc = rc ? rc : upload(&B->d_plain_tasks, tasks, st);
-    fill_codes(pl, B->hp, B->codes, B->Kp);
-    rc = rc ? rc : upload(&B->d_prow, pl.rowcodes, st);
-    rc = rc ? rc : upload(&B->d_pcol, pl.colcodes, st);
+    rc = rc ? rc : expand_codes(pl.cjobs, pl.rows_n, pl.cols_n, B->d_pairs, B->d_codes, B->Kp, &B->d_prow,
+                                &B->d_pcol, st);
     rc = rc ? rc : upload(&B->d_pptb, pl.ptb, st);
     rc = rc ? rc : dalloc(&B->d_pring, max_ring);
     rc = rc ? rc : dalloc(&B->d_ptbuf, max_tb);
