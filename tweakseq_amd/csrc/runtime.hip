@@ -234,6 +234,13 @@ int dalloc(T **p, size_t n) {
   return *p ? TA_OK : TA_E_NOMEM;
 }
 
+// A byte that std::vector does not zero on resize (every byte of such a
+// buffer is written explicitly).
+struct RawByte {
+  uint8_t v;
+  RawByte() {}
+};
+
 template <typename T>
 int upload(T **dst, const std::vector<T> &src, hipStream_t st) {
   int rc = dalloc(dst, src.size());
@@ -440,8 +447,7 @@ int expand_codes(const std::vector<CodeJob> &jobs, uint64_t rows_n, uint64_t col
 }
 
 void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const std::vector<HostPair> &hp,
-              const std::vector<uint8_t> &codes, int lo, int hi, bool single, int Kp, int n_force = 0,
-              int S_force = 0) {
+              int lo, int hi, bool single, int Kp, int n_force = 0, int S_force = 0) {
   const HostPair &A = hp[lo], &B = hp[hi];
   const int ST = GG * R;
   Task T{};
@@ -487,7 +493,7 @@ struct Unit {
 // that write nothing.  Units go out largest first, flattened (NGW tasks per
 // unit).
 void add_units(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, bool local, bool trace,
-               const std::vector<HostPair> &hp, const std::vector<uint8_t> &codes, int Kp) {
+               const std::vector<HostPair> &hp, int Kp) {
   auto strips = [](int m) { return ceil_div(std::max(m, 1), G * R); };
   std::stable_sort(u.begin(), u.end(), [&](const Unit &x, const Unit &y) {
     if (!local && x.n != y.n) return x.n > y.n;
@@ -502,7 +508,7 @@ void add_units(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, boo
     for (size_t i = k; i < e; ++i) { nq = std::max(nq, u[i].n); mq = std::max(mq, u[i].m); }
     std::vector<Task> q;
     for (size_t i = k; i < e; ++i)
-      add_task(pl, q, P, G, trace, hp, codes, u[i].lo, u[i].hi, u[i].single, Kp, nq, strips(mq));
+      add_task(pl, q, P, G, trace, hp, u[i].lo, u[i].hi, u[i].single, Kp, nq, strips(mq));
     while (q.size() < (size_t)NGW) {
       Task f = q[0];
       f.single = 2;
@@ -559,7 +565,7 @@ struct ta_batch {
   bool may_saturate = false;
   int64_t cells = 0;
   std::vector<HostPair> hp;
-  std::vector<uint8_t> codes;
+  std::vector<RawByte> codes;  // residue codes of the unique sequences, 16-byte aligned starts
   std::vector<FillOut> fo_init;
   FillOut *d_fo_init = nullptr;
   int32_t *d_order = nullptr;  // traceback walker -> pair, longest first
@@ -754,21 +760,27 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     ua[k] = uniq(q.a, q.la);
     ub[k] = uniq(q.b, q.lb);
   }
-  // the walker reads 32-byte windows: 32 bytes of slack after the last one
-  B->codes.assign(((ctotal + 15) & ~(size_t)15) + 32, 0);
+  // The walker reads 32-byte windows: 32 bytes of slack after the last
+  // sequence.  Each host thread writes its sequences' codes and the zero
+  // padding after them (so the buffer's pages are first touched in parallel).
+  const size_t cend = ((ctotal + 15) & ~(size_t)15) + 32;
+  B->codes.resize(cend);
   std::atomic<int> bad_letter{0};
   parallel_for(uq.size(), [&](size_t u) {
     Uniq &e = uq[u];
-    uint8_t *dst = B->codes.data() + e.off;
+    uint8_t *dst = (uint8_t *)B->codes.data() + e.off;
     int64_t sum = 0;
     for (int32_t i = 0; i < e.len; ++i) {
       const int c = B->alph.map[(unsigned char)e.s[i]];
-      if (c < 0) { bad_letter = 1; return; }
+      if (c < 0) { bad_letter = 1; break; }
       dst[i] = (uint8_t)c;
       sum += best[c];
     }
     e.bound = sum;
+    const size_t next = u + 1 < uq.size() ? uq[u + 1].off : cend;
+    memset(dst + e.len, 0, next - (e.off + (size_t)e.len));
   });
+  if (uq.empty()) memset(B->codes.data(), 0, cend);
   if (bad_letter) {
     delete B;
     return TA_E_ALPHABET;
@@ -867,18 +879,18 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     const int hi = single ? c.first : c.second;
     auto sh = shape(c.first, hi);
     if (pick_gg(sh.first, sh.second, mean_slot, few) == GWAVE)
-      add_task(pl, pl.tasks2[0], 2, GWAVE, B->trace, B->hp, B->codes, c.first, hi, single, B->Kp);
+      add_task(pl, pl.tasks2[0], 2, GWAVE, B->trace, B->hp, c.first, hi, single, B->Kp);
     else
       u2.push_back(Unit{c.first, hi, single, sh.first, sh.second});
   }
   for (int k : i32) {
     if (pick_gg(B->hp[k].la, B->hp[k].lb, mean_slot, few) == GWAVE)
-      add_task(pl, pl.tasks1[0], 1, GWAVE, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+      add_task(pl, pl.tasks1[0], 1, GWAVE, B->trace, B->hp, k, k, true, B->Kp);
     else
       u1.push_back(Unit{k, k, true, B->hp[k].la, B->hp[k].lb});
   }
-  add_units(pl, pl.tasks2[1], 2, u2, local, B->trace, B->hp, B->codes, B->Kp);
-  add_units(pl, pl.tasks1[1], 1, u1, local, B->trace, B->hp, B->codes, B->Kp);
+  add_units(pl, pl.tasks2[1], 2, u2, local, B->trace, B->hp, B->Kp);
+  add_units(pl, pl.tasks1[1], 1, u1, local, B->trace, B->hp, B->Kp);
   auto by_cost = [](const Task &x, const Task &y) { return task_cost(x) > task_cost(y); };
   std::stable_sort(pl.tasks2[0].begin(), pl.tasks2[0].end(), by_cost);
   std::stable_sort(pl.tasks1[0].begin(), pl.tasks1[0].end(), by_cost);
@@ -940,7 +952,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   hipStream_t st = B->stream;
   std::vector<int8_t> table = padded_table(B->alph);
   int rc = TA_OK;
-  rc = rc ? rc : upload(&B->d_codes, B->codes, st);
+  rc = rc ? rc : dalloc(&B->d_codes, B->codes.size());
+  if (!rc) HIPCHK(hipMemcpyAsync(B->d_codes, B->codes.data(), B->codes.size(), hipMemcpyHostToDevice, st));
   rc = rc ? rc : upload(&B->d_table, table, st);
   if (B->nib && !B->trace) rc = rc ? rc : upload(&B->d_table_nib, nibble_table(B->alph), st);
   for (int g = 0; g < 2; ++g) {
@@ -1162,13 +1175,13 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   std::vector<Unit> u;
   for (int k : bad) {
     if (pick_gg(B->hp[k].la, B->hp[k].lb, 0.0, bad.size() <= (size_t)ctx->cus * 4) == GWAVE) {
-      add_task(pl, pl.tasks1[0], 1, GWAVE, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+      add_task(pl, pl.tasks1[0], 1, GWAVE, B->trace, B->hp, k, k, true, B->Kp);
       pl.tasks1[0].back().flags_or = TA_F_RERUN;
     } else {
       u.push_back(Unit{k, k, true, B->hp[k].la, B->hp[k].lb});
     }
   }
-  add_units(pl, pl.tasks1[1], 1, u, B->p.mode == TA_LOCAL, B->trace, B->hp, B->codes, B->Kp);
+  add_units(pl, pl.tasks1[1], 1, u, B->p.mode == TA_LOCAL, B->trace, B->hp, B->Kp);
   for (Task &t : pl.tasks1[1]) t.flags_or = TA_F_RERUN;
   std::vector<Task> all(pl.tasks1[0]);
   all.insert(all.end(), pl.tasks1[1].begin(), pl.tasks1[1].end());
@@ -1509,7 +1522,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
         continue;
       }
       const int GG = pick_gg(h.la, h.lb, 0.0, true);
-      add_task(pl, tasks, P, GG, B->trace, B->hp, B->codes, k, k, true, B->Kp);
+      add_task(pl, tasks, P, GG, B->trace, B->hp, k, k, true, B->Kp);
       // Every plain launch reuses the same scratch: rebase to 0.
       Task &T = tasks.back();
       max_tb = std::max(max_tb, pl.tb_bytes - tb0);
