@@ -27,7 +27,8 @@
 // previous diagonal, H at the one before) from a three-slot LDS ring of int64
 // values, one barrier per diagonal.  The walk back runs on thread 0 over
 // 64 x 64 tiles of direction bytes that the workgroup stages in LDS, one
-// barrier pair per tile instead of a global load per step.
+// barrier pair per tile instead of a global load per step.  Scores and
+// direction bytes are diagonal-major, so one diagonal's rows are adjacent.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,8 +52,8 @@ struct JoinJob {
   int64_t w;        // nA * nB: gap costs scale
   int64_t cnt_off;  // la x K: residues of each code in A's columns
   int64_t pb_off;   // lb x K: PB[j][x] = sum_y S(x, y) cntB[j][y]
-  int64_t m_off;    // la x lb match scores M(i, j) (scratch)
-  int64_t tb_off;   // (la + 1) x (lb + 1) direction bytes: src | e << 2 | f << 3
+  int64_t m_off;    // match scores M(i, j), diagonal-major: [i + j - 2][i - 1], (la + lb - 1) x la
+  int64_t tb_off;   // direction bytes src | e << 2 | f << 3, diagonal-major: [i + j][i], (la + lb + 1) x (la + 1)
   int64_t ops_off;  // la + lb ops, walk order (last column first)
 };
 
@@ -78,50 +79,70 @@ __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
   const int32_t *cnt = A.cnt + J.cnt_off, *pb = A.pb + J.pb_off;
   int32_t *mm = A.mbuf + J.m_off;
   uint8_t *tb = A.tb + J.tb_off;
-  const int64_t stride = lb + 1;
+  // Both are diagonal-major so that the rows of one diagonal (consecutive
+  // threads) read and write consecutive addresses.
+  const int64_t ms = la, ts = la + 1;
 
   // ---- match scores, all cells at once
   for (int c = tid; c < la * lb; c += JT) {
-    const int i = c / lb, j = c - i * lb;
+    const int i = c / lb, j = c - i * lb;  // 0-based
     const int32_t *ca = cnt + (int64_t)i * K, *pj = pb + (int64_t)j * K;
     int32_t m = 0;
     for (int x = 0; x < K; ++x) m += ca[x] * pj[x];
-    mm[c] = m;
+    mm[(int64_t)(i + j) * ms + i] = m;
   }
   __syncthreads();
 
   // ---- the DP, one anti-diagonal d = i + j per barrier
   int64_t Hl[JR], El[JR];  // H(i, j-1), E(i, j-1) of each owned row
+  int32_t mn[JR];          // match scores of the next diagonal's cells, loaded a diagonal early
 #pragma unroll
   for (int r = 0; r < JR; ++r) {
     const int i = tid + 1 + r * JT;
     Hl[r] = go + (int64_t)(i - 1) * ge;  // H(i, 0)
     El[r] = JNEG;
+    mn[r] = (i <= la && 2 - i >= 1) ? mm[(int64_t)0 * ms + (i - 1)] : 0;
   }
   for (int d = 2; d <= la + lb; ++d) {
     const int cur = d % 3, prv = (d + 2) % 3, pp = (d + 1) % 3;  // diagonals d, d-1, d-2
+    int32_t mc[JR];
+#pragma unroll
+    for (int r = 0; r < JR; ++r) {
+      const int i = tid + 1 + r * JT, j1 = d + 1 - i;
+      mc[r] = mn[r];
+      mn[r] = (i <= la && j1 >= 1 && j1 <= lb) ? mm[(int64_t)(d - 1) * ms + (i - 1)] : 0;
+    }
+    // Loads of every owned row first (all in flight together), then the
+    // cells: a row's ring stores would otherwise order the next row's loads.
+    int64_t Hup[JR], Fup[JR], Hdg[JR];
+#pragma unroll
+    for (int r = 0; r < JR; ++r) {
+      const int i = tid + 1 + r * JT;
+      const int j = d - i;
+      Hup[r] = Fup[r] = Hdg[r] = 0;
+      if (i > la || j < 1 || j > lb) continue;
+      if (i == 1) {  // row 0 is the boundary
+        Hup[r] = go + (int64_t)(j - 1) * ge;
+        Fup[r] = JNEG;
+        Hdg[r] = j == 1 ? 0 : go + (int64_t)(j - 2) * ge;
+      } else {
+        Hup[r] = Hr[prv * rs + i - 1];
+        Fup[r] = Fr[prv * rs + i - 1];
+        Hdg[r] = j == 1 ? go + (int64_t)(i - 2) * ge : Hr[pp * rs + i - 1];
+      }
+    }
 #pragma unroll
     for (int r = 0; r < JR; ++r) {
       const int i = tid + 1 + r * JT;
       const int j = d - i;
       if (i > la || j < 1 || j > lb) continue;
-      int64_t Hup, Fup, Hdg;
-      if (i == 1) {  // row 0 is the boundary
-        Hup = go + (int64_t)(j - 1) * ge;
-        Fup = JNEG;
-        Hdg = j == 1 ? 0 : go + (int64_t)(j - 2) * ge;
-      } else {
-        Hup = Hr[prv * rs + i - 1];
-        Fup = Fr[prv * rs + i - 1];
-        Hdg = j == 1 ? go + (int64_t)(i - 2) * ge : Hr[pp * rs + i - 1];
-      }
       const int64_t eo = Hl[r] + go, ee = El[r] + ge;
       const bool eb = ee > eo;
       const int64_t E = eb ? ee : eo;
-      const int64_t fo = Hup + go, fe = Fup + ge;
+      const int64_t fo = Hup[r] + go, fe = Fup[r] + ge;
       const bool fb = fe > fo;
       const int64_t F = fb ? fe : fo;
-      int64_t h = Hdg + mm[(int64_t)(i - 1) * lb + (j - 1)];
+      int64_t h = Hdg[r] + mc[r];
       int src = 0;
       if (E > h) { h = E; src = 1; }
       if (F > h) { h = F; src = 2; }
@@ -129,9 +150,14 @@ __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
       Fr[cur * rs + i] = F;
       Hl[r] = h;
       El[r] = E;
-      tb[(int64_t)i * stride + j] = (uint8_t)(src | ((int)eb << 2) | ((int)fb << 3));
+      tb[(int64_t)d * ts + i] = (uint8_t)(src | ((int)eb << 2) | ((int)fb << 3));
     }
-    __syncthreads();
+    // The ring is the only memory shared within a diagonal: an LDS-only
+    // barrier leaves the next diagonal's score loads in flight across it
+    // (the direction bytes are ordered by the full barrier before the walk).
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
   }
 
   // ---- walk back from (la, lb): the pairwise state machine (H, E 'D',
@@ -148,7 +174,7 @@ __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
     const int ni = wi - i0 + 1, nj = wj - j0 + 1;
     for (int c = tid; c < ni * nj; c += JT) {
       const int r = c / nj, q = c - r * nj;
-      tile[r * TW + q] = tb[(int64_t)(i0 + r) * stride + (j0 + q)];
+      tile[r * TW + q] = tb[(int64_t)(i0 + r + j0 + q) * ts + (i0 + r)];
     }
     __syncthreads();
     if (tid == 0) {
@@ -298,7 +324,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       J.la = PA.len; J.lb = PB.len;
       J.w = (int64_t)PA.rows.size() * (int64_t)PB.rows.size();
       // longer row profiles, and joins whose column scores could leave int32
-      if (PA.len > JMAX || J.w > WMAX || (int64_t)PA.len * PB.len > INT32_MAX) return TA_E_INVALID;
+      if (PA.len > JMAX || J.w > WMAX || (int64_t)PA.len * (PA.len + PB.len + 1) > INT32_MAX) return TA_E_INVALID;
       J.cnt_off = (int64_t)hcnt.size();
       hcnt.resize(hcnt.size() + (size_t)PA.len * K, 0);
       int32_t *ca = hcnt.data() + J.cnt_off;
@@ -317,9 +343,9 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
           for (int32_t x = 0; x < K; ++x) pbj[(size_t)j * K + x] += S[x * K + c];
         }
       J.m_off = m_total;
-      m_total += (int64_t)PA.len * PB.len;
+      m_total += (int64_t)(PA.len + PB.len - 1) * PA.len;
       J.tb_off = tb_total;
-      tb_total += (int64_t)(PA.len + 1) * (PB.len + 1);
+      tb_total += (int64_t)(PA.len + PB.len + 1) * (PA.len + 1);
       J.ops_off = ops_total;
       ops_total += PA.len + PB.len;
       max_la = std::max(max_la, PA.len);
