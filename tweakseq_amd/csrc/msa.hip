@@ -19,8 +19,9 @@
 //     direction bytes back from (la, lb) to ops; the host applies the ops to
 //     the profiles' rows.
 //
-// A join first computes every column pair's match score at once (int32:
-// nA * nB is bounded so that 24 codes x 11 x nA * nB fits), then sweeps
+// A join first computes every column pair's match score (int32: nA * nB is
+// bounded so that 24 codes x 11 x nA * nB fits; a thread's rows' counts sit
+// in registers, each column's PB row is a wave-uniform scalar load), then sweeps
 // anti-diagonals: thread t owns rows t+1, t+1+256, ... (up to 8 each, so
 // profiles of up to 2,048 columns take the DP rows), keeps H and E of its
 // rows' last cells in registers, and reads the row above (H, F at the
@@ -44,14 +45,15 @@ constexpr int JT = 256;                  // threads per join
 constexpr int JR = 8;                    // DP rows per thread
 constexpr int JMAX = JT * JR;            // longest row profile
 constexpr int TW = 64;                   // walk tile: TW x TW direction bytes in LDS
+constexpr int KS = 24;                   // row stride of counts and PB (codes padded with zeros)
 constexpr int64_t JNEG = INT64_MIN / 4;  // -inf
 constexpr int64_t WMAX = 8000000;        // nA * nB bound: 24 codes x 11 x nA * nB fits int32
 
 struct JoinJob {
   int32_t la, lb;
   int64_t w;        // nA * nB: gap costs scale
-  int64_t cnt_off;  // la x K: residues of each code in A's columns
-  int64_t pb_off;   // lb x K: PB[j][x] = sum_y S(x, y) cntB[j][y]
+  int64_t cnt_off;  // la x KS: residues of each code in A's columns
+  int64_t pb_off;   // lb x KS: PB[j][x] = sum_y S(x, y) cntB[j][y]
   int64_t m_off;    // match scores M(i, j), diagonal-major: [i + j - 2][i - 1], (la + lb - 1) x la
   int64_t tb_off;   // direction bytes src | e << 2 | f << 3, diagonal-major: [i + j][i], (la + lb + 1) x (la + 1)
   int64_t ops_off;  // la + lb ops, walk order (last column first)
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
   __shared__ uint8_t tile[TW * TW];
   __shared__ int32_t walk[4];        // i, j, state, finished
   const JoinJob J = A.jobs[blockIdx.x];
-  const int la = J.la, lb = J.lb, K = A.K, tid = threadIdx.x;
+  const int la = J.la, lb = J.lb, tid = threadIdx.x;
   const int64_t go = (int64_t)A.open * J.w, ge = (int64_t)A.ext * J.w;
   const int rs = la + 1;
   int64_t *Hr = ring, *Fr = ring + 3 * rs;
@@ -83,13 +85,20 @@ __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
   // threads) read and write consecutive addresses.
   const int64_t ms = la, ts = la + 1;
 
-  // ---- match scores, all cells at once
-  for (int c = tid; c < la * lb; c += JT) {
-    const int i = c / lb, j = c - i * lb;  // 0-based
-    const int32_t *ca = cnt + (int64_t)i * K, *pj = pb + (int64_t)j * K;
-    int32_t m = 0;
-    for (int x = 0; x < K; ++x) m += ca[x] * pj[x];
-    mm[(int64_t)(i + j) * ms + i] = m;
+  // ---- match scores: thread t scores its rows i = t, t + JT, ... (counts
+  // in registers) against every column, whose PB row is the same for the
+  // whole wave (scalar loads); the stores of a column are adjacent.
+  for (int i = tid; i < la; i += JT) {
+    int32_t a[KS];
+#pragma unroll
+    for (int x = 0; x < KS; ++x) a[x] = cnt[(int64_t)i * KS + x];
+    for (int j = 0; j < lb; ++j) {
+      const int32_t *pj = pb + (int64_t)j * KS;
+      int32_t m = 0;
+#pragma unroll
+      for (int x = 0; x < KS; ++x) m += a[x] * pj[x];
+      mm[(int64_t)(i + j) * ms + i] = m;
+    }
   }
   __syncthreads();
 
@@ -227,6 +236,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
   int32_t K = 0;
   int rc = ta_matrix(p->alphabet, p->match, p->mismatch, S, &K);
   if (rc) return rc;
+  if (K > KS) return TA_E_INVALID;
   std::vector<uint8_t> codes((size_t)off[nseq]);
   for (int32_t s = 0; s < nseq; ++s)
     if ((rc = ta_encode(p->alphabet, seqs[s], lens[s], codes.data() + off[s]))) return rc;
@@ -326,21 +336,28 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       // longer row profiles, and joins whose column scores could leave int32
       if (PA.len > JMAX || J.w > WMAX || (int64_t)PA.len * (PA.len + PB.len + 1) > INT32_MAX) return TA_E_INVALID;
       J.cnt_off = (int64_t)hcnt.size();
-      hcnt.resize(hcnt.size() + (size_t)PA.len * K, 0);
+      hcnt.resize(hcnt.size() + (size_t)PA.len * KS, 0);
       int32_t *ca = hcnt.data() + J.cnt_off;
       for (size_t r = 0; r < PA.rows.size(); ++r)
         for (int32_t i = 0; i < PA.len; ++i) {
           const int c = PA.cell[r * PA.len + i];
-          if (c >= 0) ca[(size_t)i * K + c] += 1;
+          if (c >= 0) ca[(size_t)i * KS + c] += 1;
         }
+      // PB from B's counts: K x K per column instead of K per residue
       J.pb_off = (int64_t)hpb.size();
-      hpb.resize(hpb.size() + (size_t)PB.len * K, 0);
+      hpb.resize(hpb.size() + (size_t)PB.len * KS, 0);
       int32_t *pbj = hpb.data() + J.pb_off;
+      std::vector<int32_t> cb((size_t)PB.len * KS, 0);
       for (size_t r = 0; r < PB.rows.size(); ++r)
         for (int32_t j = 0; j < PB.len; ++j) {
           const int c = PB.cell[r * PB.len + j];
-          if (c < 0) continue;
-          for (int32_t x = 0; x < K; ++x) pbj[(size_t)j * K + x] += S[x * K + c];
+          if (c >= 0) cb[(size_t)j * KS + c] += 1;
+        }
+      for (int32_t j = 0; j < PB.len; ++j)
+        for (int32_t y = 0; y < K; ++y) {
+          const int32_t n = cb[(size_t)j * KS + y];
+          if (!n) continue;
+          for (int32_t x = 0; x < K; ++x) pbj[(size_t)j * KS + x] += n * S[x * K + y];
         }
       J.m_off = m_total;
       m_total += (int64_t)(PA.len + PB.len - 1) * PA.len;
@@ -370,7 +387,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
     if (!rc) {
       JoinArgs a;
       a.jobs = d_jobs; a.cnt = d_cnt; a.pb = d_pb; a.mbuf = d_m; a.tb = d_tb; a.ops = d_ops; a.nops = d_nops;
-      a.K = K; a.open = p->gap_open; a.ext = p->gap_extend;
+      a.K = K; a.open = p->gap_open; a.ext = p->gap_extend;  // K <= KS
       (void)hipGetLastError();
       hipLaunchKernelGGL(msa_join_kernel, dim3((unsigned)jobs.size()), dim3(JT), lds, st, a);
       if (hipGetLastError() != hipSuccess ||
