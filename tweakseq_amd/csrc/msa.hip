@@ -326,8 +326,9 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       if (level[nseq + t] == L) ts.push_back(t);
     std::vector<JoinJob> jobs(ts.size());
     std::vector<int32_t> hcnt, hpb;
-    int64_t tb_total = 0, ops_total = 0, m_total = 0;
+    int64_t tb_total = 0, ops_total = 0, m_total = 0, cnt_total = 0, pb_total = 0;
     int max_la = 0;
+    // sizes and offsets, then each join's counts and PB on the host threads
     for (size_t q = 0; q < ts.size(); ++q) {
       const Profile &PA = prof[ja[ts[q]]], &PB = prof[jb[ts[q]]];
       JoinJob &J = jobs[q];
@@ -335,17 +336,30 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       J.w = (int64_t)PA.rows.size() * (int64_t)PB.rows.size();
       // longer row profiles, and joins whose column scores could leave int32
       if (PA.len > JMAX || J.w > WMAX || (int64_t)PA.len * (PA.len + PB.len + 1) > INT32_MAX) return TA_E_INVALID;
-      J.cnt_off = (int64_t)hcnt.size();
-      hcnt.resize(hcnt.size() + (size_t)PA.len * KS, 0);
+      J.cnt_off = cnt_total;
+      cnt_total += (int64_t)PA.len * KS;
+      J.pb_off = pb_total;
+      pb_total += (int64_t)PB.len * KS;
+      J.m_off = m_total;
+      m_total += (int64_t)(PA.len + PB.len - 1) * PA.len;
+      J.tb_off = tb_total;
+      tb_total += (int64_t)(PA.len + PB.len + 1) * (PA.len + 1);
+      J.ops_off = ops_total;
+      ops_total += PA.len + PB.len;
+      max_la = std::max(max_la, PA.len);
+    }
+    hcnt.assign((size_t)cnt_total, 0);
+    hpb.assign((size_t)pb_total, 0);
+    parallel_for(ts.size(), [&](size_t q) {
+      const Profile &PA = prof[ja[ts[q]]], &PB = prof[jb[ts[q]]];
+      const JoinJob &J = jobs[q];
       int32_t *ca = hcnt.data() + J.cnt_off;
       for (size_t r = 0; r < PA.rows.size(); ++r)
         for (int32_t i = 0; i < PA.len; ++i) {
           const int c = PA.cell[r * PA.len + i];
           if (c >= 0) ca[(size_t)i * KS + c] += 1;
         }
-      // PB from B's counts: K x K per column instead of K per residue
-      J.pb_off = (int64_t)hpb.size();
-      hpb.resize(hpb.size() + (size_t)PB.len * KS, 0);
+      // PB from B's column counts: K x K per column instead of K per residue
       int32_t *pbj = hpb.data() + J.pb_off;
       std::vector<int32_t> cb((size_t)PB.len * KS, 0);
       for (size_t r = 0; r < PB.rows.size(); ++r)
@@ -359,14 +373,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
           if (!n) continue;
           for (int32_t x = 0; x < K; ++x) pbj[(size_t)j * KS + x] += n * S[x * K + y];
         }
-      J.m_off = m_total;
-      m_total += (int64_t)(PA.len + PB.len - 1) * PA.len;
-      J.tb_off = tb_total;
-      tb_total += (int64_t)(PA.len + PB.len + 1) * (PA.len + 1);
-      J.ops_off = ops_total;
-      ops_total += PA.len + PB.len;
-      max_la = std::max(max_la, PA.len);
-    }
+    });
     const size_t lds = sizeof(int64_t) * 6 * (size_t)(max_la + 1);
     JoinJob *d_jobs = (JoinJob *)device_alloc(sizeof(JoinJob) * jobs.size());
     int32_t *d_cnt = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hcnt.size()));

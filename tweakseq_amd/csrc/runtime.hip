@@ -384,24 +384,6 @@ int fill_waves_per_cu(int P, int Kp, bool nib, int want, bool walk = false) {
 // by groups of GG lanes.
 // n_force / S_force (wave units): columns and strips of the task's unit; extra
 // columns lead (local couples only), extra strips are pad rows.
-// Runs f(i) for i in [0, n) on up to 16 host threads in contiguous chunks
-// (f must not depend on the order).
-template <class F>
-void parallel_for(size_t n, F f) {
-  const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  const size_t nt = std::min<size_t>(hw, (n + 255) / 256);
-  if (nt <= 1) {
-    for (size_t i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::vector<std::thread> th;
-  th.reserve(nt);
-  for (size_t t = 0; t < nt; ++t)
-    th.emplace_back([&, t] {
-      for (size_t i = n * t / nt; i < n * (t + 1) / nt; ++i) f(i);
-    });
-  for (auto &x : th) x.join();
-}
 
 // The row and column codes of every planned task, expanded on the GPU from
 // the uploaded residues: row r of a half is its pair's query residue r (pad

@@ -2,10 +2,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
-#include <string>
-#include <vector>
 #include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/tweakalign.h"
 
@@ -41,5 +43,24 @@ int batch_results_text(ta_ctx *ctx, ta_batch *B, std::vector<ta_result> &out, st
 // hipFree while work runs; device_free returns them to the cache).
 void *device_alloc(size_t bytes);
 void device_free(void *p);
+// Runs f(i) for i in [0, n) on up to 16 host threads in contiguous chunks
+// (f must not depend on the order).
+template <class F>
+void parallel_for(size_t n, F f) {
+  const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t nt = std::min<size_t>(hw, (n + 255) / 256);
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (size_t t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      for (size_t i = n * t / nt; i < n * (t + 1) / nt; ++i) f(i);
+    });
+  for (auto &x : th) x.join();
+}
+
 // Stops the serving thread (called by ta_destroy).
 void serve_shutdown(ta_ctx *ctx);
