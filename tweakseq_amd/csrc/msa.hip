@@ -22,7 +22,7 @@
 // A join first computes every column pair's match score (int32: nA * nB is
 // bounded so that 24 codes x 11 x nA * nB fits; a thread's rows' counts sit
 // in registers, each column's PB row is a wave-uniform scalar load), then sweeps
-// anti-diagonals: thread t owns rows t+1, t+1+256, ... (up to 8 each, so
+// anti-diagonals: thread t owns rows t+1, t+1+256, ... (2, 4 or 8 each, so
 // profiles of up to 2,048 columns take the DP rows), keeps H and E of its
 // rows' last cells in registers, and reads the row above (H, F at the
 // previous diagonal, H at the one before) from a three-slot LDS ring of int64
@@ -42,8 +42,8 @@
 namespace {
 
 constexpr int JT = 256;                  // threads per join
-constexpr int JR = 8;                    // DP rows per thread
-constexpr int JMAX = JT * JR;            // longest row profile
+constexpr int JRMAX = 8;                 // DP rows per thread at most
+constexpr int JMAX = JT * JRMAX;         // longest row profile
 constexpr int TW = 64;                   // walk tile: TW x TW direction bytes in LDS
 constexpr int KS = 24;                   // row stride of counts and PB (codes padded with zeros)
 constexpr int64_t JNEG = INT64_MIN / 4;  // -inf
@@ -69,6 +69,9 @@ struct JoinArgs {
   int32_t K, open, ext;
 };
 
+// JR: DP rows per thread, the fewest that cover the level's longest row
+// profile (every unused row slot costs its checks on every diagonal).
+template <int JR>
 __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
   extern __shared__ int64_t ring[];  // H[3][la + 1], then F[3][la + 1]
   __shared__ uint8_t tile[TW * TW];
@@ -396,7 +399,10 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       a.jobs = d_jobs; a.cnt = d_cnt; a.pb = d_pb; a.mbuf = d_m; a.tb = d_tb; a.ops = d_ops; a.nops = d_nops;
       a.K = K; a.open = p->gap_open; a.ext = p->gap_extend;  // K <= KS
       (void)hipGetLastError();
-      hipLaunchKernelGGL(msa_join_kernel, dim3((unsigned)jobs.size()), dim3(JT), lds, st, a);
+      void (*kern)(JoinArgs) = max_la <= 2 * JT   ? msa_join_kernel<2>
+                               : max_la <= 4 * JT ? msa_join_kernel<4>
+                                                  : msa_join_kernel<JRMAX>;
+      hipLaunchKernelGGL(kern, dim3((unsigned)jobs.size()), dim3(JT), lds, st, a);
       if (hipGetLastError() != hipSuccess ||
           hipMemcpyAsync(hops.data(), d_ops, (size_t)ops_total, hipMemcpyDeviceToHost, st) ||
           hipMemcpyAsync(hnops.data(), d_nops, sizeof(int32_t) * jobs.size(), hipMemcpyDeviceToHost, st) ||
