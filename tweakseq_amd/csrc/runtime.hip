@@ -715,17 +715,25 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   // code array, then their codes and bounds on the host threads.
   struct Uniq { const char *s; int32_t len; uint64_t off; int64_t bound; };
   std::vector<Uniq> uq;
-  std::unordered_map<const char *, int32_t> seen;
-  seen.reserve(2 * (size_t)n);
+  // pointer -> latest unique entry: open addressing, linear probing, at
+  // most half full (a node-based map costs more than the encoding here)
+  size_t hcap = 16;
+  while (hcap < 4 * (size_t)n) hcap <<= 1;
+  std::vector<const char *> hkey(hcap);
+  std::vector<int32_t> hval(hcap, -1);
   std::vector<int32_t> ua((size_t)n), ub((size_t)n);
   size_t ctotal = 0;
   auto uniq = [&](const char *s, int32_t len) -> int32_t {
-    auto it = seen.find(s);
-    if (it != seen.end() && uq[it->second].len == len) return it->second;
+    uint64_t x = (uint64_t)(uintptr_t)s;
+    x = (x ^ (x >> 29)) * 0xbf58476d1ce4e5b9ull;
+    size_t h = (size_t)(x ^ (x >> 32)) & (hcap - 1);
+    while (hval[h] >= 0 && hkey[h] != s) h = (h + 1) & (hcap - 1);
+    if (hval[h] >= 0 && uq[hval[h]].len == len) return hval[h];
     const size_t at = (ctotal + 15) & ~(size_t)15;  // 16-byte aligned starts (walker windows)
     uq.push_back(Uniq{s, len, at, 0});
     ctotal = at + (size_t)len;
-    seen[s] = (int32_t)uq.size() - 1;
+    hkey[h] = s;
+    hval[h] = (int32_t)uq.size() - 1;
     return (int32_t)uq.size() - 1;
   };
   for (int32_t k = 0; k < n; ++k) {
