@@ -1447,24 +1447,23 @@ int batch_results_text(ta_ctx *ctx, ta_batch *B, std::vector<ta_result> &out, st
   std::vector<int64_t> off;
   const int rc = fetch_results(B, res, runs, off);
   if (rc) return rc;
+  // lengths, offsets, then the text, as ta_batch_results
   out.resize(B->n);
-  text.clear();
-  text.reserve(runs.size() * 3);
-  for (int32_t k = 0; k < B->n; ++k) {
+  const bool tb = B->p.traceback != 0;
+  parallel_for((size_t)B->n, [&](size_t k) {
     const ResultDev &r = res[k];
-    ta_result &o = out[k];
-    fill_result(o, r);
-    o.cigar_off = (int64_t)text.size();
-    if (B->p.traceback && r.cigar_len > 0) {
-      const uint32_t *rk = runs.data() + off[k] / 4;
-      const int nr = r.cigar_len / 4;
-      const int64_t need = format_runs(rk, nr, nullptr);
-      const size_t at = text.size();
-      text.resize(at + (size_t)need);
-      format_runs(rk, nr, &text[at]);
-      o.cigar_len = (int32_t)need;
-    }
+    fill_result(out[k], r);
+    if (tb && r.cigar_len > 0) out[k].cigar_len = (int32_t)format_runs(runs.data() + off[k] / 4, r.cigar_len / 4, nullptr);
+  });
+  int64_t pos = 0;
+  for (int32_t k = 0; k < B->n; ++k) {
+    out[k].cigar_off = pos;
+    pos += out[k].cigar_len;
   }
+  text.assign((size_t)pos, '\0');
+  parallel_for((size_t)B->n, [&](size_t k) {
+    if (out[k].cigar_len > 0) format_runs(runs.data() + off[k] / 4, res[k].cigar_len / 4, &text[out[k].cigar_off]);
+  });
   return TA_OK;
 }
 
