@@ -171,3 +171,32 @@ def test_workload_local_batch_prefix(eng):
     w = ta.Workload(3, size=1500)
     pairs = [w.pair_bytes(k) for k in range(w.npairs)]
     check(eng, pairs, "65,536-pair local batch, first 1,500", mode=ta.LOCAL, gap_open=-11, gap_extend=-1)
+
+
+@pytest.mark.parametrize("mode", [ta.GLOBAL, ta.LOCAL])
+def test_shared_buffers_and_prefixes(eng, mode):
+    """Pairs whose sequences point into the same buffers, at the same and at
+    different lengths (prefixes): the batch encodes a (pointer, length) once
+    and every prefix separately; results equal the oracle's on the strings."""
+    import ctypes
+    rng = random.Random(31 + mode)
+    A = rand_prot(rng, 400)
+    B = mutate(rng, A, 0.2, 0.04)
+    ba = ctypes.create_string_buffer(A.encode(), len(A) + 1)
+    bb = ctypes.create_string_buffer(B.encode(), len(B) + 1)
+    lens = [(400, len(B)), (400, len(B)), (250, 300), (250, 300), (17, 400 if len(B) >= 400 else len(B)),
+            (400, 60), (1, 1), (399, len(B) - 1), (250, 300)]
+    ps = ta.engine._Pairs([])
+    ps.arr = (ta.engine.Pair * len(lens))()
+    ps.n = len(lens)
+    ps.keep = [ba, bb]
+    strs = []
+    for k, (la, lb) in enumerate(lens):
+        ps.arr[k].a = ctypes.cast(ba, ctypes.c_void_p)
+        ps.arr[k].la = la
+        ps.arr[k].b = ctypes.cast(bb, ctypes.c_void_p)
+        ps.arr[k].lb = lb
+        strs.append((A[:la], B[:lb]))
+    params = ta.engine.make_params(mode=mode, gap_open=-11 if mode else -10)
+    got = eng.align(ps, params)
+    compare(got, oracle_results(params, strs), params, strs, "shared")
