@@ -192,8 +192,8 @@ int ta_consensus(ta_ctx *ctx, const char *const *rows, int32_t nseq, int32_t len
  * *alen and the column of residue k of sequence s to cols[coff_s + k]
  * (coff_s = lens[0] + ... + lens[s-1]); joins (may be NULL) gets the two
  * clusters of each of the nseq - 1 joins (leaves 0..nseq-1, join t makes
- * cluster nseq + t).  Profiles taking the DP rows (the lower cluster id of a
- * join) are limited to 2,048 columns (TA_E_INVALID beyond). */
+ * cluster nseq + t).  A join's nA * nB is limited to 8,000,000
+ * (TA_E_INVALID beyond). */
 int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32_t *lens, int32_t nseq,
            int32_t *alen, int32_t *cols, int32_t *joins);
 

@@ -74,6 +74,25 @@ def test_dna_and_gap_costs(eng):
     assert joins == wjoins and rows == want
 
 
+@pytest.mark.parametrize("alphabet", [0, 1])
+def test_long_profiles_in_strips(eng, alphabet):
+    """Profiles longer than one strip of DP rows (2,048): the join sweeps the
+    rows in strips, the next reading the last row of the one before."""
+    rng = random.Random(12 + alphabet)
+    letters = AA if alphabet == 0 else "ACGT"
+    seqs = family(rng, 4, 2300, psub=0.15, pindel=0.04, alphabet=letters)
+    seqs.append("".join(rng.choice(letters) for _ in range(1500)))
+    if alphabet == 0:
+        rows, joins = eng.msa(seqs)
+        want, wjoins = oracle_rows(seqs)
+    else:
+        p = ta.engine.make_params(mode=ta.GLOBAL, alphabet=ta.DNA, gap_open=-5, gap_extend=-2)
+        rows, joins = eng.msa(seqs, p)
+        want, wjoins = oracle_rows(seqs, alphabet=1, go=-5, ge=-2)
+    assert max(len(s) for s in seqs) > 2048
+    assert joins == wjoins and rows == want
+
+
 def test_two_sequences_are_the_pairwise_alignment(eng):
     rng = random.Random(11)
     a, b = family(rng, 2, 200)

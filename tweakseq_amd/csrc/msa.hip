@@ -22,8 +22,9 @@
 // A join first computes every column pair's match score (int32: nA * nB is
 // bounded so that 24 codes x 11 x nA * nB fits; a thread's rows' counts sit
 // in registers, each column's PB row is a wave-uniform scalar load), then sweeps
-// anti-diagonals: thread t owns rows t+1, t+1+256, ... (2, 4 or 8 each, so
-// profiles of up to 2,048 columns take the DP rows), keeps H and E of its
+// anti-diagonals: thread t owns rows t+1, t+1+256, ... (2, 4 or 8 each;
+// longer row profiles go in strips of 2,048 rows, each reading the last row
+// of the one before from global memory), keeps H and E of its
 // rows' last cells in registers, and reads the row above (H, F at the
 // previous diagonal, H at the one before) from a three-slot LDS ring of int64
 // values, one barrier per diagonal.  The walk back runs on thread 0 over
@@ -42,8 +43,7 @@
 namespace {
 
 constexpr int JT = 256;                  // threads per join
-constexpr int JRMAX = 8;                 // DP rows per thread at most
-constexpr int JMAX = JT * JRMAX;         // longest row profile
+constexpr int JRMAX = 8;                 // DP rows per thread at most (a strip: JT * JR rows)
 constexpr int TW = 64;                   // walk tile: TW x TW direction bytes in LDS
 constexpr int KS = 24;                   // row stride of counts and PB (codes padded with zeros)
 constexpr int64_t JNEG = INT64_MIN / 4;  // -inf
@@ -57,12 +57,14 @@ struct JoinJob {
   int64_t m_off;    // match scores M(i, j), diagonal-major: [i + j - 2][i - 1], (la + lb - 1) x la
   int64_t tb_off;   // direction bytes src | e << 2 | f << 3, diagonal-major: [i + j][i], (la + lb + 1) x (la + 1)
   int64_t ops_off;  // la + lb ops, walk order (last column first)
+  int64_t hb_off;   // 4 x (lb + 1) int64: a strip's last row (H, F), two strips' worth
 };
 
 struct JoinArgs {
   const JoinJob *jobs;
   const int32_t *cnt, *pb;
   int32_t *mbuf;
+  int64_t *hbot;
   uint8_t *tb;
   uint8_t *ops;
   int32_t *nops;
@@ -79,7 +81,8 @@ __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
   const JoinJob J = A.jobs[blockIdx.x];
   const int la = J.la, lb = J.lb, tid = threadIdx.x;
   const int64_t go = (int64_t)A.open * J.w, ge = (int64_t)A.ext * J.w;
-  const int rs = la + 1;
+  constexpr int RS = JT * JR;  // DP rows per strip
+  const int rs = (la < RS ? la : RS) + 1;
   int64_t *Hr = ring, *Fr = ring + 3 * rs;
   const int32_t *cnt = A.cnt + J.cnt_off, *pb = A.pb + J.pb_off;
   int32_t *mm = A.mbuf + J.m_off;
@@ -105,71 +108,87 @@ __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
   }
   __syncthreads();
 
-  // ---- the DP, one anti-diagonal d = i + j per barrier
-  int64_t Hl[JR], El[JR];  // H(i, j-1), E(i, j-1) of each owned row
-  int32_t mn[JR];          // match scores of the next diagonal's cells, loaded a diagonal early
-#pragma unroll
-  for (int r = 0; r < JR; ++r) {
-    const int i = tid + 1 + r * JT;
-    Hl[r] = go + (int64_t)(i - 1) * ge;  // H(i, 0)
-    El[r] = JNEG;
-    mn[r] = (i <= la && 2 - i >= 1) ? mm[(int64_t)0 * ms + (i - 1)] : 0;
-  }
-  for (int d = 2; d <= la + lb; ++d) {
-    const int cur = d % 3, prv = (d + 2) % 3, pp = (d + 1) % 3;  // diagonals d, d-1, d-2
-    int32_t mc[JR];
+  // ---- the DP in strips of RS rows (one, unless the rows outnumber the
+  // threads' registers), one anti-diagonal d = i + j per barrier.  A strip's
+  // first row reads the previous strip's last row from global memory (two
+  // buffers by strip parity: a one-row strip reads and writes the same row).
+  int64_t *hb = A.hbot + J.hb_off;
+  for (int row0 = 0, sp = 0; row0 < la; row0 += RS, sp ^= 1) {
+    const int rend = la < row0 + RS ? la : row0 + RS;  // rows row0 + 1 .. rend
+    const int64_t *Hin = hb + (int64_t)(sp ^ 1) * 2 * (lb + 1), *Fin = Hin + (lb + 1);
+    int64_t *Hout = hb + (int64_t)sp * 2 * (lb + 1), *Fout = Hout + (lb + 1);
+    int64_t Hl[JR], El[JR];  // H(i, j-1), E(i, j-1) of each owned row
+    int32_t mn[JR];          // match scores of the next diagonal's cells, loaded a diagonal early
 #pragma unroll
     for (int r = 0; r < JR; ++r) {
-      const int i = tid + 1 + r * JT, j1 = d + 1 - i;
-      mc[r] = mn[r];
-      mn[r] = (i <= la && j1 >= 1 && j1 <= lb) ? mm[(int64_t)(d - 1) * ms + (i - 1)] : 0;
+      const int i = row0 + tid + 1 + r * JT;
+      Hl[r] = go + (int64_t)(i - 1) * ge;  // H(i, 0)
+      El[r] = JNEG;
+      mn[r] = (i <= rend && row0 + 2 - i >= 1) ? mm[(int64_t)row0 * ms + (i - 1)] : 0;
     }
-    // Loads of every owned row first (all in flight together), then the
-    // cells: a row's ring stores would otherwise order the next row's loads.
-    int64_t Hup[JR], Fup[JR], Hdg[JR];
+    for (int d = row0 + 2; d <= rend + lb; ++d) {
+      const int cur = d % 3, prv = (d + 2) % 3, pp = (d + 1) % 3;  // diagonals d, d-1, d-2
+      int32_t mc[JR];
 #pragma unroll
-    for (int r = 0; r < JR; ++r) {
-      const int i = tid + 1 + r * JT;
-      const int j = d - i;
-      Hup[r] = Fup[r] = Hdg[r] = 0;
-      if (i > la || j < 1 || j > lb) continue;
-      if (i == 1) {  // row 0 is the boundary
-        Hup[r] = go + (int64_t)(j - 1) * ge;
-        Fup[r] = JNEG;
-        Hdg[r] = j == 1 ? 0 : go + (int64_t)(j - 2) * ge;
-      } else {
-        Hup[r] = Hr[prv * rs + i - 1];
-        Fup[r] = Fr[prv * rs + i - 1];
-        Hdg[r] = j == 1 ? go + (int64_t)(i - 2) * ge : Hr[pp * rs + i - 1];
+      for (int r = 0; r < JR; ++r) {
+        const int i = row0 + tid + 1 + r * JT, j1 = d + 1 - i;
+        mc[r] = mn[r];
+        mn[r] = (i <= rend && j1 >= 1 && j1 <= lb) ? mm[(int64_t)(d - 1) * ms + (i - 1)] : 0;
       }
-    }
+      // Loads of every owned row first (all in flight together), then the
+      // cells: a row's ring stores would otherwise order the next row's loads.
+      int64_t Hup[JR], Fup[JR], Hdg[JR];
 #pragma unroll
-    for (int r = 0; r < JR; ++r) {
-      const int i = tid + 1 + r * JT;
-      const int j = d - i;
-      if (i > la || j < 1 || j > lb) continue;
-      const int64_t eo = Hl[r] + go, ee = El[r] + ge;
-      const bool eb = ee > eo;
-      const int64_t E = eb ? ee : eo;
-      const int64_t fo = Hup[r] + go, fe = Fup[r] + ge;
-      const bool fb = fe > fo;
-      const int64_t F = fb ? fe : fo;
-      int64_t h = Hdg[r] + mc[r];
-      int src = 0;
-      if (E > h) { h = E; src = 1; }
-      if (F > h) { h = F; src = 2; }
-      Hr[cur * rs + i] = h;
-      Fr[cur * rs + i] = F;
-      Hl[r] = h;
-      El[r] = E;
-      tb[(int64_t)d * ts + i] = (uint8_t)(src | ((int)eb << 2) | ((int)fb << 3));
+      for (int r = 0; r < JR; ++r) {
+        const int li = tid + 1 + r * JT, i = row0 + li;  // row in the strip, in the matrix
+        const int j = d - i;
+        Hup[r] = Fup[r] = Hdg[r] = 0;
+        if (i > rend || j < 1 || j > lb) continue;
+        if (i == 1) {  // row 0 is the boundary
+          Hup[r] = go + (int64_t)(j - 1) * ge;
+          Fup[r] = JNEG;
+          Hdg[r] = j == 1 ? 0 : go + (int64_t)(j - 2) * ge;
+        } else if (li == 1) {  // the previous strip's last row
+          Hup[r] = Hin[j];
+          Fup[r] = Fin[j];
+          Hdg[r] = j == 1 ? go + (int64_t)(i - 2) * ge : Hin[j - 1];
+        } else {
+          Hup[r] = Hr[prv * rs + li - 1];
+          Fup[r] = Fr[prv * rs + li - 1];
+          Hdg[r] = j == 1 ? go + (int64_t)(i - 2) * ge : Hr[pp * rs + li - 1];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < JR; ++r) {
+        const int li = tid + 1 + r * JT, i = row0 + li;
+        const int j = d - i;
+        if (i > rend || j < 1 || j > lb) continue;
+        const int64_t eo = Hl[r] + go, ee = El[r] + ge;
+        const bool eb = ee > eo;
+        const int64_t E = eb ? ee : eo;
+        const int64_t fo = Hup[r] + go, fe = Fup[r] + ge;
+        const bool fb = fe > fo;
+        const int64_t F = fb ? fe : fo;
+        int64_t h = Hdg[r] + mc[r];
+        int src = 0;
+        if (E > h) { h = E; src = 1; }
+        if (F > h) { h = F; src = 2; }
+        Hr[cur * rs + li] = h;
+        Fr[cur * rs + li] = F;
+        if (i == rend) { Hout[j] = h; Fout[j] = F; }
+        Hl[r] = h;
+        El[r] = E;
+        tb[(int64_t)d * ts + i] = (uint8_t)(src | ((int)eb << 2) | ((int)fb << 3));
+      }
+      // The ring is the only memory shared within a diagonal: an LDS-only
+      // barrier leaves the next diagonal's score loads in flight across it
+      // (the direction bytes and a strip's last row are ordered by the full
+      // barriers after the strip).
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     }
-    // The ring is the only memory shared within a diagonal: an LDS-only
-    // barrier leaves the next diagonal's score loads in flight across it
-    // (the direction bytes are ordered by the full barrier before the walk).
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    __syncthreads();
   }
 
   // ---- walk back from (la, lb): the pairwise state machine (H, E 'D',
@@ -329,7 +348,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       if (level[nseq + t] == L) ts.push_back(t);
     std::vector<JoinJob> jobs(ts.size());
     std::vector<int32_t> hcnt, hpb;
-    int64_t tb_total = 0, ops_total = 0, m_total = 0, cnt_total = 0, pb_total = 0;
+    int64_t tb_total = 0, ops_total = 0, m_total = 0, cnt_total = 0, pb_total = 0, hb_total = 0;
     int max_la = 0;
     // sizes and offsets, then each join's counts and PB on the host threads
     for (size_t q = 0; q < ts.size(); ++q) {
@@ -337,8 +356,8 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       JoinJob &J = jobs[q];
       J.la = PA.len; J.lb = PB.len;
       J.w = (int64_t)PA.rows.size() * (int64_t)PB.rows.size();
-      // longer row profiles, and joins whose column scores could leave int32
-      if (PA.len > JMAX || J.w > WMAX || (int64_t)PA.len * (PA.len + PB.len + 1) > INT32_MAX) return TA_E_INVALID;
+      // joins whose column scores could leave int32
+      if (J.w > WMAX || (int64_t)PA.len * (PA.len + PB.len + 1) > INT32_MAX) return TA_E_INVALID;
       J.cnt_off = cnt_total;
       cnt_total += (int64_t)PA.len * KS;
       J.pb_off = pb_total;
@@ -349,6 +368,8 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       tb_total += (int64_t)(PA.len + PB.len + 1) * (PA.len + 1);
       J.ops_off = ops_total;
       ops_total += PA.len + PB.len;
+      J.hb_off = hb_total;
+      hb_total += 4 * (int64_t)(PB.len + 1);
       max_la = std::max(max_la, PA.len);
     }
     hcnt.assign((size_t)cnt_total, 0);
@@ -377,17 +398,19 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
           for (int32_t x = 0; x < K; ++x) pbj[(size_t)j * KS + x] += n * S[x * K + y];
         }
     });
-    const size_t lds = sizeof(int64_t) * 6 * (size_t)(max_la + 1);
+    const int jr = max_la <= 2 * JT ? 2 : max_la <= 4 * JT ? 4 : JRMAX;
+    const size_t lds = sizeof(int64_t) * 6 * (size_t)(std::min(max_la, JT * jr) + 1);
     JoinJob *d_jobs = (JoinJob *)device_alloc(sizeof(JoinJob) * jobs.size());
     int32_t *d_cnt = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hcnt.size()));
     int32_t *d_pb = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hpb.size()));
     int32_t *d_m = (int32_t *)device_alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(1, m_total));
+    int64_t *d_hb = (int64_t *)device_alloc(sizeof(int64_t) * (size_t)std::max<int64_t>(1, hb_total));
     uint8_t *d_tb = (uint8_t *)device_alloc((size_t)std::max<int64_t>(1, tb_total));
     uint8_t *d_ops = (uint8_t *)device_alloc((size_t)std::max<int64_t>(1, ops_total));
     int32_t *d_nops = (int32_t *)device_alloc(sizeof(int32_t) * jobs.size());
     std::vector<uint8_t> hops((size_t)std::max<int64_t>(1, ops_total));
     std::vector<int32_t> hnops(jobs.size());
-    rc = (d_jobs && d_cnt && d_pb && d_m && d_tb && d_ops && d_nops) ? TA_OK : TA_E_NOMEM;
+    rc = (d_jobs && d_cnt && d_pb && d_m && d_hb && d_tb && d_ops && d_nops) ? TA_OK : TA_E_NOMEM;
     if (!rc && (hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JoinJob) * jobs.size(), hipMemcpyHostToDevice, st) ||
                 (!hcnt.empty() &&
                  hipMemcpyAsync(d_cnt, hcnt.data(), sizeof(int32_t) * hcnt.size(), hipMemcpyHostToDevice, st)) ||
@@ -396,12 +419,11 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       rc = TA_E_DEVICE;
     if (!rc) {
       JoinArgs a;
-      a.jobs = d_jobs; a.cnt = d_cnt; a.pb = d_pb; a.mbuf = d_m; a.tb = d_tb; a.ops = d_ops; a.nops = d_nops;
+      a.jobs = d_jobs; a.cnt = d_cnt; a.pb = d_pb; a.mbuf = d_m; a.hbot = d_hb; a.tb = d_tb; a.ops = d_ops;
+      a.nops = d_nops;
       a.K = K; a.open = p->gap_open; a.ext = p->gap_extend;  // K <= KS
       (void)hipGetLastError();
-      void (*kern)(JoinArgs) = max_la <= 2 * JT   ? msa_join_kernel<2>
-                               : max_la <= 4 * JT ? msa_join_kernel<4>
-                                                  : msa_join_kernel<JRMAX>;
+      void (*kern)(JoinArgs) = jr == 2 ? msa_join_kernel<2> : jr == 4 ? msa_join_kernel<4> : msa_join_kernel<JRMAX>;
       hipLaunchKernelGGL(kern, dim3((unsigned)jobs.size()), dim3(JT), lds, st, a);
       if (hipGetLastError() != hipSuccess ||
           hipMemcpyAsync(hops.data(), d_ops, (size_t)ops_total, hipMemcpyDeviceToHost, st) ||
@@ -409,7 +431,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
           hipStreamSynchronize(st))
         rc = TA_E_DEVICE;
     }
-    device_free(d_jobs); device_free(d_cnt); device_free(d_pb); device_free(d_m);
+    device_free(d_jobs); device_free(d_cnt); device_free(d_pb); device_free(d_m); device_free(d_hb);
     device_free(d_tb); device_free(d_ops); device_free(d_nops);
     if (rc) return rc;
     // apply each join's ops (walk order: read backwards)
