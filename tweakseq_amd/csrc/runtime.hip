@@ -234,13 +234,6 @@ int dalloc(T **p, size_t n) {
   return *p ? TA_OK : TA_E_NOMEM;
 }
 
-// A byte that std::vector does not zero on resize (every byte of such a
-// buffer is written explicitly).
-struct RawByte {
-  uint8_t v;
-  RawByte() {}
-};
-
 template <typename T>
 int upload(T **dst, const std::vector<T> &src, hipStream_t st) {
   int rc = dalloc(dst, src.size());
@@ -254,6 +247,86 @@ void dfree(T *&p) {
   cache_free((void *)p);
   p = nullptr;
 }
+
+// Host staging for the residue codes.  A copy from pageable memory runs at
+// a fraction of the link's rate (the runtime copies it through bounce
+// buffers); from pinned memory the DMA engine reads the block itself.
+// Pinning a block takes milliseconds, so blocks are kept and reused: batch
+// creation takes one, encodes into it, and gives it back once its copy has
+// finished.  At most 4 free blocks stay pinned; blocks above 256 MB come
+// from the heap (pinning them would cost about what it saves).
+struct Staging {
+  uint8_t *p = nullptr;
+  size_t cap = 0;
+  bool pinned = false;
+};
+struct StagePool {
+  std::mutex mu;
+  std::vector<Staging> free;
+};
+StagePool &stage_pool() {
+  static StagePool *s = new StagePool();  // lives until the process ends
+  return *s;
+}
+
+Staging stage_take(size_t bytes) {
+  StagePool &sp = stage_pool();
+  {
+    std::lock_guard<std::mutex> g(sp.mu);
+    int best = -1;  // the smallest free block that fits
+    for (int i = 0; i < (int)sp.free.size(); ++i)
+      if (sp.free[i].cap >= bytes && (best < 0 || sp.free[i].cap < sp.free[best].cap)) best = i;
+    if (best >= 0) {
+      const Staging s = sp.free[best];
+      sp.free.erase(sp.free.begin() + best);
+      return s;
+    }
+  }
+  Staging s;
+  s.cap = size_class(std::max(bytes, (size_t)1 << 20));
+  if (s.cap <= ((size_t)256 << 20)) {
+    if (hipHostMalloc((void **)&s.p, s.cap, hipHostMallocPortable) == hipSuccess) {
+      s.pinned = true;
+      return s;
+    }
+    (void)hipGetLastError();
+    s.p = nullptr;
+  }
+  s.p = (uint8_t *)malloc(s.cap);
+  if (!s.p) s.cap = 0;
+  return s;
+}
+
+void stage_give(Staging &s) {
+  if (!s.p) return;
+  if (!s.pinned) {
+    free(s.p);
+  } else {
+    StagePool &sp = stage_pool();
+    std::lock_guard<std::mutex> g(sp.mu);
+    sp.free.push_back(s);
+    if (sp.free.size() > 4) {  // unpin the smallest
+      auto it = std::min_element(sp.free.begin(), sp.free.end(),
+                                 [](const Staging &x, const Staging &y) { return x.cap < y.cap; });
+      (void)hipHostFree(it->p);
+      sp.free.erase(it);
+    }
+  }
+  s = Staging{};
+}
+
+// A staging block held by one batch creation.  Once a copy from it is
+// queued on `stream`, the block goes back only after the stream has drained
+// (every return path, device errors included).
+struct StageHold {
+  Staging s;
+  hipStream_t stream = nullptr;
+  bool queued = false;
+  ~StageHold() {
+    if (queued) (void)hipStreamSynchronize(stream);
+    stage_give(s);
+  }
+};
 
 
 }  // namespace
@@ -460,6 +533,47 @@ void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const
   dst.push_back(T);
 }
 
+// Positions 0..n-1 in ascending order of key, equal keys in position order:
+// an LSD radix sort over 11-bit digits, as many as the largest key needs.
+// The planner orders tens of thousands of pairs and units per batch, where
+// comparison sorts that read their keys through the pair table cost
+// milliseconds.
+std::vector<int32_t> order_by_key(std::vector<uint64_t> key) {
+  const size_t n = key.size();
+  std::vector<int32_t> idx(n), ti(n);
+  std::vector<uint64_t> tk(n);
+  for (size_t t = 0; t < n; ++t) idx[t] = (int32_t)t;
+  uint64_t any = 0;
+  for (uint64_t k : key) any |= k;
+  for (int sh = 0; sh < 64 && (any >> sh) != 0; sh += 11) {
+    std::vector<size_t> at(2049, 0);
+    for (uint64_t k : key) ++at[((k >> sh) & 2047) + 1];
+    for (int d = 0; d < 2048; ++d) at[d + 1] += at[d];
+    for (size_t t = 0; t < n; ++t) {
+      const size_t q = at[(key[t] >> sh) & 2047]++;
+      ti[q] = idx[t];
+      tk[q] = key[t];
+    }
+    idx.swap(ti);
+    key.swap(tk);
+  }
+  return idx;
+}
+
+// `v` reordered by ascending key[v's position].
+template <typename T>
+void gather_by_key(std::vector<T> &v, std::vector<uint64_t> key) {
+  const std::vector<int32_t> o = order_by_key(std::move(key));
+  std::vector<T> w;
+  w.reserve(v.size());
+  for (int32_t t : o) w.push_back(std::move(v[t]));
+  v.swap(w);
+}
+
+// Lengths are at most 1,000,000 (< 2^20), strips of STRIP rows at most 2^13.
+constexpr uint64_t LEN_TOP = (1u << 20) - 1, STRIPS_TOP = (1u << 13) - 1;
+static_assert((1000000 + STRIP - 1) / STRIP <= (int)STRIPS_TOP, "strip count exceeds its key field");
+
 // A couple (or single) waiting to be grouped into a wave unit of G-lane tasks.
 struct Unit {
   int lo, hi;
@@ -477,11 +591,14 @@ struct Unit {
 void add_units(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, bool local, bool trace,
                const std::vector<HostPair> &hp, int Kp) {
   auto strips = [](int m) { return ceil_div(std::max(m, 1), G * R); };
-  std::stable_sort(u.begin(), u.end(), [&](const Unit &x, const Unit &y) {
-    if (!local && x.n != y.n) return x.n > y.n;
-    if (strips(x.m) != strips(y.m)) return strips(x.m) > strips(y.m);
-    return x.n > y.n;
-  });
+  {  // local: more strips first, then more columns; global: columns first
+    std::vector<uint64_t> key(u.size());
+    for (size_t t = 0; t < u.size(); ++t) {
+      const uint64_t sk = STRIPS_TOP - (uint64_t)strips(u[t].m), nk = LEN_TOP - (uint64_t)u[t].n;
+      key[t] = local ? (sk << 20 | nk) : (nk << 13 | sk);
+    }
+    gather_by_key(u, std::move(key));
+  }
   std::vector<std::vector<Task>> units;
   for (size_t k = 0; k < u.size();) {
     size_t e = k;
@@ -499,9 +616,11 @@ void add_units(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, boo
     units.push_back(std::move(q));
     k = e;
   }
-  std::stable_sort(units.begin(), units.end(), [](const std::vector<Task> &x, const std::vector<Task> &y) {
-    return (double)x[0].S * x[0].NB > (double)y[0].S * y[0].NB;
-  });
+  {  // more 8-step blocks first (S * NB < 2^40)
+    std::vector<uint64_t> key(units.size());
+    for (size_t t = 0; t < units.size(); ++t) key[t] = (1ull << 40) - (uint64_t)units[t][0].S * units[t][0].NB;
+    gather_by_key(units, std::move(key));
+  }
   for (auto &q : units) dst.insert(dst.end(), q.begin(), q.end());
 }
 
@@ -547,7 +666,6 @@ struct ta_batch {
   bool may_saturate = false;
   int64_t cells = 0;
   std::vector<HostPair> hp;
-  std::vector<RawByte> codes;  // residue codes of the unique sequences, 16-byte aligned starts
   std::vector<FillOut> fo_init;
   FillOut *d_fo_init = nullptr;
   int32_t *d_order = nullptr;  // traceback walker -> pair, longest first
@@ -754,11 +872,16 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   // sequence.  Each host thread writes its sequences' codes and the zero
   // padding after them (so the buffer's pages are first touched in parallel).
   const size_t cend = ((ctotal + 15) & ~(size_t)15) + 32;
-  B->codes.resize(cend);
+  StageHold codes;  // residue codes of the unique sequences, 16-byte aligned starts
+  codes.s = stage_take(cend);
+  if (!codes.s.p) {
+    delete B;
+    return TA_E_NOMEM;
+  }
   std::atomic<int> bad_letter{0};
   parallel_for(uq.size(), [&](size_t u) {
     Uniq &e = uq[u];
-    uint8_t *dst = (uint8_t *)B->codes.data() + e.off;
+    uint8_t *dst = codes.s.p + e.off;
     int64_t sum = 0;
     for (int32_t i = 0; i < e.len; ++i) {
       const int c = B->alph.map[(unsigned char)e.s[i]];
@@ -770,7 +893,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     const size_t next = u + 1 < uq.size() ? uq[u + 1].off : cend;
     memset(dst + e.len, 0, next - (e.off + (size_t)e.len));
   });
-  if (uq.empty()) memset(B->codes.data(), 0, cend);
+  if (uq.empty()) memset(codes.s.p, 0, cend);
   if (bad_letter) {
     delete B;
     return TA_E_ALPHABET;
@@ -823,16 +946,19 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     if (B->hp[k].path == 2) i16.push_back(k);
     else if (B->hp[k].path == 1) i32.push_back(k);
   }
-  auto bigger = [&](int x, int y) {
-    const HostPair &a = B->hp[x], &b = B->hp[y];
-    const int sa = ceil_div(a.la, STRIP), sb = ceil_div(b.la, STRIP);
-    if (sa != sb) return sa > sb;
-    if (a.lb != b.lb) return a.lb > b.lb;
-    return x < y;
+  // More strips first, then more columns, then lower pair index (the lists
+  // hold ascending pair indices).
+  auto sort_bigger = [&](std::vector<int> &v) {
+    std::vector<uint64_t> key(v.size());
+    for (size_t t = 0; t < v.size(); ++t) {
+      const HostPair &h = B->hp[v[t]];
+      key[t] = (STRIPS_TOP - (uint64_t)ceil_div(h.la, STRIP)) << 20 | (LEN_TOP - (uint64_t)h.lb);
+    }
+    gather_by_key(v, std::move(key));
   };
   std::vector<std::pair<int, int>> couples;
   if (local) {
-    std::sort(i16.begin(), i16.end(), bigger);
+    sort_bigger(i16);
     for (size_t t = 0; t < i16.size(); t += 2)
       couples.push_back({i16[t], t + 1 < i16.size() ? i16[t + 1] : -1});
   } else {
@@ -840,7 +966,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     for (int k : i16) byn[B->hp[k].lb].push_back(k);
     for (auto &kv : byn) {
       auto &v = kv.second;
-      std::sort(v.begin(), v.end(), bigger);
+      sort_bigger(v);
       for (size_t t = 0; t < v.size(); t += 2) couples.push_back({v[t], t + 1 < v.size() ? v[t + 1] : -1});
     }
   }
@@ -942,8 +1068,12 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   hipStream_t st = B->stream;
   std::vector<int8_t> table = padded_table(B->alph);
   int rc = TA_OK;
-  rc = rc ? rc : dalloc(&B->d_codes, B->codes.size());
-  if (!rc) HIPCHK(hipMemcpyAsync(B->d_codes, B->codes.data(), B->codes.size(), hipMemcpyHostToDevice, st));
+  rc = rc ? rc : dalloc(&B->d_codes, cend);
+  if (!rc) {
+    codes.stream = st;
+    codes.queued = true;
+    HIPCHK(hipMemcpyAsync(B->d_codes, codes.s.p, cend, hipMemcpyHostToDevice, st));
+  }
   rc = rc ? rc : upload(&B->d_table, table, st);
   if (B->nib && !B->trace) rc = rc ? rc : upload(&B->d_table_nib, nibble_table(B->alph), st);
   for (int g = 0; g < 2; ++g) {
@@ -956,11 +1086,9 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
                               &B->d_colcodes, st);
   rc = rc ? rc : upload(&B->d_fo_init, B->fo_init, st);
   {
-    std::vector<int32_t> order(n);
-    for (int32_t k = 0; k < n; ++k) order[k] = k;
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-      return B->hp[x].la + B->hp[x].lb > B->hp[y].la + B->hp[y].lb;
-    });
+    std::vector<uint64_t> key(n);  // longer pairs (la + lb <= 2,000,000) first
+    for (int32_t k = 0; k < n; ++k) key[k] = 2000000 - (uint64_t)(B->hp[k].la + B->hp[k].lb);
+    const std::vector<int32_t> order = order_by_key(std::move(key));
     rc = rc ? rc : upload(&B->d_order, order, st);
     if (B->walk) {  // pairs without a fill task: empty and long pairs
       std::vector<int32_t> rest;
@@ -986,7 +1114,9 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   rc = rc ? rc : dalloc(&B->d_wtb, B->wtb_stride * B->long_wgs);
   for (auto &e : B->ev)
     if (!rc && hipEventCreate(&e) != hipSuccess) rc = TA_E_DEVICE;
-  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = TA_E_DEVICE;
+  // Drained on every path: the blocks of a failed creation go back to the
+  // cache only after the copies queued into them.
+  if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = TA_E_DEVICE;
   if (rc) {
     ta_batch_destroy(B);
     return rc;
