@@ -878,18 +878,28 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     delete B;
     return TA_E_NOMEM;
   }
+  // One entry per raw byte: best score << 8 | code, 0xff for a letter
+  // outside the alphabet (codes are below 0x80), so the loop has no branch.
+  uint32_t enc[256];
+  for (int b = 0; b < 256; ++b) {
+    const int c = B->alph.map[b];
+    enc[b] = c < 0 ? 0xffu : (uint32_t)best[c] << 8 | (uint32_t)c;
+  }
   std::atomic<int> bad_letter{0};
   parallel_for(uq.size(), [&](size_t u) {
     Uniq &e = uq[u];
     uint8_t *dst = codes.s.p + e.off;
-    int64_t sum = 0;
+    const unsigned char *src = (const unsigned char *)e.s;
+    uint64_t sum = 0;
+    uint32_t any = 0;
     for (int32_t i = 0; i < e.len; ++i) {
-      const int c = B->alph.map[(unsigned char)e.s[i]];
-      if (c < 0) { bad_letter = 1; break; }
-      dst[i] = (uint8_t)c;
-      sum += best[c];
+      const uint32_t t = enc[src[i]];
+      dst[i] = (uint8_t)t;
+      any |= t;
+      sum += t >> 8;
     }
-    e.bound = sum;
+    if (any & 0x80) bad_letter = 1;
+    e.bound = (int64_t)sum;
     const size_t next = u + 1 < uq.size() ? uq[u + 1].off : cend;
     memset(dst + e.len, 0, next - (e.off + (size_t)e.len));
   });
