@@ -908,6 +908,20 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     delete B;
     return TA_E_ALPHABET;
   }
+  // The codes go up while the host plans: from the pinned block the copy
+  // runs on the DMA engine alone.
+  hipStream_t st = B->stream;
+  int rc = dalloc(&B->d_codes, cend);
+  if (!rc) {
+    codes.stream = st;
+    codes.queued = true;
+    if (hipMemcpyAsync(B->d_codes, codes.s.p, cend, hipMemcpyHostToDevice, st) != hipSuccess) rc = TA_E_DEVICE;
+  }
+  if (rc) {
+    (void)hipStreamSynchronize(st);
+    ta_batch_destroy(B);
+    return rc;
+  }
   B->hp.resize(n);
   B->fo_init.assign(n, FillOut{0, 0, 0, 0});
   B->pairs_h.resize(n);
@@ -1075,15 +1089,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   }
 
   // Upload.
-  hipStream_t st = B->stream;
   std::vector<int8_t> table = padded_table(B->alph);
-  int rc = TA_OK;
-  rc = rc ? rc : dalloc(&B->d_codes, cend);
-  if (!rc) {
-    codes.stream = st;
-    codes.queued = true;
-    HIPCHK(hipMemcpyAsync(B->d_codes, codes.s.p, cend, hipMemcpyHostToDevice, st));
-  }
   rc = rc ? rc : upload(&B->d_table, table, st);
   if (B->nib && !B->trace) rc = rc ? rc : upload(&B->d_table_nib, nibble_table(B->alph), st);
   for (int g = 0; g < 2; ++g) {
