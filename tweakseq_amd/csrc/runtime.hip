@@ -248,13 +248,15 @@ void dfree(T *&p) {
   p = nullptr;
 }
 
-// Host staging for the residue codes.  A copy from pageable memory runs at
-// a fraction of the link's rate (the runtime copies it through bounce
-// buffers); from pinned memory the DMA engine reads the block itself.
-// Pinning a block takes milliseconds, so blocks are kept and reused: batch
-// creation takes one, encodes into it, and gives it back once its copy has
-// finished.  At most 4 free blocks stay pinned; blocks above 256 MB come
-// from the heap (pinning them would cost about what it saves).
+// Host staging for the residue codes, the results and the CIGAR runs.  A
+// copy to or from pageable memory runs at a fraction of the link's rate (the
+// runtime copies it through bounce buffers); with pinned memory the DMA
+// engine reads or writes the block itself.  Pinning a block takes
+// milliseconds, so blocks are kept and reused: batch creation takes one and
+// encodes into it, result fetching takes two and reads from them, and each
+// goes back once its copy has finished.  At most 16 free blocks and 1 GB stay
+// pinned; blocks above 256 MB come from the heap (pinning them would cost
+// about what it saves).
 struct Staging {
   uint8_t *p = nullptr;
   size_t cap = 0;
@@ -305,9 +307,13 @@ void stage_give(Staging &s) {
     StagePool &sp = stage_pool();
     std::lock_guard<std::mutex> g(sp.mu);
     sp.free.push_back(s);
-    if (sp.free.size() > 4) {  // unpin the smallest
-      auto it = std::min_element(sp.free.begin(), sp.free.end(),
-                                 [](const Staging &x, const Staging &y) { return x.cap < y.cap; });
+    auto by_cap = [](const Staging &x, const Staging &y) { return x.cap < y.cap; };
+    size_t bytes = 0;
+    for (const Staging &f : sp.free) bytes += f.cap;
+    while (sp.free.size() > 16 || bytes > ((size_t)1 << 30)) {  // unpin the smallest, or the largest past 1 GB
+      auto it = sp.free.size() > 16 ? std::min_element(sp.free.begin(), sp.free.end(), by_cap)
+                                    : std::max_element(sp.free.begin(), sp.free.end(), by_cap);
+      bytes -= it->cap;
       (void)hipHostFree(it->p);
       sp.free.erase(it);
     }
@@ -1509,24 +1515,43 @@ __global__ void gather_runs_kernel(const uint32_t *cig, const ResultDev *res, co
   for (int k = threadIdx.x; k < nr; k += blockDim.x) d[k] = src[k];
 }
 
-// Results and CIGAR runs of a finished batch on the host: `res` per pair,
-// `runs` dense, pair k's runs at runs[off[k] / 4 ...] (cigar_len bytes).
-// Only the runs cross PCIe, not the per-pair worst-case regions.
-int fetch_results(ta_batch *B, std::vector<ResultDev> &res, std::vector<uint32_t> &runs, std::vector<int64_t> &off) {
-  res.resize(B->n);
-  off.assign(B->n, 0);
-  runs.clear();
+// Results and CIGAR runs of a finished batch on the host, in pinned staging
+// blocks: `res` per pair, `runs` dense, pair k's runs at runs[off[k] / 4 ...]
+// (cigar_len bytes).  Only the runs cross PCIe, not the per-pair worst-case
+// regions.
+struct Fetched {
+  StageHold resb, runsb;
+  const ResultDev *res = nullptr;
+  const uint32_t *runs = nullptr;
+  std::vector<int64_t> off;
+};
+
+int fetch_results(ta_batch *B, Fetched &F) {
+  F.off.assign(B->n, 0);
+  std::vector<int64_t> &off = F.off;
   hipStream_t st = B->stream;
-  if (B->n) HIPCHK(hipMemcpyAsync(res.data(), B->d_res, sizeof(ResultDev) * B->n, hipMemcpyDeviceToHost, st));
+  if (B->n) {
+    F.resb.s = stage_take(sizeof(ResultDev) * B->n);
+    if (!F.resb.s.p) return TA_E_NOMEM;
+    F.resb.stream = st;
+    F.resb.queued = true;
+    HIPCHK(hipMemcpyAsync(F.resb.s.p, B->d_res, sizeof(ResultDev) * B->n, hipMemcpyDeviceToHost, st));
+    F.res = (const ResultDev *)F.resb.s.p;
+  }
   HIPCHK(hipStreamSynchronize(st));
   if (!B->p.traceback) return TA_OK;
+  const ResultDev *res = F.res;
   int64_t total = 0;
   for (int32_t k = 0; k < B->n; ++k) {
     off[k] = total;
     if (res[k].cigar_len > 0) total += res[k].cigar_len;
   }
   if (!total) return TA_OK;
-  runs.resize((size_t)(total / 4));
+  F.runsb.s = stage_take((size_t)total);
+  if (!F.runsb.s.p) return TA_E_NOMEM;
+  F.runsb.stream = st;
+  F.runsb.queued = true;
+  F.runs = (const uint32_t *)F.runsb.s.p;
   int64_t *d_off = nullptr;
   uint32_t *d_runs = nullptr;
   int rc = upload(&d_off, off, st);
@@ -1536,7 +1561,7 @@ int fetch_results(ta_batch *B, std::vector<ResultDev> &res, std::vector<uint32_t
     hipLaunchKernelGGL(gather_runs_kernel, dim3(B->n), dim3(64), 0, st, (const uint32_t *)B->d_cig, B->d_res,
                        d_off, d_runs, B->n);
     if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(runs.data(), d_runs, (size_t)total, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(F.runsb.s.p, d_runs, (size_t)total, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
       rc = TA_E_DEVICE;
   }
@@ -1558,18 +1583,19 @@ void fill_result(ta_result &o, const ResultDev &r) {
 int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, int64_t cap, int64_t *used) {
   if (!ctx || !B || (B->n && !out)) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
-  std::vector<ResultDev> res;
-  std::vector<uint32_t> runs;
-  std::vector<int64_t> off;
-  int rc = fetch_results(B, res, runs, off);
+  Fetched F;
+  int rc = fetch_results(B, F);
   if (rc) return rc;
+  const ResultDev *res = F.res;
+  const uint32_t *runs = F.runs;
+  const std::vector<int64_t> &off = F.off;
   // CIGAR text lengths, their offsets, then the text, the first and last
   // passes on the host threads.
   const bool tb = B->p.traceback != 0;
   parallel_for((size_t)B->n, [&](size_t k) {
     const ResultDev &r = res[k];
     fill_result(out[k], r);
-    if (tb && r.cigar_len > 0) out[k].cigar_len = (int32_t)format_runs(runs.data() + off[k] / 4, r.cigar_len / 4, nullptr);
+    if (tb && r.cigar_len > 0) out[k].cigar_len = (int32_t)format_runs(runs + off[k] / 4, r.cigar_len / 4, nullptr);
   });
   int64_t pos = 0;
   for (int32_t k = 0; k < B->n; ++k) {
@@ -1580,7 +1606,7 @@ int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, 
   if (pos > 0 && (!cigar_buf || pos > cap)) return TA_E_SPACE;
   parallel_for((size_t)B->n, [&](size_t k) {
     if (out[k].cigar_len > 0)
-      format_runs(runs.data() + off[k] / 4, res[k].cigar_len / 4, cigar_buf + out[k].cigar_off);
+      format_runs(runs + off[k] / 4, res[k].cigar_len / 4, cigar_buf + out[k].cigar_off);
   });
   return rc;
 }
@@ -1588,18 +1614,19 @@ int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, 
 int batch_results_text(ta_ctx *ctx, ta_batch *B, std::vector<ta_result> &out, std::string &text) {
   if (!ctx || !B) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
-  std::vector<ResultDev> res;
-  std::vector<uint32_t> runs;
-  std::vector<int64_t> off;
-  const int rc = fetch_results(B, res, runs, off);
+  Fetched F;
+  const int rc = fetch_results(B, F);
   if (rc) return rc;
+  const ResultDev *res = F.res;
+  const uint32_t *runs = F.runs;
+  const std::vector<int64_t> &off = F.off;
   // lengths, offsets, then the text, as ta_batch_results
   out.resize(B->n);
   const bool tb = B->p.traceback != 0;
   parallel_for((size_t)B->n, [&](size_t k) {
     const ResultDev &r = res[k];
     fill_result(out[k], r);
-    if (tb && r.cigar_len > 0) out[k].cigar_len = (int32_t)format_runs(runs.data() + off[k] / 4, r.cigar_len / 4, nullptr);
+    if (tb && r.cigar_len > 0) out[k].cigar_len = (int32_t)format_runs(runs + off[k] / 4, r.cigar_len / 4, nullptr);
   });
   int64_t pos = 0;
   for (int32_t k = 0; k < B->n; ++k) {
@@ -1608,7 +1635,7 @@ int batch_results_text(ta_ctx *ctx, ta_batch *B, std::vector<ta_result> &out, st
   }
   text.assign((size_t)pos, '\0');
   parallel_for((size_t)B->n, [&](size_t k) {
-    if (out[k].cigar_len > 0) format_runs(runs.data() + off[k] / 4, res[k].cigar_len / 4, &text[out[k].cigar_off]);
+    if (out[k].cigar_len > 0) format_runs(runs + off[k] / 4, res[k].cigar_len / 4, &text[out[k].cigar_off]);
   });
   return TA_OK;
 }
