@@ -200,3 +200,23 @@ def test_shared_buffers_and_prefixes(eng, mode):
     params = ta.engine.make_params(mode=mode, gap_open=-11 if mode else -10)
     got = eng.align(ps, params)
     compare(got, oracle_results(params, strs), params, strs, "shared")
+
+
+def test_lone_pairs(eng):
+    """A batch of one fill task walks after its fill (the traceback kernel):
+    lone pairs, global and local, one strip or several, each in a batch of
+    its own; a lone identical-pointer couple too."""
+    rng = random.Random(17)
+    for la, lb in ((300, 320), (40, 700), (700, 40), (530, 515), (1200, 900), (1, 1)):
+        a = rand_prot(rng, la)
+        b = mutate(rng, a, 0.25, 0.04)[:lb] or "W"
+        for mode in (ta.GLOBAL, ta.LOCAL):
+            params = ta.engine.make_params(mode=mode)
+            pairs = [(a, b)]
+            compare(eng.align(pairs, params), oracle_results(params, pairs), params, pairs,
+                    "lone %dx%d mode %d" % (la, len(b), mode))
+    a = rand_prot(rng, 300)
+    b = mutate(rng, a)
+    params = ta.engine.make_params(mode=ta.GLOBAL)
+    pairs = [(a, b), (a, b)]
+    compare(eng.align(pairs, params), oracle_results(params, pairs), params, pairs, "lone couple")
