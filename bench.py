@@ -48,35 +48,63 @@ def timed_steps(batch, steps, warmup):
     return wall, fill_ms, run_ms
 
 
-def cpu_baseline(workload, budget_s=12.0):
+def host_cpu():
+    """The host's CPU: cores this process may use (the affinity mask, capped by
+    OMP_NUM_THREADS when set: a GPU's share of its host), nproc and the model."""
+    import subprocess
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = nproc
+    threads = aff
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if not model:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    return {"nproc": nproc, "affinity": aff, "threads": threads, "omp_num_threads": omp or None, "model": model}
+
+
+def oracle_params(w, traceback=None):
     from oracle import pyoracle as po
-    nthreads = min(os.cpu_count() or 1, 16)
-    p = workload.params
-    op = po.params(mode=p.mode, alphabet=p.alphabet, gap_open=p.gap_open, gap_extend=p.gap_extend,
-                   match=p.match, mismatch=p.mismatch, traceback=p.traceback)
-
-    def run(npairs):
-        codes, a_off, la, b_off, lb, cells = oracle_inputs(workload, 0, npairs)
-        t0 = time.perf_counter()
-        po.align_batch(op, codes, a_off, la, b_off, lb, nthreads=nthreads)
-        return cells, time.perf_counter() - t0
-
-    n = 256
-    cells, dt = run(n)
-    n = min(workload.npairs, max(n, int(n * budget_s / max(dt, 1e-3))))
-    cells, dt = run(n)
-    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": nthreads, "kind": "port",
-            "sample": "first %d of the %d pairs (%.3g cells), oracle/gotoh.c with traceback, OpenMP %d threads, %.1f s"
-                      % (n, workload.npairs, cells, nthreads, dt)}
+    p = w.params
+    return po.params(mode=p.mode, alphabet=p.alphabet, gap_open=p.gap_open, gap_extend=p.gap_extend,
+                     match=p.match, mismatch=p.mismatch, traceback=p.traceback if traceback is None else traceback)
 
 
-def oracle_inputs(workload, first, count):
+def oracle_run(workload, ks, threads, with_cigar=True):
+    """The oracle (oracle/gotoh.c, OpenMP) over pairs ks of the workload:
+    (results, cigars, cells, seconds)."""
+    from oracle import pyoracle as po
+    codes, a_off, la, b_off, lb, cells = oracle_inputs(workload, ks)
+    t0 = time.perf_counter()
+    res, cig = po.align_batch(oracle_params(workload), codes, a_off, la, b_off, lb, nthreads=threads,
+                              with_cigar=with_cigar)
+    return res, cig, cells, time.perf_counter() - t0
+
+
+def oracle_inputs(workload, ks):
     from oracle import pyoracle as po
     import numpy as np
     p = workload.params
     seqs, a_off, la, b_off, lb = [], [], [], [], []
     pos = 0
-    for k in range(first, first + count):
+    for k in ks:
         a, b = workload.pair_bytes(k)
         ea, eb = po.encode(a, p.alphabet), po.encode(b, p.alphabet)
         seqs += [ea, eb]
@@ -86,41 +114,74 @@ def oracle_inputs(workload, first, count):
     return codes, a_off, la, b_off, lb, sum(x * y for x, y in zip(la, lb))
 
 
+def parity(gpu, res, cigars, traceback):
+    """Pairs whose GPU result (score, coordinates, CIGAR) differs from the
+    oracle's, or whose walk gave up."""
+    bad = []
+    for k, g in enumerate(gpu):
+        r = res[k]
+        want = (r.score, r.a_begin, r.a_end, r.b_begin, r.b_end)
+        have = (g.score, g.a_begin, g.a_end, g.b_begin, g.b_end)
+        if have != want or (traceback and g.cigar != cigars[k]) or g.flags & 0x100:
+            bad.append(k)
+    return bad
+
+
+def cpu_full_and_parity(workload, gpu_results, threads, label):
+    """The oracle over every pair of the workload on `threads` host threads
+    (the CPU baseline), its results compared with the GPU's pair by pair, and
+    one thread over a seeded 1% sample (every 100th pair), extrapolated to the
+    whole workload by cells."""
+    n = workload.npairs
+    tb = workload.params.traceback != 0
+    res, cig, cells, dt = oracle_run(workload, range(n), threads, with_cigar=tb)
+    bad = parity(gpu_results, res, cig, tb)
+    ks = range(0, n, 100)
+    _, _, scells, sdt = oracle_run(workload, ks, 1, with_cigar=tb)
+    base = {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+            "sample": "all %d pairs (%.4g cells), oracle/gotoh.c%s, OpenMP %d threads, %.2f s"
+                      % (n, cells, " with traceback" if tb else ", score only", threads, dt),
+            "single_thread_1pct": {"value": round(scells / sdt / 1e9, 4), "unit": "GCUPS", "cores": 1,
+                                   "sample": "every 100th pair (%d pairs, %.3g cells), one thread, %.2f s"
+                                             % (len(ks), scells, sdt),
+                                   "extrapolated_s": round(sdt * cells / scells, 1),
+                                   "note": "whole-workload time extrapolated from the sample by cells"}}
+    par = {"pairs": n, "mismatches": len(bad), "first_bad": bad[:5], "checked": "score, coordinates"
+           + (", CIGAR" if tb else "") + " against oracle/gotoh.c", "workload": label}
+    return base, par
+
+
 def cpu_single_pair(workload, repeats=1000):
     """The 300 x 320 global pair (workload 1) on one core: median of `repeats`
-    alignments with traceback."""
+    alignments with traceback, timed in C around or_align."""
     from oracle import pyoracle as po
-    p = workload.params
-    op = po.params(mode=p.mode, alphabet=p.alphabet, gap_open=p.gap_open, gap_extend=p.gap_extend,
-                   match=p.match, mismatch=p.mismatch, traceback=1)
     a, b = workload.pair_bytes(0)
+    p = oracle_params(workload, traceback=1)
     ea, eb = po.encode(a, p.alphabet), po.encode(b, p.alphabet)
-    ts = []
-    for _ in range(repeats):
-        t0 = time.perf_counter()
-        po.align(op, ea, eb)
-        ts.append(time.perf_counter() - t0)
-    ts.sort()
-    med = ts[len(ts) // 2]
+    med = po.time_align(p, ea, eb, repeats)
     return {"value": round(len(ea) * len(eb) / med / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
             "median_ms": round(med * 1e3, 4),
-            "sample": "one %dx%d pair, median of %d runs (Python call overhead included)" % (len(ea), len(eb), repeats)}
+            "sample": "one %dx%d pair with traceback, median of %d runs timed in C (no call overhead)"
+                      % (len(ea), len(eb), repeats)}
 
 
 def cpu_long_dna(workload, npairs=4):
     """Long-DNA fill on the CPU: score only (a full-matrix traceback of a
-    100,000-base pair needs 10 GB), first `npairs` pairs, one thread each."""
+    100,000-base pair needs 10 GB), the first `npairs` pairs, one thread
+    each, extrapolated by cells to all 256 pairs."""
     from oracle import pyoracle as po
-    p = workload.params
-    op = po.params(mode=p.mode, alphabet=p.alphabet, gap_open=p.gap_open, gap_extend=p.gap_extend,
-                   match=p.match, mismatch=p.mismatch, traceback=0)
-    codes, a_off, la, b_off, lb, cells = oracle_inputs(workload, 0, npairs)
+    codes, a_off, la, b_off, lb, cells = oracle_inputs(workload, range(npairs))
     t0 = time.perf_counter()
-    po.align_batch(op, codes, a_off, la, b_off, lb, nthreads=npairs, with_cigar=False)
+    po.align_batch(oracle_params(workload, traceback=0), codes, a_off, la, b_off, lb, nthreads=npairs,
+                   with_cigar=False)
     dt = time.perf_counter() - t0
+    total = workload.cells()
     return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": npairs, "kind": "port",
             "sample": "first %d of %d pairs (%.3g cells), score only, one thread per pair, %.1f s"
-                      % (npairs, workload.npairs, cells, dt)}
+                      % (npairs, workload.npairs, cells, dt),
+            "extrapolated_s_all_pairs": round(dt * total / cells / npairs, 1),
+            "note": "extrapolated: all %d pairs' cells at this sample's per-thread rate, on %d threads"
+                    % (workload.npairs, npairs)}
 
 
 def fill_traffic():
@@ -144,6 +205,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shapes)")
+    ap.add_argument("--served-requests", type=int, default=120000,
+                    help="requests of the served run (120,000 = the 60 s Poisson schedule at 2,000/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
@@ -179,8 +242,8 @@ def main():
     fill_gcups = cells / (fill_avg / 1e3) / 1e9
     log("batched: %.2f ms/step (%.1f GCUPS), fill kernel %.2f ms (%.1f GCUPS), run %.2f ms"
         % (ms_step, gcups, fill_avg, fill_gcups, sum(run_ms) / len(run_ms)))
-    res = b3.results()
-    n_i32 = sum(1 for r in res if r.flags & ta.F_I32)
+    res3 = b3.results()  # every pair, checked against the oracle below (parity_full)
+    n_i32 = sum(1 for r in res3 if r.flags & ta.F_I32)
     b3.close()
 
     out = {
@@ -273,6 +336,7 @@ def main():
         b2 = eng.batch(w2.pairs_struct(), w2.params)
         c2 = b2.info()["cells"]
         wall2, f2, _ = timed_steps(b2, max(2, args.steps // 2), 1)
+        res2 = b2.results()
         b2.close()
         out["allvsall_gcups"] = round(c2 / (wall2 / max(2, args.steps // 2)) / 1e9, 2)
         out["allvsall_fill_gcups"] = round(c2 / (sum(f2) / len(f2) / 1e3) / 1e9, 2)
@@ -371,8 +435,11 @@ def main():
         # scheduled arrival.
         import subprocess
         sb = os.path.join(ROOT, "tweakseq_amd", "tweakalign-serve-bench")
-        if os.path.exists(sb):
-            r = subprocess.run([sb, "4000"], capture_output=True, text=True, timeout=300)
+        if os.path.exists(sb) and args.served_requests > 0:
+            log("served: %d requests at 2,000/s (%.0f s of arrivals)..." % (args.served_requests,
+                                                                           args.served_requests / 2000))
+            r = subprocess.run([sb, str(args.served_requests)], capture_output=True, text=True,
+                               timeout=300 + args.served_requests / 1000)
             if r.returncode == 0:
                 out["served"] = json.loads(r.stdout.strip().splitlines()[-1])
                 # served GCUPS follow the offered load (2,000 requests/s); the
@@ -391,16 +458,35 @@ def main():
 
     if tuned:
         out["autotune"] = tuned
+    bad_parity = False
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w3)
-        log("cpu baseline:", out["cpu_baseline"])
+        hc = host_cpu()
+        out["cpu_host"] = hc
+        log("cpu host:", hc)
+        base, par = cpu_full_and_parity(w3, res3, hc["threads"], "65,536 local pairs (the timed batch)")
+        base["host"] = hc["model"]
+        out["cpu_baseline"] = base
+        out["parity_full"] = {"local_65536": par}
+        bad_parity = bad_parity or par["mismatches"] > 0
+        log("cpu baseline:", base)
+        log("parity, 65,536 local pairs:", par)
         if not args.no_extra:
+            b2base, par2 = cpu_full_and_parity(ta.Workload(2), res2, hc["threads"],
+                                               "130,816 all-vs-all global pairs, score only")
+            out["cpu_allvsall"] = b2base
+            out["parity_full"]["allvsall_130816"] = par2
+            bad_parity = bad_parity or par2["mismatches"] > 0
+            log("cpu all-vs-all:", b2base)
+            log("parity, all-vs-all:", par2)
             out["cpu_single_pair"] = cpu_single_pair(ta.Workload(1))
             out["cpu_long_dna"] = cpu_long_dna(ta.Workload(5))
             log("cpu single pair:", out["cpu_single_pair"])
             log("cpu long dna:", out["cpu_long_dna"])
     eng.close()
     print(json.dumps(out), flush=True)
+    if bad_parity:
+        log("PARITY FAILURE: GPU results differ from the oracle (parity_full)")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

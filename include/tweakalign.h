@@ -33,6 +33,8 @@ extern "C" {
 #define TA_E_CANCELLED -7     /* request was cancelled */
 #define TA_E_TICKET -8        /* unknown ticket */
 #define TA_E_NODEVICE -9      /* no usable GPU */
+#define TA_E_WALK -10         /* a traceback walk did not complete: the pairs concerned carry
+                                 TA_F_WALK_FAILED; their coordinates and CIGAR are not valid */
 #define TA_E_PENDING 1        /* ta_poll: not finished yet (not an error) */
 
 #define TA_GLOBAL 0           /* Needleman-Wunsch (Gotoh affine gaps) */
@@ -53,6 +55,9 @@ extern "C" {
 #define TA_F_RERUN 0x2        /* int16 pass saturated; rerun in int32 */
 #define TA_F_LONG 0x4         /* long-pair (multi-strip pipeline) path */
 #define TA_F_EMPTY 0x8        /* local: no cell scored above zero */
+#define TA_F_WALK_FAILED 0x100 /* the traceback walk gave up (step budget spent, CIGAR region
+                                  overflowed or the walk left the planes): begin coordinates and
+                                  CIGAR are invalid; the call that returns it fails with TA_E_WALK */
 
 typedef struct ta_ctx ta_ctx;
 
@@ -95,7 +100,10 @@ void ta_destroy(ta_ctx *ctx);
  * (tweakseq/Core/AlignmentTool.h:54, tweakseq/UI/SeqEditMainWin.cpp:1616-1663,
  * :836-861).  TA_E_SPACE with *used set when
  * cap is too small (results are still written).  cigar_buf may be NULL when
- * p->traceback == 0. */
+ * p->traceback == 0.  TA_E_WALK when any pair's traceback walk gave up
+ * (results and CIGARs are still written; those pairs carry TA_F_WALK_FAILED):
+ * the counterpart of an aligner that exits non-zero, which the reference
+ * reports as "Alignment not completed" (SeqEditMainWin.cpp:852-856). */
 int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n,
                    ta_result *out, char *cigar_buf, int64_t cap, int64_t *used);
 
@@ -110,7 +118,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *b, int32_t first, int32_t count);
                                                      /* one launch per pair (baseline) */
 int ta_batch_wait(ta_ctx *ctx, ta_batch *b);
 int ta_batch_results(ta_ctx *ctx, ta_batch *b, ta_result *out, char *cigar_buf,
-                     int64_t cap, int64_t *used);
+                     int64_t cap, int64_t *used);   /* TA_E_SPACE / TA_E_WALK as ta_align_batch */
 /* Replay the batch's runs as one captured hipGraph (on != 0): the first run
  * captures init, fill and traceback kernels, later runs launch the graph.
  * Batches that may need the int32 rerun keep launching eagerly.  No
@@ -137,7 +145,10 @@ int ta_serve_config(ta_ctx *ctx, int64_t tb_budget_bytes, int32_t max_inflight);
 int ta_submit(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n,
               uint64_t *ticket);
 int ta_poll(ta_ctx *ctx, uint64_t ticket, ta_result *out, char *cigar_buf, int64_t cap,
-            int64_t *used);                          /* TA_OK done, TA_E_PENDING, <0 error */
+            int64_t *used);                          /* TA_OK done, TA_E_PENDING, <0 error;
+                                                        TA_E_WALK: delivered (results and CIGARs
+                                                        written, ticket retired) but a walk of the
+                                                        request gave up (TA_F_WALK_FAILED) */
 int ta_cancel(ta_ctx *ctx, uint64_t ticket);
 
 /* Autotuning (no counterpart in the reference).  A tuning is the launch

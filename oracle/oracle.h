@@ -77,6 +77,11 @@ int or_rescore(const or_params *p, const int8_t *smat, int32_t K,
                const uint8_t *a, const uint8_t *b, const or_result *res,
                const char *cigar, int32_t *score);
 
+/* Median wall time (seconds) of `repeats` or_align calls on one pair, timed
+ * in C (timing.c): the single-core CPU baseline without call overhead. */
+int or_time_align(const or_params *p, const int8_t *smat, int32_t K, const uint8_t *a, int32_t m,
+                  const uint8_t *b, int32_t n, int32_t repeats, char *cigar, int64_t cap, double *median_s);
+
 /* Column consensus of an alignment of nseq rows of len characters (msa is
  * row-major, row s at msa + s * len), restating Consensus::calculate
  * (tweakseq/Core/Annotations/Consensus.cpp:80-161) with the 23 x 23 BLOSUM62

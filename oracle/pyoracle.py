@@ -47,6 +47,9 @@ def lib():
         L.or_rescore.argtypes = [P(Params), P(ctypes.c_int8), ctypes.c_int32,
                                  P(ctypes.c_uint8), P(ctypes.c_uint8), P(Result),
                                  ctypes.c_char_p, P(ctypes.c_int32)]
+        L.or_time_align.argtypes = [P(Params), P(ctypes.c_int8), ctypes.c_int32, P(ctypes.c_uint8), ctypes.c_int32,
+                                    P(ctypes.c_uint8), ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p,
+                                    ctypes.c_int64, P(ctypes.c_double)]
         L.or_consensus.argtypes = [P(ctypes.c_int8), P(ctypes.c_int32), ctypes.c_char_p, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_double, ctypes.c_char_p]
         L.or_msa.argtypes = [P(ctypes.c_int8), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -63,9 +66,15 @@ def blosum62_fixture():
         return json.load(f)
 
 
+_PROTEIN = None
+
+
 def protein_matrix():
-    d = blosum62_fixture()
-    return np.array(d["matrix"], dtype=np.int8), np.array(d["map"], dtype=np.int32)
+    global _PROTEIN
+    if _PROTEIN is None:
+        d = blosum62_fixture()
+        _PROTEIN = (np.array(d["matrix"], dtype=np.int8), np.array(d["map"], dtype=np.int32))
+    return _PROTEIN[0].copy(), _PROTEIN[1].copy()
 
 
 def dna_matrix(match, mismatch):
@@ -221,3 +230,38 @@ def msa_rows(seqs_text, alen, cols):
             r[q] = s[k]
         rows.append("".join(r))
     return rows
+
+
+def align_pairs(p, pairs, nthreads=0, alphabet=None):
+    """(score, a_begin, a_end, b_begin, b_end, cigar) of every (a, b) letter
+    pair (bytes or str), through or_align_batch on `nthreads` OpenMP threads
+    (<= 0: all).  The CIGAR is "" when p.traceback is 0."""
+    alphabet = p.alphabet if alphabet is None else alphabet
+    seqs, a_off, la, b_off, lb = [], [], [], [], []
+    pos = 0
+    for a, b in pairs:
+        ea, eb = encode(a, alphabet), encode(b, alphabet)
+        seqs += [ea, eb]
+        a_off.append(pos); la.append(len(ea)); pos += len(ea)
+        b_off.append(pos); lb.append(len(eb)); pos += len(eb)
+    codes = np.concatenate(seqs + [np.zeros(1, np.uint8)]) if seqs else np.zeros(1, np.uint8)
+    res, cigars = align_batch(p, codes, a_off, la, b_off, lb, nthreads=nthreads)
+    return [(res[k].score, res[k].a_begin, res[k].a_end, res[k].b_begin, res[k].b_end,
+             cigars[k] if cigars is not None else "") for k in range(len(pairs))]
+
+
+def time_align(p, a, b, repeats=1000, smat=None):
+    """Median seconds of `repeats` or_align calls on codes a, b, timed in C."""
+    if smat is None:
+        smat = matrix_for(p)
+    smat = np.ascontiguousarray(smat, dtype=np.int8)
+    a, pa = _u8(a)
+    b, pb = _u8(b)
+    cap = 2 * (len(a) + len(b)) + 16
+    buf = ctypes.create_string_buffer(cap)
+    med = ctypes.c_double()
+    rc = lib().or_time_align(ctypes.byref(p), smat.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)), smat.shape[0],
+                             pa, len(a), pb, len(b), repeats, buf, cap, ctypes.byref(med))
+    if rc:
+        raise RuntimeError("or_time_align failed: %d" % rc)
+    return med.value

@@ -38,8 +38,18 @@ def score_ops(ops, a, b, i0, j0, S, go, ge):
     return s
 
 
+_PATHS = {}
+
+
 def _paths(m, n):
-    """All op strings from (0,0) consuming exactly m rows and n columns."""
+    """All op strings from (0,0) consuming exactly m rows and n columns
+    (cached: the 7 x 7 set holds D(7,7) = 48,639 strings)."""
+    if (m, n) not in _PATHS:
+        _PATHS[(m, n)] = _enumerate(m, n)
+    return _PATHS[(m, n)]
+
+
+def _enumerate(m, n):
     out = []
 
     def rec(i, j, acc):

@@ -53,6 +53,9 @@ def oracle_results(params, pairs):
 
 
 def compare(got, want, params, pairs, label):
+    assert len(got) == len(want)
+    walk_failed = [k for k, g in enumerate(got) if g.flags & ta.F_WALK_FAILED]
+    assert not walk_failed, "%s: walks gave up on pairs %r" % (label, walk_failed[:5])
     bad = []
     for k, (g, w) in enumerate(zip(got, want)):
         gt = (g.score, g.a_begin, g.a_end, g.b_begin, g.b_end)
@@ -161,10 +164,43 @@ def test_batched_equals_plain(eng):
         assert batched == plain
 
 
+def test_plain_reruns_saturated_pairs(eng):
+    """The plain form (one launch per pair) reruns a saturating int16 pair in
+    int32 as the batched form does: 3,000 x 'W' scores 33,000."""
+    rng = random.Random(14)
+    w = "W" * 3000
+    pairs = [(w, w), (rand_prot(rng, 400), rand_prot(rng, 380)), (w[:2990] + "A" * 5, w)]
+    params = ta.engine.make_params(mode=ta.LOCAL, gap_open=-11)
+    b = eng.batch(pairs, params)
+    b.run(); b.wait()
+    batched = b.results()
+    b.run_plain(); b.wait()
+    plain = b.results()
+    b.close()
+    assert plain == batched
+    assert plain[0].score == 33000 and plain[0].flags & ta.F_RERUN
+    compare(plain, oracle_results(params, pairs), params, pairs, "plain saturating")
+
+
 def test_workload_single_pair_300x320(eng):
     w = ta.Workload(1)
     a, b = w.pair_bytes(0)
     check(eng, [(a, b)], "300 x 320 global pair", mode=w.params.mode, gap_open=w.params.gap_open)
+
+
+def test_workload_allvsall_with_derived_sequences(eng):
+    """All-vs-all guide-tree scoring on 64 of the seeded sequences (2,016
+    global pairs, score only; 30% of the sequences derived from a parent,
+    so related high-scoring pairs are in the mix) against the oracle."""
+    w = ta.Workload(2, size=64)
+    pairs = [w.pair_bytes(k) for k in range(w.npairs)]
+    assert w.params.traceback == 0 and len(pairs) == 64 * 63 // 2
+    got = eng.align(w.pairs_struct(), w.params)
+    p = po.params(mode=w.params.mode, gap_open=w.params.gap_open, gap_extend=w.params.gap_extend, traceback=0)
+    want = po.align_pairs(p, pairs)
+    compare(got, want, w.params, pairs, "all-vs-all 64")
+    # the derived pairs score far above the unrelated ones: the mix is in the test
+    assert max(g.score for g in got) > 300
 
 
 def test_workload_local_batch_prefix(eng):

@@ -61,6 +61,38 @@ def test_results_identical_alone_and_in_the_mix():
         eng.close()
 
 
+def test_served_config4_mix_matches_oracle():
+    """The served workload's mix -- 80 requests of 1-256 pairs (geometric,
+    mean 16), lengths uniform in [50, 5,000], global and local interleaved
+    per request, -10/-1 -- submitted at once through ta_submit / ta_poll; every
+    pair's score, coordinates and CIGAR equal the CPU oracle's."""
+    from oracle import pyoracle as po
+    w = ta.Workload(4, size=80, max_len=5000)
+    reqs = w.requests()
+    assert {m for _, _, _, m in reqs} == {ta.GLOBAL, ta.LOCAL}
+    eng = ta.Engine(0)
+    try:
+        eng.serve_config(64 << 30, 4)
+        tickets = {}
+        for q, (_, first, count, mode) in enumerate(reqs):
+            params = ta.engine.make_params(mode=mode, gap_open=-10, gap_extend=-1)
+            tickets[q] = eng.submit(w.pairs_struct(first, count), params)
+        served = wait_all(eng, tickets, timeout=300)
+    finally:
+        eng.close()
+    for mode in (ta.GLOBAL, ta.LOCAL):
+        idx = [(q, k) for q, (_, first, count, m) in enumerate(reqs) if m == mode for k in range(count)]
+        pairs = [w.pair_bytes(reqs[q][1] + k) for q, k in idx]
+        want = po.align_pairs(po.params(mode=mode, gap_open=-10, gap_extend=-1), pairs)
+        bad = []
+        for (q, k), (a, b), wt in zip(idx, pairs, want):
+            g = served[q][k]
+            if (g.score, g.a_begin, g.a_end, g.b_begin, g.b_end, g.cigar) != wt or g.flags & ta.F_WALK_FAILED:
+                bad.append((q, k, len(a), len(b), (g.score, g.a_begin, g.a_end, g.b_begin, g.b_end), wt[:5]))
+        assert not bad, "mode %d: %d/%d served pairs differ from the oracle, first %r" % (
+            mode, len(bad), len(idx), bad[:3])
+
+
 def test_small_budget_serialises_but_completes():
     eng = ta.Engine(0)
     try:

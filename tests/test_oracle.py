@@ -71,6 +71,27 @@ def test_local_bruteforce(go, ge):
         assert got == want, (a, b, got, want)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_bruteforce_to_seven(mode):
+    """Lengths 6 and 7 (up to D(7,7) = 48,639 global alignments per pair),
+    three-letter alphabet for ties, two gap settings."""
+    import time
+    rng = random.Random(77 + mode)
+    t0 = time.time()
+    for go, ge in ((-10, -1), (-3, -3)):
+        p = po.params(mode=mode, gap_open=go, gap_extend=ge)
+        for _ in range(6 if mode == 0 else 4):
+            m, n = rng.randint(6, 7), rng.randint(6, 7)
+            if mode == 1:
+                m, n = 7, rng.randint(6, 7)  # local enumerates every start and end as well
+            a, b = rand_codes(rng, m, 3), rand_codes(rng, n, 3)
+            got = po.align(p, a, b)
+            fn = bf.global_align if mode == 0 else bf.local_align
+            want = fn(list(a), list(b), SP, go, ge)
+            assert got == want, (a, b, got, want)
+    assert time.time() - t0 < 120
+
+
 def test_dna_bruteforce():
     rng = random.Random(7)
     p = po.params(mode=0, alphabet=1, gap_open=-5, gap_extend=-2, match=2, mismatch=-3)

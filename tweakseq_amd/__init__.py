@@ -9,5 +9,5 @@ built library every entry point raises.
 """
 from .engine import (  # noqa: F401
     Engine, Batch, Result, Workload, lib, encode, matrix, LOCAL, GLOBAL, PROTEIN, DNA,
-    LANES_AUTO, LANES_I16, LANES_I32, LANES_LONG, F_I32, F_RERUN, F_LONG, F_EMPTY,
+    LANES_AUTO, LANES_I16, LANES_I32, LANES_LONG, F_I32, F_RERUN, F_LONG, F_EMPTY, F_WALK_FAILED, E_WALK,
 )

@@ -44,7 +44,7 @@ def build_lib():
         run(["hipcc", "-c"] + HIP_FLAGS + [s, "-o", o])
         objs.append(o)
     wo = os.path.join(CSRC, "workload.o")
-    run(["g++", "-c", "-O2", "-std=c++17", "-fPIC", "-Wall", os.path.join(CSRC, "workload.cpp"), "-o", wo])
+    run(["g++", "-c", "-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", os.path.join(CSRC, "workload.cpp"), "-o", wo])
     objs.append(wo)
     tmp = LIB + ".tmp"
     run(["hipcc", "-shared", "--offload-arch=" + ARCH, "-fPIC", "-o", tmp] + objs)

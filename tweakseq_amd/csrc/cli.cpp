@@ -31,93 +31,9 @@
 #include <vector>
 
 #include "../../include/tweakalign.h"
+#include "fasta.h"
 
 namespace {
-
-struct Record {
-  std::string header;  // verbatim, including '>' or ';'
-  std::string seq;
-};
-
-std::string trim(const std::string &s) {
-  size_t a = 0, b = s.size();
-  while (a < b && isspace((unsigned char)s[a])) ++a;
-  while (b > a && isspace((unsigned char)s[b - 1])) --b;
-  return s.substr(a, b - a);
-}
-
-// FASTAFile::read rules (FASTAFile.cpp:71-147).
-bool read_fasta(std::istream &in, std::vector<Record> &out) {
-  enum { SEEK, COMMENT, SEQ } st = SEEK;
-  std::string line;
-  while (std::getline(in, line)) {
-    const std::string s = trim(line);
-    if (s.empty()) continue;
-    const char c = s[0];
-    if (st == SEEK) {
-      if (c == ';' || c == '>') { out.push_back({s, ""}); st = COMMENT; }
-    } else if (st == COMMENT) {
-      if (c == ';') continue;
-      out.back().seq = s;
-      st = SEQ;
-    } else {
-      if (c == ';' || c == '>') { out.push_back({s, ""}); st = COMMENT; }
-      else out.back().seq += s;
-    }
-  }
-  return true;
-}
-
-// FASTAFile::write rules (FASTAFile.cpp:149-171).
-void write_fasta(std::ostream &o, const std::vector<Record> &recs) {
-  for (const Record &r : recs) {
-    o << r.header << '\n';
-    for (size_t p = 0; p < r.seq.size(); p += 80) o << r.seq.substr(p, 80) << '\n';
-  }
-}
-
-std::string dealign(const std::string &s) {
-  std::string r;
-  for (char c : s)
-    if (c != '-' && c != '.') r.push_back(c);
-  return r;
-}
-
-bool looks_dna(const std::vector<Record> &recs) {
-  size_t n = 0;
-  for (const Record &r : recs)
-    for (char c : r.seq) {
-      const char u = (char)toupper((unsigned char)c);
-      if (u != 'A' && u != 'C' && u != 'G' && u != 'T') return false;
-      ++n;
-    }
-  return n > 0;
-}
-
-// CIGAR text -> ops string (M / I / D per column).
-std::string expand(const char *cig, int len) {
-  std::string ops;
-  int v = 0;
-  for (int p = 0; p < len; ++p) {
-    if (isdigit((unsigned char)cig[p])) { v = v * 10 + (cig[p] - '0'); continue; }
-    ops.append((size_t)v, cig[p]);
-    v = 0;
-  }
-  return ops;
-}
-
-// Gapped rows of a global alignment of a (query) and b (reference).
-void render(const std::string &a, const std::string &b, const std::string &ops, std::string &ra,
-            std::string &rb) {
-  size_t i = 0, j = 0;
-  ra.clear();
-  rb.clear();
-  for (char o : ops) {
-    if (o == 'M') { ra += a[i++]; rb += b[j++]; }
-    else if (o == 'I') { ra += a[i++]; rb += '-'; }
-    else { ra += '-'; rb += b[j++]; }
-  }
-}
 
 int fail(const std::string &msg) {
   std::cerr << "tweakalign: " << msg << std::endl;
@@ -134,12 +50,14 @@ struct Aligner {
     std::vector<ta_pair> tp;
     for (auto &q : pairs) tp.push_back({q.first->data(), (int32_t)q.first->size(), q.second->data(), (int32_t)q.second->size()});
     res.resize(tp.size());
+    // One run: a CIGAR run of length L consumes L residues and takes at most
+    // L + 1 <= 2L characters, so 2 (la + lb) bytes per pair always suffice.
+    int64_t cap = 1;
+    for (auto &q : tp) cap += 2 * ((int64_t)q.la + q.lb);
+    cig.assign(p.traceback ? (size_t)cap : 1, 0);
     int64_t used = 0;
-    int rc = ta_align_batch(ctx, &p, tp.data(), (int32_t)tp.size(), res.data(), nullptr, 0, &used);
-    if (rc != TA_OK && rc != TA_E_SPACE) return rc;
-    if (!p.traceback) return TA_OK;
-    cig.assign((size_t)used + 1, 0);
-    return ta_align_batch(ctx, &p, tp.data(), (int32_t)tp.size(), res.data(), cig.data(), used + 1, &used);
+    return ta_align_batch(ctx, &p, tp.data(), (int32_t)tp.size(), res.data(), p.traceback ? cig.data() : nullptr,
+                          p.traceback ? cap : 0, &used);
   }
 };
 
@@ -203,8 +121,14 @@ int main(int argc, char **argv) {
       al.p.traceback = 1;
       rc = al.run(pairs, res, cig);
       if (rc) return fail(std::string("alignment failed: ") + ta_strerror(rc));
+      const std::string ops = expand(cig.data() + res[0].cigar_off, res[0].cigar_len);
+      // A global alignment's ops consume both sequences exactly; anything
+      // else is an engine failure, reported as a failed run (exit status 1,
+      // no output file) rather than written as a misaligned FASTA.
+      if (!consumes(ops, recs[0].seq.size(), recs[1].seq.size()))
+        return fail("alignment failed: the CIGAR does not consume both sequences");
       std::string ra, rb;
-      render(recs[0].seq, recs[1].seq, expand(cig.data() + res[0].cigar_off, res[0].cigar_len), ra, rb);
+      render(recs[0].seq, recs[1].seq, ops, ra, rb);
       outrecs[0].seq = ra;
       outrecs[1].seq = rb;
     } else {

@@ -506,7 +506,7 @@ __global__ __launch_bounds__(64 * LW) void long_tb_kernel(LongArgs A) {
       r.a_begin = 0; r.a_end = P.m - 1; r.b_begin = 0; r.b_end = P.n - 1;
       r.cigar_off = (int64_t)((char *)rw.wp - A.cig);
       r.cigar_len = (int32_t)(A.cig + pd.cig_off + pd.cig_cap - (char *)rw.wp);
-      r.flags = f.flags | (rw.overflow ? 0x100 : 0);
+      r.flags = f.flags | (rw.overflow ? WALK_FAILED : 0);
       A.res[P.pid] = r;
     }
   }

@@ -354,6 +354,7 @@ const char *ta_strerror(int code) {
     case TA_E_CANCELLED: return "request cancelled";
     case TA_E_TICKET: return "unknown ticket";
     case TA_E_NODEVICE: return "no usable GPU";
+    case TA_E_WALK: return "a traceback walk did not complete";
     case TA_E_PENDING: return "pending";
     default: return "unknown error";
   }
@@ -418,6 +419,7 @@ struct HostPair {
   uint64_t a_off, b_off;
   int32_t la, lb;
   int path;  // 3 long-pair pipeline, 2 packed int16, 1 int32, 0 no DP (empty side)
+  bool maysat = false;  // path 2, local: the int16 fill may saturate (int32 rerun then)
 };
 
 // Global pairs of at least this many cells go through the long-pair
@@ -720,6 +722,9 @@ struct ta_batch {
   uint8_t *d_wtb = nullptr;
   uint64_t lring_elems = 0, lck_elems = 0, lcol_elems = 0, wring_stride = 0, wtb_stride = 0;
   int32_t lprog_n = 0, long_wgs = 0, spin_limit = 1 << 20;  // about a second of polling
+  // Most steps a traceback walk may take (TWEAKALIGN_WALK_GUARD lowers it so
+  // that walks end flagged; otherwise above every walk's length).
+  int32_t guard_cap = INT32_MAX;
   // captured run
   bool graph_on = false, phases_timed = false;
   hipGraph_t graph = nullptr;
@@ -731,6 +736,11 @@ struct ta_batch {
   bool plain_ready = false;
   std::vector<Task> plain_tasks;
   Task *d_plain_tasks = nullptr;
+  // int32 reruns of the pairs whose int16 fill may saturate: NGW entries
+  // each (index plain_rerun[k]), their planes' PairTB in d_pptb_r
+  std::vector<int32_t> plain_rerun;
+  Task *d_plain_rtasks = nullptr;
+  PairTB *d_pptb_r = nullptr;
   uint16_t *d_prow = nullptr, *d_pcol = nullptr;
   uint32_t *d_pring = nullptr;
   uint8_t *d_ptbuf = nullptr;
@@ -777,7 +787,7 @@ void ta_batch_destroy(ta_batch *B) {
   dfree(B->d_lp); dfree(B->d_litems); dfree(B->d_lring); dfree(B->d_lck); dfree(B->d_lcol); dfree(B->d_lprog);
   dfree(B->d_wring); dfree(B->d_wtb);
   dfree(B->d_plain_tasks); dfree(B->d_prow); dfree(B->d_pcol); dfree(B->d_pring);
-  dfree(B->d_ptbuf); dfree(B->d_pptb);
+  dfree(B->d_ptbuf); dfree(B->d_pptb); dfree(B->d_plain_rtasks); dfree(B->d_pptb_r);
   for (auto &e : B->ev)
     if (e) (void)hipEventDestroy(e);
   if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
@@ -956,7 +966,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       h.path = 1;  // packed end-cell keys hold the column in 15 bits
     } else if (local) {
       h.path = 2;
-      if (std::min(bound_a, bound_b) > 32767 - 2L * B->alph.maxS) B->may_saturate = true;
+      h.maysat = std::min(bound_a, bound_b) > 32767 - 2L * B->alph.maxS;
+      if (h.maysat) B->may_saturate = true;
     } else {
       h.path = global_i16_safe(*p, B->alph.maxS, q.la, q.lb) ? 2 : 1;
     }
@@ -1095,6 +1106,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       B->long_wgs = std::max(1, std::min((int)lk.size(), 2 * ctx->cus));
     }
     if (const char *e = getenv("TWEAKALIGN_SPIN_LIMIT")) B->spin_limit = std::max(1, atoi(e));
+    if (const char *e = getenv("TWEAKALIGN_WALK_GUARD")) B->guard_cap = std::max(0, atoi(e));
   }
 
   // Upload.
@@ -1190,6 +1202,7 @@ TbArgs tb_args(ta_batch *B) {
   a.res = B->d_res;
   a.local = B->p.mode == TA_LOCAL; a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.do_cigar = B->p.traceback;
+  a.guard_cap = B->guard_cap;
   return a;
 }
 
@@ -1248,6 +1261,7 @@ int launch_traceback(ta_ctx *ctx, ta_batch *B, const PairDev *pairs, const PairT
   a.res = whole ? B->d_res : B->d_res + first;
   a.local = B->p.mode == TA_LOCAL; a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.do_cigar = B->p.traceback;
+  a.guard_cap = B->guard_cap;
   (void)hipGetLastError();
   hipLaunchKernelGGL(traceback_kernel, dim3((count + TB_WALKERS - 1) / TB_WALKERS), dim3(TB_WALKERS), 0,
                      B->stream, a);
@@ -1611,6 +1625,8 @@ int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, 
     if (out[k].cigar_len > 0)
       format_runs(runs + off[k] / 4, res[k].cigar_len / 4, cigar_buf + out[k].cigar_off);
   });
+  for (int32_t k = 0; k < B->n; ++k)
+    if (out[k].flags & TA_F_WALK_FAILED) return TA_E_WALK;
   return rc;
 }
 
@@ -1670,11 +1686,13 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     // NGW entries per pair: the task, then fillers for the other groups of
     // its wave (a group task runs in a wave of NGW groups; fillers write
     // nothing).
-    std::vector<Task> tasks;
-    auto fill_unit = [&]() {
-      Task f = tasks.back();
+    std::vector<Task> tasks, rtasks;
+    std::vector<PairTB> ptb_r(B->n, PairTB{0, 0, 0, 0, 0, 0, 0});
+    std::vector<int32_t> rerun_of(B->n, -1);
+    auto fill_unit = [&](std::vector<Task> &v) {
+      Task f = v.back();
       f.single = 2;
-      for (int q = 1; q < NGW; ++q) tasks.push_back(f);
+      for (int q = 1; q < NGW; ++q) v.push_back(f);
     };
     for (int k = 0; k < B->n; ++k) {
       const HostPair &h = B->hp[k];
@@ -1683,22 +1701,36 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
       if (h.path == 0 || h.path == 3) {
         if (h.path == 3) pl.ptb[k].P = 3;  // the long-pair kernels write its result
         tasks.push_back(Task{});
-        fill_unit();
+        fill_unit(tasks);
         continue;
       }
       const int GG = pick_gg(h.la, h.lb, 0.0, true);
-      add_task(pl, tasks, P, GG, B->trace, B->hp, k, k, true, B->Kp);
       // Every plain launch reuses the same scratch: rebase to 0.
-      Task &T = tasks.back();
-      max_tb = std::max(max_tb, pl.tb_bytes - tb0);
-      max_ring = std::max(max_ring, pl.ring_elems - r0);
-      T.tb_off = 0; T.ring_off = 0;
-      pl.ptb[k].tb_off = 0;
-      pl.tb_bytes = tb0; pl.ring_elems = r0;
-      fill_unit();
+      auto rebase = [&](std::vector<Task> &v) {
+        Task &T = v.back();
+        max_tb = std::max(max_tb, pl.tb_bytes - tb0);
+        max_ring = std::max(max_ring, pl.ring_elems - r0);
+        T.tb_off = 0; T.ring_off = 0;
+        pl.ptb[k].tb_off = 0;
+        pl.tb_bytes = tb0; pl.ring_elems = r0;
+        fill_unit(v);
+      };
+      add_task(pl, tasks, P, GG, B->trace, B->hp, k, k, true, B->Kp);
+      rebase(tasks);
+      if (h.maysat) {  // the int32 task a saturated int16 fill reruns as
+        const PairTB p16 = pl.ptb[k];
+        rerun_of[k] = (int32_t)rtasks.size();
+        add_task(pl, rtasks, 1, GG, B->trace, B->hp, k, k, true, B->Kp);
+        rtasks.back().flags_or = TA_F_RERUN;
+        rebase(rtasks);
+        ptb_r[k] = pl.ptb[k];
+        pl.ptb[k] = p16;
+      }
     }
     int rc = TA_OK;
     rc = rc ? rc : upload(&B->d_plain_tasks, tasks, st);
+    rc = rc ? rc : upload(&B->d_plain_rtasks, rtasks, st);
+    rc = rc ? rc : upload(&B->d_pptb_r, ptb_r, st);
     rc = rc ? rc : expand_codes(pl.cjobs, pl.rows_n, pl.cols_n, B->d_pairs, B->d_codes, B->Kp, &B->d_prow,
                                 &B->d_pcol, st);
     rc = rc ? rc : upload(&B->d_pptb, pl.ptb, st);
@@ -1706,6 +1738,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     rc = rc ? rc : dalloc(&B->d_ptbuf, max_tb);
     if (rc) return rc;
     B->plain_tasks = tasks;
+    B->plain_rerun = rerun_of;
     B->plain_ready = true;
   }
   HIPCHK(hipEventRecord(B->ev[0], st));
@@ -1723,6 +1756,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
       if (rc) return rc;
       continue;
     }
+    const PairTB *ptb = B->d_pptb;
     if (h.path) {
       HIPCHK(hipMemsetAsync(B->d_ctr + 6, 0, 2 * sizeof(int32_t), st));
       const bool wave = B->plain_tasks[NGW * k].S && pick_gg(h.la, h.lb, 0.0, true) == GWAVE;
@@ -1731,8 +1765,22 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
       int rc = h.path == 1 ? launch_fill<1>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1)
                            : launch_fill<2>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1);
       if (rc) return rc;
+      if (B->plain_rerun[k] >= 0) {
+        // The int16 fill may have saturated: as the batched form, rerun the
+        // pair in int32 when it flagged so (a host round trip per such pair).
+        FillOut fo{};
+        HIPCHK(hipMemcpyAsync(&fo, B->d_fo + k, sizeof(FillOut), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (fo.flags & TA_F_RERUN) {
+          HIPCHK(hipMemsetAsync(B->d_ctr + 6, 0, 2 * sizeof(int32_t), st));
+          Task *const rt[2] = {B->d_plain_rtasks + B->plain_rerun[k], B->d_plain_rtasks + B->plain_rerun[k]};
+          rc = launch_fill<1>(ctx, B, rt, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1);
+          if (rc) return rc;
+          ptb = B->d_pptb_r;
+        }
+      }
     }
-    int rc = launch_traceback(ctx, B, B->d_pairs, B->d_pptb, B->d_ptbuf, k, 1);
+    int rc = launch_traceback(ctx, B, B->d_pairs, ptb, B->d_ptbuf, k, 1);
     if (rc) return rc;
   }
   HIPCHK(hipEventRecord(B->ev[1], st));

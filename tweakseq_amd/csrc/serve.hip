@@ -40,6 +40,7 @@ struct Request {
   int64_t tb_bytes = 0;
   RState state = RState::Pending;
   int err = 0;
+  bool walk_failed = false;    // a pair's walk gave up: delivered with TA_E_WALK
   std::vector<ta_result> res;  // cigar_off relative to `cigar`
   std::string cigar;
 };
@@ -108,7 +109,10 @@ void Serve::run_group(Group &g, hipStream_t st) {
       int64_t len = 0;
       for (auto &x : r->res) len += x.cigar_len;
       r->cigar.assign(text.data() + base, (size_t)len);
-      for (auto &x : r->res) x.cigar_off -= base;
+      for (auto &x : r->res) {
+        x.cigar_off -= base;
+        if (x.flags & TA_F_WALK_FAILED) r->walk_failed = true;
+      }
       r->state = RState::Done;
       r->a.clear(); r->a.shrink_to_fit();
       r->b.clear(); r->b.shrink_to_fit();
@@ -288,9 +292,10 @@ int ta_poll(ta_ctx *ctx, uint64_t ticket, ta_result *out, char *cigar_buf, int64
   memcpy(out, r->res.data(), r->res.size() * sizeof(ta_result));
   if ((int64_t)r->cigar.size() > cap || (!cigar_buf && !r->cigar.empty())) return TA_E_SPACE;
   if (!r->cigar.empty()) memcpy(cigar_buf, r->cigar.data(), r->cigar.size());
+  const int rc = r->walk_failed ? TA_E_WALK : TA_OK;
   s->all.erase(it);  // delivered: the ticket is retired
   delete r;
-  return TA_OK;
+  return rc;
 }
 
 int ta_cancel(ta_ctx *ctx, uint64_t ticket) {

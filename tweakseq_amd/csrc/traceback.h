@@ -39,7 +39,11 @@ struct TbArgs {
   ResultDev *res;
   int32_t local, open, ext;
   int32_t do_cigar;
+  int32_t guard_cap;     // most steps a walk may take before it ends flagged (normally above any walk)
 };
+
+// ResultDev::flags of a walk that gave up (TA_F_WALK_FAILED in the C ABI).
+constexpr int32_t WALK_FAILED = 0x100;
 
 // Run-length CIGAR written backwards from the end of the pair's region as
 // one dword per run, (length << 2) | op with op 0 M, 1 D, 2 I (the host turns
@@ -161,7 +165,8 @@ struct Walker {
       k = r0 & 7;
     }
     done = false;
-    guard = 2 * (pd.la + pd.lb) + 2;
+    guard = 2 * (pd.la + pd.lb) + 2;  // a walk takes at most la + lb steps
+    guard = guard < A.guard_cap ? guard : A.guard_cap;
   }
 
   __device__ __forceinline__ int first_block(int t) const {
@@ -193,7 +198,7 @@ struct Walker {
   // Result of the finished walk.
   __device__ __forceinline__ void finish(const TbArgs &A) {
     cw.finish();
-    if (guard <= 0 || cw.overflow) r.flags |= 0x100;
+    if (guard <= 0 || cw.overflow) r.flags |= WALK_FAILED;
     if (A.local) { r.a_begin = i; r.b_begin = j; }
     if (cw.on) {
       r.cigar_off = (int64_t)((char *)cw.wp - A.cig);
@@ -248,7 +253,7 @@ struct Walker {
       if (i == 0) { if (j) cw.emit(1, j); finish(A); return true; }
       if (j == 0) { cw.emit(2, i); finish(A); return true; }
     } else if (i <= 0 || j <= 0) {
-      r.flags |= 0x100;
+      r.flags |= WALK_FAILED;
       finish(A);
       return true;
     }

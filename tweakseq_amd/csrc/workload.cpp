@@ -1,9 +1,11 @@
 // Seeded synthetic workloads (include/tweakalign_workload.h).
 #include "../../include/tweakalign_workload.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -51,6 +53,7 @@ const double kFreq[20] = {.074, .052, .045, .053, .025, .034, .054, .074, .026, 
 struct Alph {
   std::string letters;
   std::vector<uint64_t> cdf;  // cumulative thresholds on 32-bit draws
+  uint8_t first[256];         // per top byte of a draw: the first threshold it can fall under
   explicit Alph(bool dna) {
     if (dna) {
       letters = "ACGT";
@@ -65,10 +68,17 @@ struct Alph {
       }
       cdf.back() = 1ull << 32;
     }
+    // Draws with top byte t are >= t << 24, so no threshold at or below that
+    // can hold them: the scan starts past those (same letter as a full scan).
+    for (int t = 0; t < 256; ++t) {
+      size_t i = 0;
+      while (i + 1 < cdf.size() && cdf[i] <= ((uint64_t)t << 24)) ++i;
+      first[t] = (uint8_t)i;
+    }
   }
   char draw(Rng &r) const {
     const uint64_t u = r.next() >> 32;
-    for (size_t i = 0; i < cdf.size(); ++i)
+    for (size_t i = first[u >> 24]; i < cdf.size(); ++i)
       if (u < cdf[i]) return letters[i];
     return letters.back();
   }
@@ -174,20 +184,40 @@ int ta_workload_create(int32_t config, uint64_t seed, int32_t size, int32_t max_
       break;
     }
     case 4: {
+      // Arrivals, pair counts and modes come from the workload's stream; the
+      // sequences of request q from a stream of their own (seeded by the
+      // workload seed and q), so that requests are generated on all host
+      // threads (1.92 M pairs, ~9.7 GB of residues at full size).
       const int nreq = size ? size : 120000;
       const int lmax = max_len ? max_len : 5000;
       const int lmin = lmax < 50 ? 1 : 50;
       double t = 0;
+      int32_t npairs = 0;
       for (int q = 0; q < nreq; ++q) {
         t += -std::log(1.0 - r.uniform()) / 2000.0;
         int cnt = r.geometric(16.0);
         if (cnt > 256) cnt = 256;
         const int mode = r.uniform() < 0.5 ? TA_GLOBAL : TA_LOCAL;
-        Request rq{t, (int32_t)w->pairs.size(), cnt, mode};
-        for (int k = 0; k < cnt; ++k)
-          add_pair(random_seq(r, prot, (int)r.range(lmin, lmax)), random_seq(r, prot, (int)r.range(lmin, lmax)));
-        w->reqs.push_back(rq);
+        w->reqs.push_back(Request{t, npairs, cnt, mode});
+        npairs += cnt;
       }
+      w->seqs.resize(2 * (size_t)npairs);
+      w->pairs.resize(npairs);
+      for (int32_t k = 0; k < npairs; ++k) w->pairs[k] = {2 * k, 2 * k + 1};
+      const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+      std::vector<std::thread> th;
+      for (unsigned x = 0; x < nth; ++x)
+        th.emplace_back([&, x] {
+          for (int q = (int)x; q < nreq; q += (int)nth) {
+            Rng rq(seed ^ (0x9e3779b97f4a7c15ull * (uint64_t)(q + 1)));
+            const Request &R = w->reqs[q];
+            for (int k = R.first; k < R.first + R.count; ++k) {
+              w->seqs[2 * (size_t)k] = random_seq(rq, prot, (int)rq.range(lmin, lmax));
+              w->seqs[2 * (size_t)k + 1] = random_seq(rq, prot, (int)rq.range(lmin, lmax));
+            }
+          }
+        });
+      for (auto &x : th) x.join();
       break;
     }
     case 5: {
