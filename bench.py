@@ -355,6 +355,15 @@ def main():
         out["single_pair_ms"] = round(lat[len(lat) // 2] * 1e3, 4)
         out["single_pair_gcups"] = round(w1.cells() / lat[len(lat) // 2] / 1e9, 3)
         log("single 300x320 pair: median %.3f ms" % out["single_pair_ms"])
+        # Call -> CIGAR of the same pair through the C ABI, timed in C: the
+        # single-pair path (ta_align_pair) and ta_align_batch with n = 1.
+        import subprocess
+        pb = os.path.join(ROOT, "tweakseq_amd", "tweakalign-pair-bench")
+        if os.path.exists(pb):
+            r = subprocess.run([pb, "2000"], capture_output=True, text=True, timeout=300)
+            if r.returncode == 0:
+                out["single_pair_call"] = json.loads(r.stdout.strip().splitlines()[-1])
+                log("single pair call -> CIGAR:", out["single_pair_call"])
         # Long DNA pairs: strips pipelined across CUs + checkpointed traceback.
         w5 = ta.Workload(5)
         b5 = eng.batch(w5.pairs_struct(), w5.params)

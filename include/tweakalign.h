@@ -107,6 +107,17 @@ void ta_destroy(ta_ctx *ctx);
 int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n,
                    ta_result *out, char *cigar_buf, int64_t cap, int64_t *used);
 
+/* One pair at low latency: the editor's "realign selection" of two rows
+ * (tweakseq/UI/SeqEditMainWin.cpp:1616-1663 runs an external aligner on the
+ * selection; tweakseq/Core/Project.cpp:964-1026 splices its output back).
+ * One kernel launch with the letters read from mapped host staging, the
+ * direction codes kept on chip, and the result written back to staging; no
+ * device allocation.  Same result and conventions as ta_align_batch with
+ * n = 1 (which pairs too large for the on-chip form, or with an empty side,
+ * go through).  Results carry TA_F_I32 (the pair kernel computes in int32). */
+int ta_align_pair(ta_ctx *ctx, const ta_params *p, const char *a, int32_t la, const char *b, int32_t lb,
+                  ta_result *out, char *cigar_buf, int64_t cap, int64_t *used);
+
 /* Device-resident batch: inputs encoded and uploaded once, the alignment
  * run as often as wanted with inputs already in HBM.  No counterpart in the
  * reference (it starts one process per job). */

@@ -3,6 +3,7 @@
   libtweakalign.so  HIP kernels (gfx950) + host runtime + C ABI
   tweakalign        CLI honouring the external-aligner "-in FILE -out FILE" contract
   tweakalign-serve-bench   open-loop serving benchmark
+  tweakalign-pair-bench    call -> CIGAR latency of one pair (single-pair path and batch path)
   tweakalign-valu-peak     measured VALU throughput of the fill's instruction kinds
 
 Usage: python -m tweakseq_amd.build   (or build() from __graft_entry__)
@@ -37,7 +38,7 @@ def run(cmd):
 
 
 def build_lib():
-    srcs = [os.path.join(CSRC, f) for f in ("runtime.hip", "serve.hip", "consensus.hip", "msa.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("runtime.hip", "serve.hip", "consensus.hip", "msa.hip", "pair.hip")]
     objs = []
     for s in srcs:
         o = os.path.join(CSRC, os.path.basename(s) + ".o")
@@ -55,7 +56,8 @@ def build_lib():
 
 def build_tools():
     """Executables linked against libtweakalign.so (rpath = package dir)."""
-    tools = [("cli.cpp", CLI), ("serve_bench.cpp", os.path.join(HERE, "tweakalign-serve-bench"))]
+    tools = [("cli.cpp", CLI), ("serve_bench.cpp", os.path.join(HERE, "tweakalign-serve-bench")),
+             ("pair_bench.cpp", os.path.join(HERE, "tweakalign-pair-bench"))]
     for src, out in tools:
         src = os.path.join(CSRC, src)
         if not os.path.exists(src):

@@ -44,21 +44,6 @@ struct Aligner {
   ta_ctx *ctx = nullptr;
   ta_params p{};
   ~Aligner() { if (ctx) ta_destroy(ctx); }
-
-  int run(const std::vector<std::pair<const std::string *, const std::string *>> &pairs, std::vector<ta_result> &res,
-          std::vector<char> &cig) {
-    std::vector<ta_pair> tp;
-    for (auto &q : pairs) tp.push_back({q.first->data(), (int32_t)q.first->size(), q.second->data(), (int32_t)q.second->size()});
-    res.resize(tp.size());
-    // One run: a CIGAR run of length L consumes L residues and takes at most
-    // L + 1 <= 2L characters, so 2 (la + lb) bytes per pair always suffice.
-    int64_t cap = 1;
-    for (auto &q : tp) cap += 2 * ((int64_t)q.la + q.lb);
-    cig.assign(p.traceback ? (size_t)cap : 1, 0);
-    int64_t used = 0;
-    return ta_align_batch(ctx, &p, tp.data(), (int32_t)tp.size(), res.data(), p.traceback ? cig.data() : nullptr,
-                          p.traceback ? cap : 0, &used);
-  }
 };
 
 }  // namespace
@@ -115,13 +100,18 @@ int main(int argc, char **argv) {
     if (rc) return fail(std::string("GPU: ") + ta_strerror(rc));
     const size_t N = recs.size();
     if (N == 2) {
-      std::vector<std::pair<const std::string *, const std::string *>> pairs{{&recs[0].seq, &recs[1].seq}};
-      std::vector<ta_result> res;
-      std::vector<char> cig;
+      // Two rows (the editor's "realign selection"): the single-pair path.
+      // A CIGAR run of length L consumes L residues in at most L + 1 <= 2L
+      // characters, so 2 (la + lb) bytes always suffice: one call.
+      const std::string &sa = recs[0].seq, &sb = recs[1].seq;
+      ta_result res{};
+      std::vector<char> cig(2 * (sa.size() + sb.size()) + 1);
+      int64_t used = 0;
       al.p.traceback = 1;
-      rc = al.run(pairs, res, cig);
+      rc = ta_align_pair(al.ctx, &al.p, sa.data(), (int32_t)sa.size(), sb.data(), (int32_t)sb.size(), &res,
+                         cig.data(), (int64_t)cig.size(), &used);
       if (rc) return fail(std::string("alignment failed: ") + ta_strerror(rc));
-      const std::string ops = expand(cig.data() + res[0].cigar_off, res[0].cigar_len);
+      const std::string ops = expand(cig.data() + res.cigar_off, res.cigar_len);
       // A global alignment's ops consume both sequences exactly; anything
       // else is an engine failure, reported as a failed run (exit status 1,
       // no output file) rather than written as a misaligned FASTA.

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/tweakalign.h"
+#include "blosum62.h"
 #include "common.h"
 #include "fill.h"
 #include "long.h"
@@ -33,34 +34,9 @@ using namespace ta;
 // tweakseq/Core/Annotations/Consensus.cpp:34-69 (same numbers, by necessity).
 namespace {
 
-const char kProtOrder[] = "ARNDCQEGHILKMFPSTWYVBZX";
-// Lower triangle, row by row (row r has r + 1 entries).
-const int8_t kBlosumLower[] = {
-    4,
-    -1, 5,
-    -2, 0, 6,
-    -2, -2, 1, 6,
-    0, -3, -3, -3, 9,
-    -1, 1, 0, 0, -3, 5,
-    -1, 0, 0, 2, -4, 2, 5,
-    0, -2, 0, -1, -3, -2, -2, 6,
-    -2, 0, 1, -1, -3, 0, 0, -2, 8,
-    -1, -3, -3, -3, -1, -3, -3, -4, -3, 4,
-    -1, -2, -3, -4, -1, -2, -3, -4, -3, 2, 4,
-    -1, 2, 0, -1, -3, 1, 1, -2, -1, -3, -2, 5,
-    -1, -1, -2, -3, -1, 0, -2, -3, -2, 1, 2, -1, 5,
-    -2, -3, -3, -3, -2, -3, -3, -3, -1, 0, 0, -3, 0, 6,
-    -1, -2, -2, -1, -3, -1, -1, -2, -2, -3, -3, -1, -2, -4, 7,
-    1, -1, 1, 0, -1, 0, 0, 0, -1, -2, -2, 0, -1, -2, -1, 4,
-    0, -1, 0, -1, -1, -1, -1, -2, -2, -1, -1, -1, -1, -2, -1, 1, 5,
-    -3, -3, -4, -4, -2, -2, -3, -2, -2, -3, -2, -3, -1, 1, -4, -3, -2, 11,
-    -2, -2, -2, -3, -2, -1, -2, -3, 2, -1, -1, -2, -1, 3, -3, -2, -2, 2, 7,
-    0, -3, -3, -3, -1, -2, -2, -3, -3, 3, 1, -2, 1, -1, -2, -2, 0, -3, -1, 4,
-    -2, -1, 3, 4, -3, 0, 1, -1, 0, -3, -4, 0, -3, -3, -2, 0, -1, -4, -3, -3, 4,
-    -1, 0, 0, 1, -3, 3, 4, -2, 0, -3, -3, 1, -1, -3, -1, 0, -1, -3, -2, -2, 1, 4,
-    0, -1, -1, -1, -2, -1, -1, -1, -1, -1, -1, -1, -1, -1, -2, 0, 0, -2, -1, -1, -1, -1, -1,
-};
-constexpr int kProtK = 23;
+const char kProtOrder[] = TA_PROT_ORDER;
+const int8_t kBlosumLower[] = {TA_BLOSUM62_LOWER};
+constexpr int kProtK = TA_PROT_K;
 
 struct Alphabet {
   int K = 0;
@@ -407,8 +383,24 @@ void ta_destroy(ta_ctx *c) {
   if (!c) return;
   serve_shutdown(c);
   (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->pair_stage) (void)hipHostFree(c->pair_stage);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+int params_check(const ta_params *p) {
+  if (!p) return TA_E_INVALID;
+  if (p->mode != TA_GLOBAL && p->mode != TA_LOCAL) return TA_E_INVALID;
+  if (p->alphabet != TA_PROTEIN && p->alphabet != TA_DNA) return TA_E_INVALID;
+  if (p->gap_open > 0 || p->gap_extend > 0 || p->gap_open < -1000 || p->gap_extend < -1000) return TA_E_INVALID;
+  if (p->alphabet == TA_DNA && (p->match < -100 || p->match > 100 || p->mismatch < -100 || p->mismatch > 100))
+    return TA_E_INVALID;
+  if (p->lanes != TA_LANES_AUTO && p->lanes != TA_LANES_I16 && p->lanes != TA_LANES_I32 &&
+      p->lanes != TA_LANES_LONG)
+    return TA_E_INVALID;
+  if (p->lanes == TA_LANES_LONG && p->mode != TA_GLOBAL) return TA_E_INVALID;
+  return TA_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -803,16 +795,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
                     ta_batch **out, const ta_tuning *force) {
   if (!ctx || !p || !out || n < 0 || (n > 0 && !pairs)) return TA_E_INVALID;
   *out = nullptr;
-  if (p->mode != TA_GLOBAL && p->mode != TA_LOCAL) return TA_E_INVALID;
-  if (p->alphabet != TA_PROTEIN && p->alphabet != TA_DNA) return TA_E_INVALID;
-  if (p->gap_open > 0 || p->gap_extend > 0 || p->gap_open < -1000 || p->gap_extend < -1000)
-    return TA_E_INVALID;
-  if (p->alphabet == TA_DNA && (p->match < -100 || p->match > 100 || p->mismatch < -100 || p->mismatch > 100))
-    return TA_E_INVALID;
-  if (p->lanes != TA_LANES_AUTO && p->lanes != TA_LANES_I16 && p->lanes != TA_LANES_I32 &&
-      p->lanes != TA_LANES_LONG)
-    return TA_E_INVALID;
-  if (p->lanes == TA_LANES_LONG && p->mode != TA_GLOBAL) return TA_E_INVALID;
+  if (params_check(p)) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
 
   ta_batch *B = new ta_batch();
@@ -1497,8 +1480,7 @@ int ta_batch_timings(ta_batch *B, double *ms, int32_t cap) {
 }
 
 // Binary runs ((len << 2) | op, op 0 M / 1 D / 2 I) -> CIGAR text.
-namespace {
-int64_t format_runs(const uint32_t *runs, int n, char *out) {
+int64_t cigar_text(const uint32_t *runs, int n, char *out) {
   static const char ops[4] = {'M', 'D', 'I', '?'};
   int64_t pos = 0;
   char tmp[12];
@@ -1515,6 +1497,8 @@ int64_t format_runs(const uint32_t *runs, int n, char *out) {
   }
   return pos;
 }
+namespace {
+int64_t format_runs(const uint32_t *runs, int n, char *out) { return cigar_text(runs, n, out); }
 }  // namespace
 
 namespace {

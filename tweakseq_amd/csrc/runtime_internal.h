@@ -22,7 +22,18 @@ struct ta_ctx {
   Serve *serve = nullptr;
   std::mutex tune_mu;                // the serving thread creates batches too
   std::map<int, ta_tuning> tune;     // tune_key -> tuning
+  // ta_align_pair's host staging (pinned, mapped: the kernel reads the
+  // letters and writes the result there), grown on demand, kept until
+  // ta_destroy.
+  void *pair_stage = nullptr;
+  size_t pair_stage_cap = 0;
 };
+
+// TA_OK, or TA_E_INVALID for parameters outside what the kernels take.
+int params_check(const ta_params *p);
+// CIGAR text of binary runs ((len << 2) | op, op 0 M / 1 D / 2 I); with out
+// == nullptr only the length.  Returns the text length.
+int64_t cigar_text(const uint32_t *runs, int n, char *out);
 
 // Key of a kind of batch in ta_ctx::tune.
 inline int tune_key(const ta_params *p, int bucket) {

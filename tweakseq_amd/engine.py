@@ -58,6 +58,8 @@ def lib():
         "ta_create": ([P(vp), ctypes.c_int], ctypes.c_int),
         "ta_destroy": ([vp], None),
         "ta_align_batch": ([vp, P(Params), P(Pair), i32, P(CResult), ctypes.c_char_p, i64, P(i64)], ctypes.c_int),
+        "ta_align_pair": ([vp, P(Params), ctypes.c_char_p, i32, ctypes.c_char_p, i32, P(CResult), ctypes.c_char_p,
+                           i64, P(i64)], ctypes.c_int),
         "ta_batch_create": ([vp, P(Params), P(Pair), i32, P(vp)], ctypes.c_int),
         "ta_batch_run": ([vp, vp], ctypes.c_int),
         "ta_batch_run_plain": ([vp, vp, i32, i32], ctypes.c_int),
@@ -228,6 +230,18 @@ class Engine:
             return b.results()
         finally:
             b.close()
+
+    def align_pair(self, a, b, params):
+        """One pair through ta_align_pair (the low-latency path); a Result."""
+        a = a.encode() if isinstance(a, str) else bytes(a)
+        b = b.encode() if isinstance(b, str) else bytes(b)
+        res = CResult()
+        cap = 2 * (len(a) + len(b)) + 16
+        buf = ctypes.create_string_buffer(cap)
+        used = ctypes.c_int64()
+        _check(lib().ta_align_pair(self.ctx, ctypes.byref(params), a, len(a), b, len(b), ctypes.byref(res), buf, cap,
+                                   ctypes.byref(used)), "ta_align_pair")
+        return _collect([res], buf, 1, params.traceback)[0]
 
     def consensus(self, rows, plurality=-1.0):
         """Column consensus of aligned rows (equal lengths; bytes or str) on
