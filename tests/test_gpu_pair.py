@@ -44,10 +44,12 @@ def eng():
 
 def check(eng, pairs, label, **kw):
     params = ta.engine.make_params(**kw)
+    # the oracle walks (and so gives local begin coordinates) only with a
+    # traceback: ask it for one, compare the CIGAR only when it was asked for
     p = po.params(mode=params.mode, alphabet=params.alphabet, gap_open=params.gap_open,
                   gap_extend=params.gap_extend, match=params.match, mismatch=params.mismatch,
-                  traceback=params.traceback)
-    want = po.align_pairs(p, pairs)
+                  traceback=1)
+    want = [w if params.traceback else w[:5] + ("",) for w in po.align_pairs(p, pairs)]
     bad = []
     for k, ((a, b), w) in enumerate(zip(pairs, want)):
         g = eng.align_pair(a, b, params)

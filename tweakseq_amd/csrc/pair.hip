@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "../../include/tweakalign.h"
 #include "blosum62.h"
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void pair_kernel(PairArgs A) {
   int2 *ring = (int2 *)(lds + Y.ring);
   uint8_t *planes = lds + Y.planes;
   uint32_t *runs = (uint32_t *)(lds + Y.runs);
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int m = A.m, n = A.n, T = Y.T, S = Y.S;
   const int open = A.open, ext = A.ext;
 
@@ -121,17 +122,31 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void pair_kernel(PairArgs A) {
     auto left = [&](int i) { return (LOCAL || i == 0) ? 0 : open + (i - 1) * ext; };  // H(i, 0), 1-based
     int H0 = left(r0 + 1), H1 = left(r0 + 2), E0 = NEG32, E1 = NEG32;
     int hd = left(r0);  // H(r0, c - 1) of the step to come (1-based row r0 = the row above)
-    int Hbot = 0, Fbot = NEG32, code = PAIR_PAD;
-    int rb0 = 0, rc0 = 0, rb1 = 0, rc1 = 0;  // local: best H of each row and its first column
+    int Hbot = 0, Fbot = NEG32;
+    // Local: one key per row, (H << 16) | (0xFFFF - column): the maximum is
+    // the best H at its first column (columns and H stay below 2^16 here).
+    uint32_t key0 = 0, key1 = 0;
     const int2 *rin = ring + ((s + PAIR_WAVES - 1) % PAIR_WAVES) * Y.ring_len;
     int2 *rout = ring + (s % PAIR_WAVES) * Y.ring_len;
     int32_t *pin = sy + SY_PROG + (s + PAIR_WAVES - 1) % PAIR_WAVES, *pout = sy + SY_PROG + s % PAIR_WAVES;
     const bool wr = s + 1 < S;
-    uint8_t *pl = planes + (size_t)s * T * 64 + lane;
-    for (int t = 0; t < T; ++t) {
-      if (s > 0 && (t & 7) == 0) {
-        // ring columns [t, t + 8) of the strip above: wait for them (bounded)
-        const int need = ((s) << 20) | (t + 8 < n ? t + 8 : n);
+    uint32_t *pl = (uint32_t *)(planes + (size_t)s * T * 64) + lane;  // [t / 4][lane] dwords, byte t % 4
+    // the column code of step 0 (lane 0: column 0; the others idle) and its
+    // scores; later steps read theirs one step ahead
+    int code = wave_shr1(bcode[0], PAIR_PAD);
+    int s0n = t0[code], s1n = t1[code];
+    uint32_t pk = 0;  // plane bytes of the current 4 steps
+    for (int tb = 0; tb < T; tb += 8) {
+      int topH[8], topF[8];
+      if (s == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          topH[q] = LOCAL ? 0 : open + (tb + q) * ext;  // H(0, tb + q + 1)
+          topF[q] = NEG32;
+        }
+      } else {
+        // ring columns [tb, tb + 8) of the strip above: wait for them (bounded)
+        const int need = (s << 20) | (tb + 8 < n ? tb + 8 : n);
         int spins = 0;
         while (__hip_atomic_load(pin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
           if (++spins > A.spin_limit) {
@@ -140,57 +155,90 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void pair_kernel(PairArgs A) {
           }
           __builtin_amdgcn_s_sleep(1);
         }
-      }
-      int topH, topF;
-      if (s == 0) {
-        topH = LOCAL ? 0 : open + t * ext;  // H(0, t + 1)
-        topF = NEG32;
-      } else {
-        const int2 v = rin[t < n ? t : n - 1];
-        topH = v.x;
-        topF = v.y;
-      }
-      code = wave_shr1(bcode[t], code);
-      const int Hup = wave_shr1(topH, Hbot), Fup = wave_shr1(topF, Fbot);
-      const int c0 = t - lane;  // 0-based column of this lane at this step
-      if (c0 >= 0 && c0 < n) {
-        const int S0 = t0[code], S1 = t1[code];
-        // row r0 + 1 (1-based)
-        const int HdS0 = hd + S0;
-        const int Eo0 = H0 + open, Ee0 = E0 + ext, En0 = max(Eo0, Ee0);
-        const int Fo0 = Hup + open, Fe0 = Fup + ext, Fn0 = max(Fo0, Fe0);
-        int Hn0 = max(HdS0, max(En0, Fn0));
-        if (LOCAL) Hn0 = max(Hn0, 0);
-        int k0 = (Ee0 > Eo0) | ((Fe0 > Fo0) << 1) | ((HdS0 < Hn0) << 2) | ((Fn0 > En0) << 3);
-        // row r0 + 2: the row above is the one just computed
-        const int HdS1 = H0 + S1;
-        const int Eo1 = H1 + open, Ee1 = E1 + ext, En1 = max(Eo1, Ee1);
-        const int Fo1 = Hn0 + open, Fe1 = Fn0 + ext, Fn1 = max(Fo1, Fe1);
-        int Hn1 = max(HdS1, max(En1, Fn1));
-        if (LOCAL) Hn1 = max(Hn1, 0);
-        int k1 = (Ee1 > Eo1) | ((Fe1 > Fo1) << 1) | ((HdS1 < Hn1) << 2) | ((Fn1 > En1) << 3);
-        if (LOCAL) {
-          k0 = Hn0 == 0 ? 8 : k0;
-          k1 = Hn1 == 0 ? 8 : k1;
-          if (Hn0 > rb0) { rb0 = Hn0; rc0 = c0 + 1; }
-          if (Hn1 > rb1) { rb1 = Hn1; rc1 = c0 + 1; }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int2 v = rin[tb + q < n ? tb + q : n - 1];
+          topH[q] = v.x;
+          topF[q] = v.y;
         }
-        pl[(size_t)t * 64] = (uint8_t)(k0 | (k1 << 4));
-        if (wr && lane == 63) rout[c0] = make_int2(Hn1, Fn1);
-        hd = Hup;
-        H0 = Hn0; H1 = Hn1; E0 = En0; E1 = En1;
-        Hbot = Hn1; Fbot = Fn1;
       }
-      if (wr && (((t + 1) & 7) == 0 || t + 1 == T)) {
-        int done = t - 62;  // lane 63 has stored columns 0 .. t - 63
+      const uint2 bw0 = *(const uint2 *)(bcode + tb), bw1 = *(const uint2 *)(bcode + tb + 8);
+      const uint32_t bw[4] = {bw0.x, bw0.y, bw1.x, bw1.y};
+      // The 8 steps, in two forms: with the per-lane column test, and without
+      // it where every lane is inside the columns (most blocks).
+      auto steps = [&](auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int t = tb + q;
+          const int S0 = s0n, S1 = s1n;
+          // the next step's column code (lane l takes lane l-1's) and scores
+          const int bnext = (int)((bw[(q + 1) >> 2] >> (8 * ((q + 1) & 3))) & 0xff);
+          const int cnext = wave_shr1(bnext, code);
+          s0n = t0[cnext];
+          s1n = t1[cnext];
+          code = cnext;
+          const int Hup = wave_shr1(topH[q], Hbot), Fup = wave_shr1(topF[q], Fbot);
+          const int c0 = t - lane;  // 0-based column of this lane at this step
+          if (FULL || (c0 >= 0 && c0 < n)) {
+            // Direction bits are sign bits of differences (|values| < 2^30),
+            // pushed into acc: e0 f0 d0 x0 e1 f1 d1 x1 from bit 0 up.  x (F
+            // above E) is kept only where d (H not from the diagonal) is set,
+            // so that d = 0, x = 1 marks a local stop cell (H = 0), whose
+            // other bits no walk reads.
+            uint32_t acc = 0;
+            // row r0 + 1 (1-based)
+            const int HdS0 = hd + S0;
+            const int Eo0 = H0 + open, Ee0 = E0 + ext, En0 = max(Eo0, Ee0);
+            const int Fo0 = Hup + open, Fe0 = Fup + ext, Fn0 = max(Fo0, Fe0);
+            int Hn0 = max(HdS0, max(En0, Fn0));
+            if (LOCAL) Hn0 = max(Hn0, 0);
+            // row r0 + 2: the row above is the one just computed
+            const int HdS1 = H0 + S1;
+            const int Eo1 = H1 + open, Ee1 = E1 + ext, En1 = max(Eo1, Ee1);
+            const int Fo1 = Hn0 + open, Fe1 = Fn0 + ext, Fn1 = max(Fo1, Fe1);
+            int Hn1 = max(HdS1, max(En1, Fn1));
+            if (LOCAL) Hn1 = max(Hn1, 0);
+            const uint32_t dD0 = (uint32_t)(HdS0 - Hn0), dD1 = (uint32_t)(HdS1 - Hn1);
+            uint32_t xd0 = (uint32_t)(En0 - Fn0) & dD0, xd1 = (uint32_t)(En1 - Fn1) & dD1;
+            uint32_t dd0 = dD0, dd1 = dD1;
+            if (LOCAL) {  // H >= 0: H = 0 sets x and clears d
+              xd0 |= (uint32_t)(Hn0 - 1); xd1 |= (uint32_t)(Hn1 - 1);
+              dd0 &= (uint32_t)(-Hn0); dd1 &= (uint32_t)(-Hn1);
+              key0 = max(key0, ((uint32_t)Hn0 << 16) | (uint32_t)(0xFFFF - c0));
+              key1 = max(key1, ((uint32_t)Hn1 << 16) | (uint32_t)(0xFFFF - c0));
+            }
+            acc = __builtin_amdgcn_alignbit(acc, xd1, 31);
+            acc = __builtin_amdgcn_alignbit(acc, dd1, 31);
+            acc = __builtin_amdgcn_alignbit(acc, (uint32_t)(Fo1 - Fe1), 31);
+            acc = __builtin_amdgcn_alignbit(acc, (uint32_t)(Eo1 - Ee1), 31);
+            acc = __builtin_amdgcn_alignbit(acc, xd0, 31);
+            acc = __builtin_amdgcn_alignbit(acc, dd0, 31);
+            acc = __builtin_amdgcn_alignbit(acc, (uint32_t)(Fo0 - Fe0), 31);
+            acc = __builtin_amdgcn_alignbit(acc, (uint32_t)(Eo0 - Ee0), 31);
+            pk = (pk & ~(0xffu << (8 * (q & 3)))) | ((acc & 0xffu) << (8 * (q & 3)));
+            if (wr && lane == 63) rout[c0] = make_int2(Hn1, Fn1);
+            hd = Hup;
+            H0 = Hn0; H1 = Hn1; E0 = En0; E1 = En1;
+            Hbot = Hn1; Fbot = Fn1;
+          }
+          if ((q & 3) == 3) pl[(size_t)(t >> 2) * 64] = pk;
+        }
+      };
+      if (tb >= 63 && tb + 8 <= n) steps(std::true_type{});
+      else steps(std::false_type{});
+      if (wr) {
+        int done = tb + 8 - 63;  // lane 63 has stored columns 0 .. tb + 7 - 63
         done = done < 0 ? 0 : (done > n ? n : done);
         if (lane == 63)
           __hip_atomic_store(pout, ((s + 1) << 20) | done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
     if (LOCAL) {
-      if (r0 < m && better(rb0, r0 + 1, rc0, bh, bi, bj) && rb0 > 0) { bh = rb0; bi = r0 + 1; bj = rc0; }
-      if (r0 + 1 < m && better(rb1, r0 + 2, rc1, bh, bi, bj) && rb1 > 0) { bh = rb1; bi = r0 + 2; bj = rc1; }
+      const int rb0 = (int)(key0 >> 16), rc0 = 0x10000 - (int)(key0 & 0xffff);  // 1-based column
+      const int rb1 = (int)(key1 >> 16), rc1 = 0x10000 - (int)(key1 & 0xffff);
+      if (r0 < m && rb0 > 0 && better(rb0, r0 + 1, rc0, bh, bi, bj)) { bh = rb0; bi = r0 + 1; bj = rc0; }
+      if (r0 + 1 < m && rb1 > 0 && better(rb1, r0 + 2, rc1, bh, bi, bj)) { bh = rb1; bi = r0 + 2; bj = rc1; }
     } else {
       if (r0 + 1 == m) sy[SY_SCORE] = H0;
       if (r0 + 2 == m) sy[SY_SCORE] = H1;
@@ -240,7 +288,7 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void pair_kernel(PairArgs A) {
   auto flush = [&]() {
     if (len) {
       --wp;
-      if (lane == 0) runs[wp] = ((uint32_t)len << 2) | (uint32_t)op;
+      runs[wp] = ((uint32_t)len << 2) | (uint32_t)op;  // every lane stores the same word
     }
   };
   auto emit = [&](int o, int k) {
@@ -258,20 +306,25 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void pair_kernel(PairArgs A) {
       int byte = 0;
       if (rp >= 0 && cj >= 0) {
         const int ss = rp >> 6, ll = rp & 63;
-        byte = planes[((size_t)ss * T + cj + ll) * 64 + ll];
+        const int t = cj + ll;
+        byte = planes[((size_t)ss * T + (t & ~3)) * 64 + 4 * ll + (t & 3)];
       }
       v[h] = byte;
     }
+    // Steps inside the window, all on wave-uniform (scalar) values: the
+    // code by readlane, the transition by selects.
     for (;;) {
-      if (i == 0 || j == 0) break;
       const int dyp = rp0 - ((i - 1) >> 1), dx = cw0 - (j - 1);
-      if (dyp > 7 || dx > 15) break;
-      const int byte = __builtin_amdgcn_readlane(dyp < 4 ? v[0] : v[1], ((dyp & 3) << 4) | dx);
-      const int cd = (byte >> (((i - 1) & 1) << 2)) & 15;
+      if (i == 0 || j == 0 || dyp > 7 || dx > 15) break;
+      const int lo = __builtin_amdgcn_readlane(v[0], ((dyp & 3) << 4) | dx);
+      const int hi = __builtin_amdgcn_readlane(v[1], ((dyp & 3) << 4) | dx);
+      const int cd = ((dyp < 4 ? lo : hi) >> (((i - 1) & 1) << 2)) & 15;
       if (LOCAL && state == 0 && (cd & 12) == 8) { stop = true; break; }  // entered a cell whose H is 0
-      const int g = state ? state : ((cd & 4) ? ((cd & 8) ? 2 : 1) : 0);
-      const int xb = g == 1 ? (cd & 1) : (g == 2 ? ((cd >> 1) & 1) : 0);
-      emit(g, 1);
+      const int src = ((cd >> 2) & 1) * (1 + ((cd >> 3) & 1));  // H-source: 0 diag, 1 E, 2 F
+      const int g = state ? state : src;
+      const int xb = g ? (cd >> (g - 1)) & 1 : 0;              // E: bit 0, F: bit 1 (extend)
+      if (g != op) { flush(); op = g; len = 0; }
+      ++len;
       state = xb ? g : 0;
       j -= g != 2;
       i -= g != 1;
@@ -367,6 +420,10 @@ bool pair_path(const ta_ctx *ctx, const ta_params *p, int32_t la, int32_t lb) {
   if (la <= 0 || lb <= 0 || p->lanes == TA_LANES_LONG) return false;
   if (p->mode == TA_GLOBAL && (int64_t)(la + lb + 2) * -p->gap_extend + 2 * (int64_t)-p->gap_open >= (1ll << 28))
     return false;  // scores near the int32 -infinity: the batch path reports it
+  if (p->mode == TA_LOCAL) {  // the kernel's end-cell keys hold H in 16 bits
+    const int maxS = p->alphabet == TA_PROTEIN ? 11 : std::max(0, std::max(p->match, p->mismatch));
+    if ((int64_t)std::min(la, lb) * maxS >= 65535) return false;
+  }
   return pair_layout(la, lb).bytes <= ctx->lds_max;
 }
 
