@@ -1,8 +1,9 @@
 """A traceback walk that gives up is an error at the boundary, never a
 silent short CIGAR.  The walk's step budget is lowered from the host
-(TWEAKALIGN_WALK_GUARD, read when a batch is created), so every walk ends
+(TWEAKALIGN_WALK_GUARD, read when a batch is created or a single-pair call
+starts), so every walk ends
 normally, flagged TA_F_WALK_FAILED, and each entry point reports TA_E_WALK:
-ta_align_batch, ta_batch_results, ta_poll, and the CLI, which then exits 1
+ta_align_batch, ta_batch_results, ta_poll, ta_align_pair, and the CLI, which then exits 1
 and writes no output file (the reference shows "Alignment not completed",
 tweakseq/UI/SeqEditMainWin.cpp:852-856)."""
 import ctypes
@@ -65,6 +66,17 @@ def test_align_batch_reports_walk_failure(short_walks, mode):
         with pytest.raises(RuntimeError, match="walk"):
             b.results()
         b.close()
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("mode", [ta.GLOBAL, ta.LOCAL])
+def test_align_pair_reports_walk_failure(short_walks, mode):
+    eng = ta.Engine(0)
+    try:
+        a, b = pairs_of(11 + mode, n=1)[0]
+        with pytest.raises(RuntimeError, match="walk"):
+            eng.align_pair(a, b, ta.engine.make_params(mode=mode))
     finally:
         eng.close()
 

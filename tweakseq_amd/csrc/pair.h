@@ -37,6 +37,7 @@ struct PairArgs {
   int32_t alphabet, match, mismatch;
   int32_t open, ext, local, traceback;
   int32_t spin_limit;
+  int32_t walk_cap;         // most steps the walk may take before it ends flagged (above any walk normally)
   PairLayout lay;
   PairOut *out;             // device-writable; m + n + 2 run slots follow it
 };

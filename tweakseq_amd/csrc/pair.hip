@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(64 * PAIR_WAVES) void pair_kernel(PairArgs A) {
       state = xb ? g : 0;
       j -= g != 2;
       i -= g != 1;
-      if (++steps > m + n) { fail = true; break; }
+      if (++steps > A.walk_cap) { fail = true; break; }
     }
   }
   if (walk && !fail && !LOCAL) {
@@ -434,6 +435,8 @@ PairArgs pair_args(const ta_params *p, int32_t la, int32_t lb) {
   A.open = p->gap_open; A.ext = p->gap_extend;
   A.local = p->mode == TA_LOCAL; A.traceback = p->traceback != 0;
   A.spin_limit = 1 << 22;
+  A.walk_cap = la + lb;  // a walk takes at most la + lb steps
+  if (const char *e = getenv("TWEAKALIGN_WALK_GUARD")) A.walk_cap = std::min(A.walk_cap, std::max(0, atoi(e)));
   A.lay = pair_layout(la, lb);
   return A;
 }
