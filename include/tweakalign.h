@@ -35,6 +35,7 @@ extern "C" {
 #define TA_E_NODEVICE -9      /* no usable GPU */
 #define TA_E_WALK -10         /* a traceback walk did not complete: the pairs concerned carry
                                  TA_F_WALK_FAILED; their coordinates and CIGAR are not valid */
+#define TA_E_LIMIT -11        /* input beyond the fixed size of ta_align_pair_static's scratch */
 #define TA_E_PENDING 1        /* ta_poll: not finished yet (not an error) */
 
 #define TA_GLOBAL 0           /* Needleman-Wunsch (Gotoh affine gaps) */
@@ -117,6 +118,20 @@ int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_
  * go through).  Results carry TA_F_I32 (the pair kernel computes in int32). */
 int ta_align_pair(ta_ctx *ctx, const ta_params *p, const char *a, int32_t la, const char *b, int32_t lb,
                   ta_result *out, char *cigar_buf, int64_t cap, int64_t *used);
+
+/* ta_align_pair without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The letters go to, and the result
+ * comes back from, a fixed device array of TA_PAIR_STATIC_SCRATCH bytes that
+ * each GPU holds until the process ends (set up by ta_create); host staging
+ * is the calling thread's stack.  Limits: la + lb <= TA_PAIR_STATIC_MAX and
+ * the pair's direction codes must fit on chip (the form ta_align_pair uses
+ * for such pairs); larger pairs get TA_E_LIMIT, never a fallback.  Calls
+ * that overlap on one GPU (several contexts or threads) race on its array. */
+#define TA_PAIR_STATIC_MAX 4096
+#define TA_PAIR_STATIC_SCRATCH 32768
+int ta_align_pair_static(ta_ctx *ctx, const ta_params *p, const char *a, int32_t la, const char *b, int32_t lb,
+                         ta_result *out, char *cigar_buf, int64_t cap, int64_t *used);
 
 /* Device-resident batch: inputs encoded and uploaded once, the alignment
  * run as often as wanted with inputs already in HBM.  No counterpart in the

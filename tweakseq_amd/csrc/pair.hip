@@ -26,8 +26,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "../../include/tweakalign.h"
@@ -480,4 +482,116 @@ int ta_align_pair(ta_ctx *ctx, const ta_params *p, const char *a, int32_t la, co
   if (rc) return rc;
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) return TA_E_DEVICE;
   return pair_finish(*o, (const uint32_t *)(st + o_runs), p, out, cigar_buf, cap, used);
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free variant.  `scratch` is the only device memory it uses:
+// [PairOut][runs][letters of a][letters of b].  In host code its name is a
+// host-side stand-in, never a device address: every use goes through the
+// address hipGetSymbolAddress gives for the device (pair_scratch_setup).
+namespace {
+
+static __device__ __align__(256) uint8_t scratch[TA_PAIR_STATIC_SCRATCH];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+uint8_t *scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  return g_scratch[device];
+}
+
+// Offsets inside the scratch for an la x lb pair (within its size whenever
+// la + lb <= TA_PAIR_STATIC_MAX).
+struct StaticLayout {
+  size_t runs, a, b, end;
+};
+StaticLayout static_layout(int32_t la, int32_t lb) {
+  StaticLayout L;
+  L.runs = sizeof(PairOut);
+  L.a = (L.runs + 4 * ((size_t)la + lb + 2) + 15) & ~(size_t)15;
+  L.b = (L.a + (size_t)la + 15) & ~(size_t)15;
+  L.end = L.b + (size_t)lb;
+  return L;
+}
+static_assert(sizeof(PairOut) + 4 * (TA_PAIR_STATIC_MAX + 2) + 15 + TA_PAIR_STATIC_MAX + 15 <= TA_PAIR_STATIC_SCRATCH,
+              "scratch too small for the largest pair it admits");
+
+// Result of a pair with an empty side, as the batch path gives it.
+int empty_side(const ta_params *p, int32_t la, int32_t lb, ta_result *out, char *cigar_buf, int64_t cap,
+               int64_t *used) {
+  *out = ta_result{};
+  out->cigar_off = 0;
+  if (p->mode == TA_LOCAL) {
+    out->a_begin = out->a_end = out->b_begin = out->b_end = -1;
+    out->flags = TA_F_EMPTY;
+    if (used) *used = 0;
+    return TA_OK;
+  }
+  const int32_t len = la + lb;
+  out->score = len ? p->gap_open + (len - 1) * p->gap_extend : 0;
+  out->a_begin = 0; out->a_end = la - 1; out->b_begin = 0; out->b_end = lb - 1;
+  uint32_t run = len ? ((uint32_t)len << 2) | (la ? 2u : 1u) : 0u;  // la I's or lb D's
+  const int64_t need = (p->traceback && len) ? cigar_text(&run, 1, nullptr) : 0;
+  out->cigar_len = (int32_t)need;
+  if (used) *used = need;
+  if (need > 0) {
+    if (!cigar_buf || need > cap) return TA_E_SPACE;
+    cigar_text(&run, 1, cigar_buf);
+  }
+  return TA_OK;
+}
+
+}  // namespace
+
+void pair_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  g_scratch_done[device] = true;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fprintf(stderr, "tweakalign: no address for the pair scratch on device %d: %s\n", device, hipGetErrorString(e));
+    return;
+  }
+  g_scratch[device] = (uint8_t *)ptr;
+}
+
+int ta_align_pair_static(ta_ctx *ctx, const ta_params *p, const char *a, int32_t la, const char *b, int32_t lb,
+                         ta_result *out, char *cigar_buf, int64_t cap, int64_t *used) {
+  if (!ctx || !p || !out || la < 0 || lb < 0 || (la && !a) || (lb && !b)) return TA_E_INVALID;
+  if (params_check(p)) return TA_E_INVALID;
+  if ((int64_t)la + lb > TA_PAIR_STATIC_MAX) return TA_E_LIMIT;
+  if (la == 0 || lb == 0) return empty_side(p, la, lb, out, cigar_buf, cap, used);
+  if (!pair_path(ctx, p, la, lb)) return TA_E_LIMIT;
+  uint8_t *scr = scratch_of(ctx->device);
+  if (!scr) return TA_E_DEVICE;
+  if (hipSetDevice(ctx->device) != hipSuccess) return TA_E_DEVICE;
+  const StaticLayout L = static_layout(la, lb);
+  hipStream_t st = ctx->stream;
+  if (hipMemcpyAsync(scr + L.a, a, (size_t)la, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(scr + L.b, b, (size_t)lb, hipMemcpyHostToDevice, st) != hipSuccess)
+    return TA_E_DEVICE;
+  PairArgs A = pair_args(p, la, lb);
+  A.a = (const char *)(scr + L.a);
+  A.b = (const char *)(scr + L.b);
+  A.out = (PairOut *)scr;  // the kernel writes every field the host reads (status included)
+  int rc = pair_launch(A, st);
+  if (rc) return rc;
+  // result and runs to the calling thread's stack
+  struct {
+    PairOut o;
+    uint32_t runs[TA_PAIR_STATIC_MAX + 2];
+  } back;
+  static_assert(sizeof(PairOut) % 4 == 0, "runs follow the record");
+  if (hipMemcpyAsync(&back, scr, sizeof(PairOut) + 4 * ((size_t)la + lb + 2), hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return TA_E_DEVICE;
+  return pair_finish(back.o, back.runs, p, out, cigar_buf, cap, used);
 }

@@ -331,6 +331,7 @@ const char *ta_strerror(int code) {
     case TA_E_TICKET: return "unknown ticket";
     case TA_E_NODEVICE: return "no usable GPU";
     case TA_E_WALK: return "a traceback walk did not complete";
+    case TA_E_LIMIT: return "input exceeds the fixed scratch of the allocation-free entry point";
     case TA_E_PENDING: return "pending";
     default: return "unknown error";
   }
@@ -375,6 +376,7 @@ int ta_create(ta_ctx **out, int device) {
     delete c;
     return TA_E_DEVICE;
   }
+  pair_scratch_setup(device);  // ta_align_pair_static's array on this device (reports its own failure)
   *out = c;
   return TA_OK;
 }

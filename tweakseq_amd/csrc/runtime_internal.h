@@ -29,6 +29,10 @@ struct ta_ctx {
   size_t pair_stage_cap = 0;
 };
 
+// Finds the device address of ta_align_pair_static's scratch array on
+// `device` (current), once per device; later calls return at once.
+void pair_scratch_setup(int device);
+
 // TA_OK, or TA_E_INVALID for parameters outside what the kernels take.
 int params_check(const ta_params *p);
 // CIGAR text of binary runs ((len << 2) | op, op 0 M / 1 D / 2 I); with out
