@@ -38,7 +38,7 @@ def run(cmd):
 
 
 def build_lib():
-    srcs = [os.path.join(CSRC, f) for f in ("runtime.hip", "serve.hip", "consensus.hip", "msa.hip", "pair.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("runtime.hip", "serve.hip", "consensus.hip", "msa.hip", "pair.hip", "devq.hip")]
     objs = []
     for s in srcs:
         o = os.path.join(CSRC, os.path.basename(s) + ".o")

@@ -748,6 +748,11 @@ bool global_i16_safe(const ta_params &p, int maxS, int la, int lb) {
   const long hi = (long)std::min(la, lb) * maxS + maxS;
   return lo < 16000 && hi < 32000;
 }
+}  // namespace
+
+bool global_int16_safe(const ta_params *p, int maxS, int la, int lb) { return global_i16_safe(*p, maxS, la, lb); }
+
+namespace {
 
 // Resident fill waves per CU a batch asks for: its tuning, else one per SIMD
 // when direction planes are written (those kernels keep a SIMD busy alone and
@@ -1248,7 +1253,7 @@ int launch_traceback(ta_ctx *ctx, ta_batch *B, const PairDev *pairs, const PairT
   a.do_cigar = B->p.traceback;
   a.guard_cap = B->guard_cap;
   (void)hipGetLastError();
-  hipLaunchKernelGGL(traceback_kernel, dim3((count + TB_WALKERS - 1) / TB_WALKERS), dim3(TB_WALKERS), 0,
+  hipLaunchKernelGGL(traceback_kernel<0>, dim3((count + TB_WALKERS - 1) / TB_WALKERS), dim3(TB_WALKERS), 0,
                      B->stream, a);
   HIPCHK(hipGetLastError());
   return TA_OK;
@@ -1385,7 +1390,7 @@ int launch_tracebacks(ta_ctx *ctx, ta_batch *B) {
     a.npairs = B->n_rest;
     a.order = B->d_order_rest;
     (void)hipGetLastError();
-    hipLaunchKernelGGL(traceback_kernel, dim3((B->n_rest + TB_WALKERS - 1) / TB_WALKERS), dim3(TB_WALKERS), 0,
+    hipLaunchKernelGGL(traceback_kernel<0>, dim3((B->n_rest + TB_WALKERS - 1) / TB_WALKERS), dim3(TB_WALKERS), 0,
                        B->stream, a);
     HIPCHK(hipGetLastError());
   }

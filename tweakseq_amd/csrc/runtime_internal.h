@@ -33,6 +33,31 @@ struct ta_ctx {
 // `device` (current), once per device; later calls return at once.
 void pair_scratch_setup(int device);
 
+// A global pair the packed int16 lanes hold without saturating (the batch
+// runtime's rule, shared with the device-queue planner).
+bool global_int16_safe(const ta_params *p, int maxS, int la, int lb);
+
+// Device-queue serving (devq.hip): requests planned straight into the
+// running persistent kernel.  devq_create returns nullptr when the device
+// cannot host it (serving then builds a batch per request group).
+struct DevQ;
+struct DevqJob;
+DevQ *devq_create(ta_ctx *ctx, int64_t tb_budget);
+void devq_destroy(DevQ *q);  // waits for the kernel to leave
+// Whether this request can run in the queue (parameters, sizes).
+bool devq_eligible(DevQ *q, const ta_params *p, const std::vector<std::string> &a, const std::vector<std::string> &b);
+// Admit: TA_OK with *job, TA_E_PENDING when memory or slots are short now,
+// another error when the request cannot run (a letter outside the alphabet).
+int devq_admit(DevQ *q, const ta_params *p, const std::vector<std::string> &a, const std::vector<std::string> &b,
+               DevqJob **job);
+bool devq_done(DevQ *q, DevqJob *job);
+// Results of a finished job (cigar_off into text), then its memory is freed.
+int devq_finish(DevQ *q, DevqJob *job, std::vector<ta_result> &res, std::string &text);
+// Frees a job's memory once it has finished (cancelled requests).
+void devq_drop(DevQ *q, DevqJob *job);
+// Relaunches the kernel when jobs are waiting and the last launch has ended.
+void devq_tick(DevQ *q, bool have_jobs);
+
 // TA_OK, or TA_E_INVALID for parameters outside what the kernels take.
 int params_check(const ta_params *p);
 // CIGAR text of binary runs ((len << 2) | op, op 0 M / 1 D / 2 I); with out

@@ -43,6 +43,8 @@ struct Request {
   bool walk_failed = false;    // a pair's walk gave up: delivered with TA_E_WALK
   std::vector<ta_result> res;  // cigar_off relative to `cigar`
   std::string cigar;
+  int dq = -1;                 // runs in the device queue: -1 not known yet, 0 no, 1 yes
+  DevqJob *job = nullptr;      // its job there while in flight
 };
 
 struct Group {
@@ -69,11 +71,121 @@ struct Serve {
   bool stop = false, stop_workers = false;
   std::thread th;
   std::vector<std::thread> workers;
+  // device queue (devq.hip): requests admitted into the running persistent kernel
+  DevQ *dq = nullptr;
+  bool dq_tried = false, dq_stop = false;
+  std::vector<Request *> dq_jobs;  // in flight there
+  std::thread dq_th;
+  std::condition_variable dq_cv;
 
   void loop();
   void worker();
   void run_group(Group &g, hipStream_t st);
+  void dq_loop();
+  void admit_dq();
 };
+
+// Requests the device queue can take, in arrival order, while it has room
+// (lock held).  The queue is created on first use; without it (no room on the
+// device, or TWEAKALIGN_DEVICE_QUEUE=0) every request goes to a batch.
+void Serve::admit_dq() {
+  if (!dq_tried) {
+    dq_tried = true;
+    dq = devq_create(ctx, budget);
+    if (dq) dq_th = std::thread([this] { dq_loop(); });
+  }
+  if (!dq) return;
+  bool any = false;
+  for (auto it = pending.begin(); it != pending.end();) {
+    Request *r = *it;
+    if (r->state == RState::Cancelled) { it = pending.erase(it); continue; }
+    if (r->dq < 0) r->dq = devq_eligible(dq, &r->p, r->a, r->b) ? 1 : 0;
+    if (!r->dq) { ++it; continue; }
+    if (r->tb_bytes > budget) {  // the budget was lowered below it: it can never run
+      r->state = RState::Failed;
+      r->err = TA_E_BUDGET;
+      it = pending.erase(it);
+      continue;
+    }
+    if (used + r->tb_bytes > budget) break;
+    DevqJob *job = nullptr;
+    const int rc = devq_admit(dq, &r->p, r->a, r->b, &job);
+    if (rc == TA_E_PENDING) break;  // device memory or slots: wait for completions
+    it = pending.erase(it);
+    if (rc != TA_OK) {
+      r->state = RState::Failed;
+      r->err = rc;
+      continue;
+    }
+    r->job = job;
+    r->state = RState::InFlight;
+    used += r->tb_bytes;
+    dq_jobs.push_back(r);
+    any = true;
+  }
+  if (any) {
+    devq_tick(dq, true);
+    dq_cv.notify_all();
+  }
+}
+
+// Completions of device-queue jobs: results formatted from host memory, the
+// job's memory freed, waiting requests admitted; the kernel relaunched while
+// jobs wait and stopped after a quiet spell.
+void Serve::dq_loop() {
+  std::unique_lock<std::mutex> lk(mu);
+  while (!dq_stop) {
+    std::vector<Request *> fin;
+    for (auto it = dq_jobs.begin(); it != dq_jobs.end();) {
+      if (devq_done(dq, (*it)->job)) {
+        fin.push_back(*it);
+        it = dq_jobs.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    const bool have = !dq_jobs.empty();
+    std::vector<std::vector<ta_result>> res(fin.size());
+    std::vector<std::string> text(fin.size());
+    std::vector<int> rcs(fin.size(), TA_OK);
+    std::vector<bool> cancelled(fin.size());
+    for (size_t k = 0; k < fin.size(); ++k) cancelled[k] = fin[k]->state == RState::Cancelled;
+    lk.unlock();
+    devq_tick(dq, have);
+    for (size_t k = 0; k < fin.size(); ++k) {
+      if (cancelled[k]) devq_drop(dq, fin[k]->job);
+      else rcs[k] = devq_finish(dq, fin[k]->job, res[k], text[k]);
+      fin[k]->job = nullptr;
+    }
+    lk.lock();
+    for (size_t k = 0; k < fin.size(); ++k) {
+      Request *r = fin[k];
+      used -= r->tb_bytes;
+      if (r->state == RState::Cancelled) {
+        all.erase(r->ticket);
+        delete r;
+        continue;
+      }
+      r->res.swap(res[k]);
+      r->cigar.swap(text[k]);
+      r->walk_failed = rcs[k] == TA_E_WALK;
+      r->state = RState::Done;
+      r->a.clear(); r->a.shrink_to_fit();
+      r->b.clear(); r->b.shrink_to_fit();
+    }
+    if (!fin.empty()) {
+      ++changes;
+      cv.notify_all();
+    }
+    if (dq_jobs.empty()) {
+      dq_cv.wait_for(lk, std::chrono::milliseconds(10), [&] { return dq_stop || !dq_jobs.empty(); });
+    } else {
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      lk.lock();
+    }
+  }
+}
 
 // One batch from build to per-request results (a worker, without the lock).
 void Serve::run_group(Group &g, hipStream_t st) {
@@ -161,6 +273,7 @@ void Serve::loop() {
   for (;;) {
     if (stop) break;
     const uint64_t seen = changes;
+    if (!pending.empty()) admit_dq();
     // Admission in arrival order while memory and in-flight slots allow;
     // requests with the parameters of the first admitted one form the batch.
     if (!pending.empty() && inflight < max_inflight) {
@@ -168,6 +281,7 @@ void Serve::loop() {
       for (auto it = pending.begin(); it != pending.end();) {
         Request *r = *it;
         if (r->state == RState::Cancelled) { it = pending.erase(it); continue; }
+        if (r->dq == 1) { ++it; continue; }  // waits for room in the device queue
         if (r->tb_bytes > budget) {  // the budget was lowered below it: it can never run
           r->state = RState::Failed;
           r->err = TA_E_BUDGET;
@@ -211,6 +325,17 @@ void serve_shutdown(ta_ctx *ctx) {
   }
   s->wcv.notify_all();
   for (auto &w : s->workers) w.join();
+  if (s->dq) {  // device-queue jobs in flight finish (the kernel's waits are bounded)
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv.wait_for(lk, std::chrono::seconds(60), [&] { return s->dq_jobs.empty(); });
+    s->dq_stop = true;
+    lk.unlock();
+    s->dq_cv.notify_all();
+    s->dq_th.join();
+    for (Request *r : s->dq_jobs)  // only when the wait above ran out
+      if (r->job) devq_drop(s->dq, r->job);
+    devq_destroy(s->dq);
+  }
   for (auto &kv : s->all) delete kv.second;
   delete s;
   ctx->serve = nullptr;

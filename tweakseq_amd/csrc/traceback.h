@@ -131,6 +131,7 @@ struct Walker {
   char *cig_end;
   int p, GG, hsh, NB, off;
   int i, j, state, h, s, l, k, guard;
+  int loc, gop, gex;  // mode and gap scores of this walk (taken from the TbArgs at begin)
   bool done;
 
   __device__ __forceinline__ void begin(const TbArgs &A, int p_, const PairDev &pd, const FillOut &f,
@@ -153,6 +154,7 @@ struct Walker {
     off = d.off;
     tbase = A.tb + d.tb_off;
     const bool local = A.local != 0;
+    loc = A.local; gop = A.open; gex = A.ext;
     i = local ? f.end_i : pd.la;
     j = local ? f.end_j : pd.lb;
     state = 0;  // 0 H, 1 E, 2 F
@@ -199,7 +201,7 @@ struct Walker {
   __device__ __forceinline__ void finish(const TbArgs &A) {
     cw.finish();
     if (guard <= 0 || cw.overflow) r.flags |= WALK_FAILED;
-    if (A.local) { r.a_begin = i; r.b_begin = j; }
+    if (loc) { r.a_begin = i; r.b_begin = j; }
     if (cw.on) {
       r.cigar_off = (int64_t)((char *)cw.wp - A.cig);
       r.cigar_len = (int32_t)(cig_end - (char *)cw.wp);  // bytes of runs
@@ -249,7 +251,7 @@ struct Walker {
   // The walk's head: borders that end it.  True when it is over (result
   // written).
   __device__ __forceinline__ bool at_border(const TbArgs &A) {
-    if (!A.local) {
+    if (!loc) {
       if (i == 0) { if (j) cw.emit(1, j); finish(A); return true; }
       if (j == 0) { cw.emit(2, i); finish(A); return true; }
     } else if (i <= 0 || j <= 0) {
@@ -262,8 +264,8 @@ struct Walker {
 
   // One epoch; true when the walk has ended and its result is written.
   __device__ __forceinline__ bool epoch(const TbArgs &A, const int8_t *stab, uint32_t *slice) {
-    const bool local = A.local != 0;
-    const int open = A.open, ext = A.ext;
+    const bool local = loc != 0;
+    const int open = gop, ext = gex;
     if (at_border(A)) return true;
     // ---- epoch loads
     const int t0 = j - 1 + off + l;
@@ -329,6 +331,9 @@ __device__ __forceinline__ void walk_pair(const TbArgs &A, int p, const int8_t *
   }
 }
 
+// A template so that every source including this header may hold a copy
+// (the serving kernel's source includes the fill, which includes this).
+template <int = 0>
 __global__ __launch_bounds__(TB_WALKERS) void traceback_kernel(TbArgs A) {
   __shared__ uint32_t wlds[TB_WALKERS * WSTRIDE];
   __shared__ int8_t stab[1024];
