@@ -32,7 +32,7 @@ struct SQArgs {
   const uint16_t *hcodes;         // host code arena
   const uint32_t *stop;           // host: nonzero asks the kernel to leave (new gap scores)
   uint32_t *halt;                 // device: the stop request as seen by the waves
-  uint32_t *done;                 // host: per request, pairs finished
+  uint32_t *flag;                 // host: per pair record, set (plain store) once its result is out
   int32_t idle_ticks;             // 100 MHz ticks without an item before a wave leaves
 };
 

@@ -7,8 +7,8 @@
 // ring in pinned host memory and bumps a producer index there; a persistent
 // kernel (devq_kernel) that is already running claims items as they appear,
 // fills them and walks their tracebacks beside the fill, and writes each
-// pair's result and CIGAR runs straight back to pinned host memory, bumping
-// the request's completion counter.  No copy command, allocation or launch
+// pair's result and CIGAR runs straight back to pinned host memory, then
+// sets the pair's completion flag there.  No copy command, allocation or launch
 // happens per request while the kernel runs.
 //
 //  * Communication is through coherent (fine-grained) pinned host memory
@@ -18,7 +18,10 @@
 //    cache.  Each fill wave copies its task's row and column codes into a
 //    private device scratch before sweeping it (the sweep reads them every 8
 //    steps).  Direction planes live in a device arena the host hands out per
-//    request (the traceback-memory admission budget).
+//    request (the traceback-memory admission budget).  Completion is a plain
+//    store of a per-pair flag after the result's stores have completed: no
+//    atomic operation ever targets host memory (PCIe atomics are not
+//    assumed).
 //  * Claims: a device counter of claimed items; the producer index is read
 //    from host memory by one wave at a time, at most every SQ_REFRESH ticks,
 //    into a device mirror the others poll.  A wave claims only items below
@@ -120,19 +123,19 @@ __device__ void stage_codes(const uint4 *src, uint4 *dst, int count) {
 }
 
 // Walker side of the serving kernel: as walk_loop, for pairs of either mode
-// and either lane width, and each finished pair counted in its request's
-// completion counter (host memory) once its result and runs are stored.
+// and either lane width, and each finished pair's flag set (host memory)
+// once its result and runs are stored.
 __device__ void devq_walk(const SQArgs &S, WalkQ *q, const int8_t *tab, uint32_t *wlds) {
   const int wl = threadIdx.x & 63;
   uint32_t *slice = wlds + wl * WSTRIDE;
   Walker<2> w2;
   Walker<1> w1;
   int kind = 0;  // 0 idle, 2 / 1: walking with Walker<2> / Walker<1>
-  int local = 0, group = 0;
+  int local = 0, pair = 0;
   int head = 0;
-  auto signal = [&](int g) {
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the result and runs have left
-    __hip_atomic_fetch_add(S.done + g, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  auto signal = [&](int p) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the result and runs have been stored
+    __hip_atomic_store(S.flag + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   };
   for (;;) {
     const uint64_t idle = __ballot(kind == 0);
@@ -146,9 +149,8 @@ __device__ void devq_walk(const SQArgs &S, WalkQ *q, const int8_t *tab, uint32_t
         const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
         if (kind == 0 && rank < take) {
           const int p = q->ids[(head + rank) & (WQ_CAP - 1)];
-          const int tag = S.fa[0].wk.pairs[p].pad_;
-          local = (tag >> 30) & 1;
-          group = tag & 0x3fffffff;
+          local = (S.fa[0].wk.pairs[p].pad_ >> 30) & 1;
+          pair = p;
           // mode-dependent arguments only here: the walkers keep mode and gap
           // scores from begin(), so the epochs take the shared arguments
           PairDev pd;
@@ -164,7 +166,7 @@ __device__ void devq_walk(const SQArgs &S, WalkQ *q, const int8_t *tab, uint32_t
             }
             kind = P == 2 ? 2 : 1;
           } else {
-            signal(group);
+            signal(p);
           }
         }
         head += take;
@@ -180,7 +182,7 @@ __device__ void devq_walk(const SQArgs &S, WalkQ *q, const int8_t *tab, uint32_t
       else if (kind == 1) end = w1.epoch(S.fa[0].wk, tab, slice);
       if (end) {
         kind = 0;
-        signal(group);
+        signal(pair);
       }
     } else {
       __builtin_amdgcn_s_sleep(64);
@@ -279,7 +281,6 @@ using namespace ta;
 namespace {
 
 constexpr int32_t NSLOT = 1 << 16;   // pair records
-constexpr int32_t NGROUP = 1 << 14;  // requests in flight (completion counters)
 constexpr size_t HDATA = (size_t)256 << 20, CIGB = (size_t)256 << 20;
 
 // First-fit allocator over [0, cap) that merges freed neighbours.
@@ -335,8 +336,8 @@ T *host_coherent(size_t n) {
 
 struct DevqJob {
   ta_params p{};
-  int32_t group = 0;
   size_t slot0 = 0, nslot = 0;           // pair records
+  size_t checked = 0;                    // records before this one have their flags set
   size_t h_off = 0, h_len = 0, c_off = 0, c_len = 0, t_off = 0, t_len = 0;
   uint32_t first_item = 0;
   int32_t ngpu = 0;                      // pairs the kernel walks
@@ -356,7 +357,7 @@ struct DevQ {
   int best[32];
   // coherent host memory
   SQItem *items = nullptr;
-  uint32_t *tail = nullptr, *done = nullptr;
+  uint32_t *tail = nullptr, *flag = nullptr;
   PairDev *pairs = nullptr;
   PairTB *ptb = nullptr;
   ResultDev *res = nullptr;
@@ -371,7 +372,7 @@ struct DevQ {
   int8_t *table = nullptr;
   int32_t *err = nullptr;
   size_t tb_cap = 0;
-  RangeAlloc ha, ca, ta, slots, groups;
+  RangeAlloc ha, ca, ta, slots;
   uint32_t published = 0;
   std::multiset<uint32_t> firsts;        // first items of the jobs in flight (ring reuse)
   bool lane_set[2] = {false, false};
@@ -384,7 +385,7 @@ struct DevQ {
 namespace {
 
 void free_all(DevQ *q) {
-  for (void *h : {(void *)q->items, (void *)q->tail, (void *)q->done, (void *)q->pairs, (void *)q->ptb,
+  for (void *h : {(void *)q->items, (void *)q->tail, (void *)q->flag, (void *)q->pairs, (void *)q->ptb,
                   (void *)q->res, (void *)q->hdata, (void *)q->cig})
     if (h) (void)hipHostFree(h);
   for (void *d : {(void *)q->head, (void *)q->mirror, (void *)q->wrow, (void *)q->wcol, (void *)q->wring,
@@ -422,7 +423,7 @@ void launch(DevQ *q) {
   S.stop = q->tail + 32;  // its own 128 bytes of the same block
   S.halt = q->head + 32;
   S.hcodes = (const uint16_t *)q->hdata;
-  S.done = q->done;
+  S.flag = q->flag;
   S.idle_ticks = 200000000;  // 2 s: a safety bound; the host stops an idle kernel itself
   if (hipMemsetAsync(q->head + 32, 0, 4, q->stream) == hipSuccess &&  // halt cleared before the launch
       devq_launch(S, q->grid, q->lds, q->stream) == TA_OK && hipEventRecord(q->ev, q->stream) == hipSuccess)
@@ -434,7 +435,6 @@ void release(DevQ *q, DevqJob *j) {
   if (j->c_len) q->ca.give(j->c_off, j->c_len);
   if (j->t_len) q->ta.give(j->t_off, j->t_len);
   if (j->nslot) q->slots.give(j->slot0, j->nslot);
-  q->groups.give((size_t)j->group, 1);
   if (j->ngpu) {
     auto it = q->firsts.find(j->first_item);
     if (it != q->firsts.end()) q->firsts.erase(it);
@@ -475,7 +475,7 @@ DevQ *devq_create(ta_ctx *ctx, int64_t tb_budget) {
   bool ok = q->lds <= ctx->lds_max;
   const size_t nw = (size_t)q->grid * WALK_FW;
   ok = ok && (q->items = host_coherent<SQItem>(SQ_CAP)) && (q->tail = host_coherent<uint32_t>(64)) &&
-       (q->done = host_coherent<uint32_t>(NGROUP)) && (q->pairs = host_coherent<PairDev>(NSLOT)) &&
+       (q->flag = host_coherent<uint32_t>(NSLOT)) && (q->pairs = host_coherent<PairDev>(NSLOT)) &&
        (q->ptb = host_coherent<PairTB>(NSLOT)) && (q->res = host_coherent<ResultDev>(NSLOT)) &&
        (q->hdata = host_coherent<uint8_t>(HDATA)) && (q->cig = host_coherent<uint8_t>(CIGB));
   auto dev = [&](void **p, size_t bytes) { return ok && hipMalloc(p, bytes) == hipSuccess; };
@@ -510,7 +510,6 @@ DevQ *devq_create(ta_ctx *ctx, int64_t tb_budget) {
   q->ca.init(CIGB);
   q->ta.init(q->tb_cap);
   q->slots.init(NSLOT);
-  q->groups.init(NGROUP);
   return q;
 }
 
@@ -656,14 +655,10 @@ int devq_admit(DevQ *q, const ta_params *p, const std::vector<std::string> &a, c
   std::lock_guard<std::mutex> g(q->mu);
   // room: ring slots, records, memory
   const uint32_t oldest = q->firsts.empty() ? q->published : *q->firsts.begin();
-  size_t grp = 0;
-  bool ok = q->published + (uint32_t)tasks.size() - oldest <= SQ_CAP;
-  ok = ok && q->groups.take(1, 1, grp);
-  if (!ok) {
+  if (q->published + (uint32_t)tasks.size() - oldest > SQ_CAP) {
     delete j;
     return TA_E_PENDING;
   }
-  j->group = (int32_t)grp;
   bool got_s = ng == 0 || q->slots.take((size_t)ng, 1, j->slot0);
   bool got_h = got_s && (ng == 0 || q->ha.take(hbytes, 16, j->h_off));
   bool got_c = got_h && (ng == 0 || q->ca.take(cbytes, 16, j->c_off));
@@ -672,7 +667,6 @@ int devq_admit(DevQ *q, const ta_params *p, const std::vector<std::string> &a, c
     if (got_c && ng) q->ca.give(j->c_off, cbytes);
     if (got_h && ng) q->ha.give(j->h_off, hbytes);
     if (got_s && ng) q->slots.give(j->slot0, (size_t)ng);
-    q->groups.give(grp, 1);
     delete j;
     return TA_E_PENDING;
   }
@@ -681,7 +675,7 @@ int devq_admit(DevQ *q, const ta_params *p, const std::vector<std::string> &a, c
   j->c_len = ng ? cbytes : 0;
   j->t_len = tbytes;
   j->ngpu = ng;
-  __atomic_store_n(q->done + grp, 0u, __ATOMIC_RELAXED);
+
   // residue codes, pair records
   size_t hp = j->h_off, cp = j->c_off;
   std::vector<uint64_t> aoff(n), boff(n);
@@ -703,7 +697,8 @@ int devq_admit(DevQ *q, const ta_params *p, const std::vector<std::string> &a, c
     d.cig_cap = 4 * (d.la + d.lb + 1);
     d.cig_off = cp;
     cp += (size_t)d.cig_cap;
-    d.pad_ = (int32_t)grp | (local ? (1 << 30) : 0);
+    d.pad_ = local ? (1 << 30) : 0;
+    __atomic_store_n(q->flag + s, 0u, __ATOMIC_RELAXED);
   }
   memset(q->hdata + hp, 0, 32);
   hp += 32;
@@ -749,7 +744,9 @@ int devq_admit(DevQ *q, const ta_params *p, const std::vector<std::string> &a, c
 }
 
 bool devq_done(DevQ *q, DevqJob *j) {
-  return j->ngpu == 0 || __atomic_load_n(q->done + j->group, __ATOMIC_ACQUIRE) >= (uint32_t)j->ngpu;
+  while (j->checked < (size_t)j->ngpu && __atomic_load_n(q->flag + j->slot0 + j->checked, __ATOMIC_ACQUIRE))
+    ++j->checked;
+  return j->checked == (size_t)j->ngpu;
 }
 
 int devq_finish(DevQ *q, DevqJob *j, std::vector<ta_result> &res, std::string &text) {
