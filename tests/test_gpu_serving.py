@@ -1,5 +1,11 @@
 """Serving: continuous batching returns, for every request, exactly what the
-same request gives when aligned alone; cancellation and admission limits."""
+same request gives when aligned alone; cancellation and admission limits.
+
+Each test runs through both admission paths: the device queue (requests
+pushed into the running persistent kernel) and batches (requests grouped and
+launched per batch, TWEAKALIGN_DEVICE_QUEUE=0).  The switch is read when a
+context's serving loop first admits work, so it is set before the Engine
+exists."""
 import random
 import time
 
@@ -32,6 +38,12 @@ def make_requests(seed, nreq, maxlen=700):
     return reqs
 
 
+@pytest.fixture(params=["device_queue", "batches"])
+def path(request, monkeypatch):
+    monkeypatch.setenv("TWEAKALIGN_DEVICE_QUEUE", "1" if request.param == "device_queue" else "0")
+    return request.param
+
+
 def wait_all(eng, tickets, timeout=120):
     out = {}
     t0 = time.time()
@@ -47,7 +59,7 @@ def wait_all(eng, tickets, timeout=120):
     return out
 
 
-def test_results_identical_alone_and_in_the_mix():
+def test_results_identical_alone_and_in_the_mix(path):
     eng = ta.Engine(0)
     try:
         eng.serve_config(64 << 30, 4)
@@ -61,7 +73,7 @@ def test_results_identical_alone_and_in_the_mix():
         eng.close()
 
 
-def test_served_config4_mix_matches_oracle():
+def test_served_config4_mix_matches_oracle(path):
     """The served workload's mix -- 80 requests of 1-256 pairs (geometric,
     mean 16), lengths uniform in [50, 5,000], global and local interleaved
     per request, -10/-1 -- submitted at once through ta_submit / ta_poll; every
@@ -93,7 +105,7 @@ def test_served_config4_mix_matches_oracle():
             mode, len(bad), len(idx), bad[:3])
 
 
-def test_small_budget_serialises_but_completes():
+def test_small_budget_serialises_but_completes(path):
     eng = ta.Engine(0)
     try:
         reqs = make_requests(2, 12, maxlen=400)
@@ -107,7 +119,7 @@ def test_small_budget_serialises_but_completes():
         eng.close()
 
 
-def test_budget_rejects_oversized_request():
+def test_budget_rejects_oversized_request(path):
     eng = ta.Engine(0)
     try:
         eng.serve_config(1 << 20, 2)
@@ -118,7 +130,7 @@ def test_budget_rejects_oversized_request():
         eng.close()
 
 
-def test_lowered_budget_fails_queued_request_and_serves_the_rest():
+def test_lowered_budget_fails_queued_request_and_serves_the_rest(path):
     eng = ta.Engine(0)
     try:
         eng.serve_config(64 << 30, 1)
@@ -139,9 +151,23 @@ def test_lowered_budget_fails_queued_request_and_serves_the_rest():
             r = eng.poll(t_small, 1)
             assert time.time() - t0 < 60
         assert r == eng.align(small, params)
-        with pytest.raises(RuntimeError, match="budget"):
-            while eng.poll(t_big, 1) is None:
-                assert time.time() - t0 < 60
+        if path == "batches":
+            with pytest.raises(RuntimeError, match="budget"):
+                while eng.poll(t_big, 1) is None:
+                    assert time.time() - t0 < 60
+        else:
+            # The device queue takes requests as they arrive (max_inflight
+            # counts batches, which it does not form): the big request was
+            # admitted before the change, and then runs exactly, or it was
+            # still waiting and failed.
+            try:
+                rb = None
+                while rb is None:
+                    rb = eng.poll(t_big, 1)
+                    assert time.time() - t0 < 60
+                assert rb == eng.align(big, params)
+            except RuntimeError as e:
+                assert "budget" in str(e)
         # The first request ran, or (if the scheduler first saw it after the
         # change) failed the same way; either way it left the queue.
         try:
@@ -153,7 +179,7 @@ def test_lowered_budget_fails_queued_request_and_serves_the_rest():
         eng.close()
 
 
-def test_cancel():
+def test_cancel(path):
     eng = ta.Engine(0)
     try:
         eng.serve_config(64 << 30, 1)

@@ -30,6 +30,11 @@ struct SQArgs {
   uint32_t *head;                 // device: items claimed
   unsigned long long *mirror;     // device: (tail << 32) | time of the last host read
   const uint16_t *hcodes;         // host code arena
+  uint8_t *dcodes;                // device copy of its residue codes (same offsets), for the walkers
+  const PairDev *hpairs;          // host pair records ...
+  const PairTB *hptb;
+  PairDev *dpairs;                // ... and their device copies (the walkers read these)
+  PairTB *dptb;
   const uint32_t *stop;           // host: nonzero asks the kernel to leave (new gap scores)
   uint32_t *halt;                 // device: the stop request as seen by the waves
   uint32_t *flag;                 // host: per pair record, set (plain store) once its result is out

@@ -106,19 +106,54 @@ __device__ int64_t devq_claim(const SQArgs &S) {
   return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Copy of `count` 16-byte words from coherent host memory to this wave's
-// device scratch.
+// Copy of `count` 16-byte words from host memory to device memory with
+// system-coherent loads (never a stale cached line of host memory).
+__device__ __forceinline__ uint4 load16_sys(const uint4 *p) {
+  const uint32_t *w = (const uint32_t *)p;
+  return make_uint4(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                    __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                    __hip_atomic_load(w + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                    __hip_atomic_load(w + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+}
 __device__ void stage_codes(const uint4 *src, uint4 *dst, int count) {
   const int lane = threadIdx.x & 63;
   for (int base = lane; base < count; base += 4 * 64) {
     // four unconditional (clamped) loads in flight, then the stores
     const int q1 = base + 64, q2 = base + 128, q3 = base + 192, last = count - 1;
-    const uint4 v0 = src[base], v1 = src[q1 < count ? q1 : last];
-    const uint4 v2 = src[q2 < count ? q2 : last], v3 = src[q3 < count ? q3 : last];
+    const uint4 v0 = load16_sys(src + base), v1 = load16_sys(src + (q1 < count ? q1 : last));
+    const uint4 v2 = load16_sys(src + (q2 < count ? q2 : last)), v3 = load16_sys(src + (q3 < count ? q3 : last));
     dst[base] = v0;
     if (q1 < count) dst[q1] = v1;
     if (q2 < count) dst[q2] = v2;
     if (q3 < count) dst[q3] = v3;
+  }
+}
+
+// A claimed task's pair records and residue codes, from host memory to the
+// device copies its walker reads (the walker runs on this CU, so it sees
+// these stores).
+__device__ void stage_pairs(const SQArgs &S, const Task &T) {
+  const int lane = threadIdx.x & 63;
+  constexpr int PDW = (int)(sizeof(PairDev) / 4), TDW = (int)(sizeof(PairTB) / 4);
+  const int h = lane >> 5, k = lane & 31;  // half h: lanes 32 h .. 32 h + PDW + TDW - 1
+  const int np = T.single ? 1 : 2;
+  const int p = T.pair[h];
+  uint32_t v = 0;
+  if (h < np && k < PDW + TDW) {
+    const uint32_t *src = k < PDW ? (const uint32_t *)(S.hpairs + p) + k : (const uint32_t *)(S.hptb + p) + (k - PDW);
+    uint32_t *dst = k < PDW ? (uint32_t *)(S.dpairs + p) + k : (uint32_t *)(S.dptb + p) + (k - PDW);
+    v = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    *dst = v;
+  }
+  // residues: a at dwords 0-1 (offset) and 4 (length), b at 2-3 and 5
+  for (int hh = 0; hh < np; ++hh) {
+    const uint32_t aoff_lo = __shfl(v, 32 * hh + 0, 64), aoff_hi = __shfl(v, 32 * hh + 1, 64);
+    const uint32_t boff_lo = __shfl(v, 32 * hh + 2, 64), boff_hi = __shfl(v, 32 * hh + 3, 64);
+    const int la = (int)__shfl(v, 32 * hh + 4, 64), lb = (int)__shfl(v, 32 * hh + 5, 64);
+    const uint64_t aoff = ((uint64_t)aoff_hi << 32) | aoff_lo, boff = ((uint64_t)boff_hi << 32) | boff_lo;
+    const uint8_t *hc = (const uint8_t *)S.hcodes;
+    stage_codes((const uint4 *)(hc + aoff), (uint4 *)(S.dcodes + aoff), (la + 15) / 16);
+    stage_codes((const uint4 *)(hc + boff), (uint4 *)(S.dcodes + boff), (lb + 15) / 16);
   }
 }
 
@@ -134,8 +169,9 @@ __device__ void devq_walk(const SQArgs &S, WalkQ *q, const int8_t *tab, uint32_t
   int local = 0, pair = 0;
   int head = 0;
   auto signal = [&](int p) {
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the result and runs have been stored
-    __hip_atomic_store(S.flag + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    // release at system scope: the result and runs (system-scope stores) are
+    // visible to the host before the flag is
+    __hip_atomic_store(S.flag + p, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   };
   for (;;) {
     const uint64_t idle = __ballot(kind == 0);
@@ -235,6 +271,7 @@ __global__ __launch_bounds__(128 * WALK_FW) void devq_kernel(SQArgs S) {
                 T.S * GWAVE * R / 8);
     stage_codes((const uint4 *)(S.hcodes + T.col_off), (uint4 *)const_cast<uint16_t *>(S.fa[0].colcodes + (size_t)gw * SQ_COLS),
                 (8 * T.NB + 8) / 8);
+    stage_pairs(S, T);
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -365,6 +402,9 @@ struct DevQ {
   // device memory
   uint32_t *head = nullptr;
   unsigned long long *mirror = nullptr;
+  PairDev *dpairs = nullptr;
+  PairTB *dptb = nullptr;
+  uint8_t *dcodes = nullptr;
   uint16_t *wrow = nullptr, *wcol = nullptr;
   uint32_t *wring = nullptr;
   FillOut *fo = nullptr;
@@ -388,7 +428,8 @@ void free_all(DevQ *q) {
   for (void *h : {(void *)q->items, (void *)q->tail, (void *)q->flag, (void *)q->pairs, (void *)q->ptb,
                   (void *)q->res, (void *)q->hdata, (void *)q->cig})
     if (h) (void)hipHostFree(h);
-  for (void *d : {(void *)q->head, (void *)q->mirror, (void *)q->wrow, (void *)q->wcol, (void *)q->wring,
+  for (void *d : {(void *)q->head, (void *)q->mirror, (void *)q->dpairs, (void *)q->dptb, (void *)q->dcodes,
+                  (void *)q->wrow, (void *)q->wcol, (void *)q->wring,
                   (void *)q->fo, (void *)q->tb, (void *)q->table, (void *)q->err})
     if (d) (void)hipFree(d);
   if (q->ev) (void)hipEventDestroy(q->ev);
@@ -406,10 +447,11 @@ FillArgs fill_args(DevQ *q, int local) {
   a.ncoop = 0;
   a.err = q->err;
   a.spin_limit = 1 << 24;
-  TbArgs &w = a.wk;
-  w.pairs = q->pairs; w.npairs = NSLOT; w.order = nullptr; w.fo = q->fo; w.ptb = q->ptb;
-  w.codes = q->hdata; w.table = q->table; w.Kp = q->Kp; w.tb = q->tb; w.cig = (char *)q->cig; w.res = q->res;
+  TbArgs &w = a.wk;  // pair records and residues: the device copies; results and runs: host memory
+  w.pairs = q->dpairs; w.npairs = NSLOT; w.order = nullptr; w.fo = q->fo; w.ptb = q->dptb;
+  w.codes = q->dcodes; w.table = q->table; w.Kp = q->Kp; w.tb = q->tb; w.cig = (char *)q->cig; w.res = q->res;
   w.local = local; w.open = a.open; w.ext = a.ext; w.do_cigar = 1;
+  w.sys = 1;
   w.guard_cap = INT32_MAX;
   if (const char *e = getenv("TWEAKALIGN_WALK_GUARD")) w.guard_cap = std::max(0, atoi(e));
   return a;
@@ -423,6 +465,8 @@ void launch(DevQ *q) {
   S.stop = q->tail + 32;  // its own 128 bytes of the same block
   S.halt = q->head + 32;
   S.hcodes = (const uint16_t *)q->hdata;
+  S.dcodes = q->dcodes;
+  S.hpairs = q->pairs; S.hptb = q->ptb; S.dpairs = q->dpairs; S.dptb = q->dptb;
   S.flag = q->flag;
   S.idle_ticks = 200000000;  // 2 s: a safety bound; the host stops an idle kernel itself
   if (hipMemsetAsync(q->head + 32, 0, 4, q->stream) == hipSuccess &&  // halt cleared before the launch
@@ -480,6 +524,8 @@ DevQ *devq_create(ta_ctx *ctx, int64_t tb_budget) {
        (q->hdata = host_coherent<uint8_t>(HDATA)) && (q->cig = host_coherent<uint8_t>(CIGB));
   auto dev = [&](void **p, size_t bytes) { return ok && hipMalloc(p, bytes) == hipSuccess; };
   ok = dev((void **)&q->head, 256) && dev((void **)&q->mirror, 256) &&
+       dev((void **)&q->dpairs, NSLOT * sizeof(PairDev)) && dev((void **)&q->dptb, NSLOT * sizeof(PairTB)) &&
+       dev((void **)&q->dcodes, HDATA) &&
        dev((void **)&q->wrow, nw * devq_scratch_rows() * 2) && dev((void **)&q->wcol, nw * devq_scratch_cols() * 2) &&
        dev((void **)&q->wring, nw * devq_scratch_ring() * 4) && dev((void **)&q->fo, NSLOT * sizeof(FillOut)) &&
        dev((void **)&q->table, 1024) && dev((void **)&q->err, 256);

@@ -1193,6 +1193,7 @@ TbArgs tb_args(ta_batch *B) {
   a.local = B->p.mode == TA_LOCAL; a.open = B->p.gap_open; a.ext = B->p.gap_extend;
   a.do_cigar = B->p.traceback;
   a.guard_cap = B->guard_cap;
+  a.sys = 0;
   return a;
 }
 

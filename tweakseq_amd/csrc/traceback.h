@@ -40,7 +40,26 @@ struct TbArgs {
   int32_t local, open, ext;
   int32_t do_cigar;
   int32_t guard_cap;     // most steps a walk may take before it ends flagged (normally above any walk)
+  int32_t sys;           // res / cig are in host memory: store them system-coherent (bypassing the GPU caches)
 };
+
+// A dword to memory the host reads while the kernel runs (sys) or as usual.
+__device__ __forceinline__ void put_word(uint32_t *p, uint32_t v, bool sys) {
+  if (sys) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  else *p = v;
+}
+
+// The result of pair p.
+__device__ __forceinline__ void put_result(const TbArgs &A, int p, const ResultDev &r) {
+  if (A.sys) {
+    const uint32_t *w = (const uint32_t *)&r;
+    uint32_t *d = (uint32_t *)(A.res + p);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(ResultDev) / 4); ++k) put_word(d + k, w[k], true);
+  } else {
+    A.res[p] = r;
+  }
+}
 
 // ResultDev::flags of a walk that gave up (TA_F_WALK_FAILED in the C ABI).
 constexpr int32_t WALK_FAILED = 0x100;
@@ -55,11 +74,12 @@ struct RunWriter {
   int len = 0;
   bool on = true;
   bool overflow = false;
+  bool sys = false;  // the region is in host memory
 
   __device__ __forceinline__ void store() {
     if (!len || !on) return;
     if (wp <= lo) { overflow = true; return; }
-    *--wp = ((uint32_t)len << 2) | (uint32_t)op;
+    put_word(--wp, ((uint32_t)len << 2) | (uint32_t)op, sys);
   }
   __device__ __forceinline__ void emit(int o, int k) {
     if (o != op) { store(); op = o; len = 0; }
@@ -109,7 +129,7 @@ __device__ __forceinline__ bool walk_prologue(const TbArgs &A, int p, PairDev &p
   }
   if (empty || (!local && !A.do_cigar)) {
     if (empty) r.flags |= 8;
-    A.res[p] = r;
+    put_result(A, p, r);
     return false;
   }
   return A.ptb[p].P != 3;
@@ -141,6 +161,7 @@ struct Walker {
     const PairTB d = A.ptb[p];
     cw = RunWriter();
     cw.on = A.do_cigar != 0;
+    cw.sys = A.sys != 0;
     cig_end = A.cig + pd.cig_off + pd.cig_cap;
     if (cw.on) {
       cw.lo = (uint32_t *)(A.cig + pd.cig_off);
@@ -206,7 +227,7 @@ struct Walker {
       r.cigar_off = (int64_t)((char *)cw.wp - A.cig);
       r.cigar_len = (int32_t)(cig_end - (char *)cw.wp);  // bytes of runs
     }
-    A.res[p] = r;
+    put_result(A, p, r);
   }
 
   // Direction code (e f d x in bits 0..3) of row k in a lane's block entry.
