@@ -42,9 +42,9 @@ struct SQArgs {
 };
 
 int devq_lds_bytes(int Kp);
-size_t devq_scratch_rows();       // uint16 row codes per fill wave
-size_t devq_scratch_cols();       // uint16 column codes per fill wave
-size_t devq_scratch_ring();       // uint32 ring entries per fill wave
-int devq_launch(const SQArgs &S, int grid, int lds, hipStream_t st);
+size_t devq_scratch_rows();       // uint16 row codes per workgroup
+size_t devq_scratch_cols();       // uint16 column codes per workgroup
+size_t devq_scratch_ring();       // uint32 ring entries per workgroup
+int devq_launch(const SQArgs *S, int grid, int lds, hipStream_t st);  // S: device memory
 
 }  // namespace ta

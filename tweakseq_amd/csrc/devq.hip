@@ -15,13 +15,14 @@
 //    only: the kernel's loads of items, codes and pair records and its
 //    stores of results bypass the GPU caches, so nothing written by the host
 //    can be stale on the device and nothing written by the device waits in a
-//    cache.  Each fill wave copies its task's row and column codes into a
+//    cache.  A workgroup copies its task's row and column codes into its
 //    private device scratch before sweeping it (the sweep reads them every 8
-//    steps).  Direction planes live in a device arena the host hands out per
-//    request (the traceback-memory admission budget).  Completion is a plain
-//    store of a per-pair flag after the result's stores have completed: no
-//    atomic operation ever targets host memory (PCIe atomics are not
-//    assumed).
+//    steps), and the task's pair records and residues into device copies
+//    its walkers read.  Direction planes live in a device arena the host hands out per
+//    request (the traceback-memory admission budget).  Completion is a store of
+//    a per-pair flag with system-scope release order after the result's and
+//    runs' system-scope stores: no read-modify-write ever targets host memory
+//    (PCIe atomics are not assumed).
 //  * Claims: a device counter of claimed items; the producer index is read
 //    from host memory by one wave at a time, at most every SQ_REFRESH ticks,
 //    into a device mirror the others poll.  A wave claims only items below
@@ -157,68 +158,67 @@ __device__ void stage_pairs(const SQArgs &S, const Task &T) {
   }
 }
 
-// Walker side of the serving kernel: as walk_loop, for pairs of either mode
-// and either lane width, and each finished pair's flag set (host memory)
-// once its result and runs are stored.
-__device__ void devq_walk(const SQArgs &S, WalkQ *q, const int8_t *tab, uint32_t *wlds) {
+// Walker side of the serving kernel: as walk_loop, for pairs of either mode,
+// and each finished pair's flag set (host memory) once its result and runs
+// are stored.  A run walks pairs of one lane width PW (its own register
+// allocation: the two walkers never hold state at once); when the oldest
+// waiting pair has the other width, it drains its lanes and returns so the
+// caller switches.  Returns (head << 1) | 1 for a switch, head << 1 when the
+// fill side has finished and every lane is idle.
+template <int PW>
+__device__ __noinline__ int devq_walk_run(const SQArgs &S, WalkQ *q, const int8_t *tab, uint32_t *wlds, int head) {
   const int wl = threadIdx.x & 63;
   uint32_t *slice = wlds + wl * WSTRIDE;
-  Walker<2> w2;
-  Walker<1> w1;
-  int kind = 0;  // 0 idle, 2 / 1: walking with Walker<2> / Walker<1>
-  int local = 0, pair = 0;
-  int head = 0;
-  auto signal = [&](int p) {
-    // release at system scope: the result and runs (system-scope stores) are
-    // visible to the host before the flag is
-    __hip_atomic_store(S.flag + p, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-  };
+  Walker<PW> w;
+  bool busy = false;
+  int pair = 0;
   for (;;) {
-    const uint64_t idle = __ballot(kind == 0);
+    const uint64_t all = __ballot(true);
+    const uint64_t idle = __ballot(!busy);
     if (idle) {
       const int done = __hip_atomic_load(&q->done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
       const int tail = __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
       const int avail = tail - head;
       if (avail > 0) {
         const int nidle = __popcll(idle);
-        const int take = avail < nidle ? avail : nidle;
+        int take = avail < nidle ? avail : nidle;
         const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
-        if (kind == 0 && rank < take) {
-          const int p = q->ids[(head + rank) & (WQ_CAP - 1)];
-          local = (S.fa[0].wk.pairs[p].pad_ >> 30) & 1;
-          pair = p;
-          // mode-dependent arguments only here: the walkers keep mode and gap
-          // scores from begin(), so the epochs take the shared arguments
-          PairDev pd;
-          FillOut f;
-          ResultDev r;
-          const bool go = local ? walk_prologue(S.fa[1].wk, p, pd, f, r) : walk_prologue(S.fa[0].wk, p, pd, f, r);
-          if (go) {
-            const int P = S.fa[0].wk.ptb[p].P;
-            if (local) {
-              if (P == 2) w2.begin(S.fa[1].wk, p, pd, f, r); else w1.begin(S.fa[1].wk, p, pd, f, r);
+        const bool cand = !busy && rank < take;
+        const int p = cand ? q->ids[(head + rank) & (WQ_CAP - 1)] : 0;
+        const bool other = cand && ((S.fa[0].wk.ptb[p].P == 1 ? 1 : 2) != PW);
+        const uint64_t oth = __ballot(other);
+        if (oth) take = __shfl(rank, __builtin_ctzll(oth), 64);  // the leading run of this width only
+        if (take == 0) {
+          if (idle == all) return (head << 1) | 1;  // the oldest pair needs the other walker
+        } else {
+          if (!busy && rank < take) {
+            const int local = (S.fa[0].wk.pairs[p].pad_ >> 30) & 1;
+            const TbArgs &A = S.fa[local].wk;
+            PairDev pd;
+            FillOut f;
+            ResultDev r;
+            if (walk_prologue(A, p, pd, f, r)) {
+              w.begin(A, p, pd, f, r);  // keeps the mode and gap scores of A
+              pair = p;
+              busy = true;
             } else {
-              if (P == 2) w2.begin(S.fa[0].wk, p, pd, f, r); else w1.begin(S.fa[0].wk, p, pd, f, r);
+              __hip_atomic_store(S.flag + p, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
             }
-            kind = P == 2 ? 2 : 1;
-          } else {
-            signal(p);
           }
+          head += take;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // ids read before the slots are freed
+          if (wl == 0) __hip_atomic_store(&q->head, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        head += take;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // ids read before the slots are freed
-        if (wl == 0) __hip_atomic_store(&q->head, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      } else if (done && idle == __ballot(true)) {
-        return;
+      } else if (done && idle == all) {
+        return head << 1;
       }
     }
-    if (__any(kind != 0)) {
-      bool end = false;
-      if (kind == 2) end = w2.epoch(S.fa[0].wk, tab, slice);
-      else if (kind == 1) end = w1.epoch(S.fa[0].wk, tab, slice);
-      if (end) {
-        kind = 0;
-        signal(pair);
+    if (__any(busy)) {
+      if (busy && w.epoch(S.fa[0].wk, tab, slice)) {
+        busy = false;
+        // release at system scope: the result and runs (system-scope stores)
+        // are visible to the host before the flag is
+        __hip_atomic_store(S.flag + pair, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
       }
     } else {
       __builtin_amdgcn_s_sleep(64);
@@ -226,11 +226,59 @@ __device__ void devq_walk(const SQArgs &S, WalkQ *q, const int8_t *tab, uint32_t
   }
 }
 
-// One workgroup per CU: WALK_FW fill waves, each claiming items and sweeping
-// them, and WALK_FW walker waves (the walking fill kernel's layout), plus per
-// fill wave an LDS copy of its item and a task counter for fill_phase.
-__global__ __launch_bounds__(128 * WALK_FW) void devq_kernel(SQArgs S) {
+__device__ void devq_walk(const SQArgs &S, WalkQ *q, const int8_t *tab, uint32_t *wlds) {
+  int head = 0, pw = 2;
+  for (;;) {
+    const int r = pw == 2 ? devq_walk_run<2>(S, q, tab, wlds, head) : devq_walk_run<1>(S, q, tab, wlds, head);
+    head = r >> 1;
+    if (!(r & 1)) return;
+    pw = 3 - pw;
+  }
+}
+
+// A cooperative sweep of one item (own register allocation per lane width
+// and mode).
+template <int P, bool LOCAL>
+__device__ __noinline__ void devq_fill(const FillArgs &A, const Task *tk, int32_t *ctr, const int8_t *tab, int8_t *slot,
+                                       WalkQ *to, CoopQ *cq) {
+  int wtail = __hip_atomic_load(&to->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  fill_phase<GWAVE, P, LOCAL, true, true>(A, tk, 1, ctr, tab, slot, to, wtail, cq);
+}
+
+// Barrier of a workgroup's WALK_FW fill waves (the walker waves never join
+// it): LDS arrival count bar[0] and generation bar[1].  A wait gives up after
+// 10 s (the leader's claim returns within its 2 s idle bound), returning
+// false; the caller then leaves.
+__device__ bool fill_barrier(int32_t *bar) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  int ok = 1;
+  if ((threadIdx.x & 63) == 0) {
+    const int g = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (__hip_atomic_fetch_add(bar, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == WALK_FW - 1) {
+      __hip_atomic_store(bar, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_store(bar + 1, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+      const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+      while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == g) {
+        if (__builtin_amdgcn_s_memrealtime() - t0 > 1000000000ull) { ok = 0; break; }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+  }
+  ok = __shfl(ok, 0, 64);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  return ok != 0;
+}
+
+// One workgroup per CU: WALK_FW fill waves and WALK_FW walker waves (the
+// walking fill kernel's layout).  The workgroup takes one item at a time:
+// wave 0 claims it and stages its codes into the workgroup's scratch, then
+// the fill waves sweep it together as a cooperative task (wave w takes
+// strips w, w + WALK_FW, ...), so a long pair's fill time is its cells over
+// four waves, not one; wave 0 hands the pairs to the walker waves in turn.
+__global__ __launch_bounds__(128 * WALK_FW) void devq_kernel(const SQArgs *__restrict__ Sp) {
   extern __shared__ __align__(16) uint8_t lds[];
+  const SQArgs &S = *Sp;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int8_t *tab = (int8_t *)lds;
   const int Kp = S.fa[0].Kp;
@@ -244,54 +292,61 @@ __global__ __launch_bounds__(128 * WALK_FW) void devq_kernel(SQArgs S) {
   uint32_t *wlds = (uint32_t *)(lds + base) + f * WALK_LANES * WSTRIDE;
   WalkQ *wq0 = (WalkQ *)(lds + base + WALK_FW * WALK_LANES * WSTRIDE * 4);
   WalkQ *wq = wq0 + f;
-  SQItem *items = (SQItem *)((uint8_t *)(wq0 + WALK_FW) + sizeof(CoopQ));
-  int32_t *ctrs = (int32_t *)(items + WALK_FW);
+  CoopQ *cq = (CoopQ *)(wq0 + WALK_FW);
+  SQItem *it = (SQItem *)((uint8_t *)(wq0 + WALK_FW) + sizeof(CoopQ));  // the round's item
+  int32_t *ctrs = (int32_t *)(it + WALK_FW);  // [0] task counter, [1..2] barrier, [3] round state
   if (wave < WALK_FW && lane == 0) { wq->tail = 0; wq->head = 0; wq->done = 0; }
+  if (threadIdx.x == 0) { ctrs[0] = 0; ctrs[1] = 0; ctrs[2] = 0; ctrs[3] = 0; }
   __syncthreads();
   if (wave >= WALK_FW) {
     if (lane < WALK_LANES) devq_walk(S, wq, tab, wlds);
     return;
   }
   int8_t *slot = (int8_t *)lds + 1024 + threadIdx.x * stride;
-  SQItem *it = items + wave;
-  int32_t *ctr = ctrs + wave;
-  const int gw = blockIdx.x * WALK_FW + wave;  // this fill wave's scratch
-  int wtail = 0;
+  const size_t gw = (size_t)blockIdx.x;  // this workgroup's scratch
+  int round = 0;
   for (;;) {
-    const int64_t c = devq_claim(S);
-    if (c < 0) break;
-    const uint32_t *src = (const uint32_t *)(S.items + (c & (SQ_CAP - 1)));
-    if (lane < SQ_TASK_WORDS) ((uint32_t *)it)[lane] = __hip_atomic_load(src + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const Task T = it->task;
-    const int kind = it->kind;
-    // this task's codes into the wave's scratch
-    stage_codes((const uint4 *)(S.hcodes + T.row_off), (uint4 *)const_cast<uint16_t *>(S.fa[0].rowcodes + (size_t)gw * SQ_ROWS),
-                T.S * GWAVE * R / 8);
-    stage_codes((const uint4 *)(S.hcodes + T.col_off), (uint4 *)const_cast<uint16_t *>(S.fa[0].colcodes + (size_t)gw * SQ_COLS),
-                (8 * T.NB + 8) / 8);
-    stage_pairs(S, T);
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    if (lane == 0) {
-      it->task.row_off = (uint64_t)gw * SQ_ROWS;
-      it->task.col_off = (uint64_t)gw * SQ_COLS;
-      it->task.ring_off = (uint64_t)gw * SQ_RING;
-      *ctr = 0;
+    if (wave == 0) {
+      const int64_t c = devq_claim(S);
+      if (c >= 0) {
+        const uint32_t *src = (const uint32_t *)(S.items + (c & (SQ_CAP - 1)));
+        if (lane < SQ_TASK_WORDS)
+          ((uint32_t *)it)[lane] = __hip_atomic_load(src + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const Task T = it->task;
+        // this task's codes into the workgroup's scratch
+        stage_codes((const uint4 *)(S.hcodes + T.row_off), (uint4 *)const_cast<uint16_t *>(S.fa[0].rowcodes + gw * SQ_ROWS),
+                    T.S * GWAVE * R / 8);
+        stage_codes((const uint4 *)(S.hcodes + T.col_off), (uint4 *)const_cast<uint16_t *>(S.fa[0].colcodes + gw * SQ_COLS),
+                    (8 * T.NB + 8) / 8);
+        stage_pairs(S, T);
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        if (lane == 0) {
+          it->task.row_off = gw * SQ_ROWS;
+          it->task.col_off = gw * SQ_COLS;
+          it->task.ring_off = gw * SQ_RING;
+          ctrs[0] = 0;
+          cq->epoch = 0;
+          cq->arrived = 0;
+        }
+      }
+      if (lane == 0) ctrs[3] = c >= 0 ? 1 : 0;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (!fill_barrier(ctrs + 1)) break;
+    if (!__hip_atomic_load(ctrs + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
     const Task *tk = &it->task;
-    switch (kind & 3) {
-      case 0: fill_phase<GWAVE, 2, false, true>(S.fa[0], tk, 1, ctr, tab, slot, wq, wtail); break;
-      case 1: fill_phase<GWAVE, 2, true, true>(S.fa[1], tk, 1, ctr, tab, slot, wq, wtail); break;
-      case 2: fill_phase<GWAVE, 1, false, true>(S.fa[0], tk, 1, ctr, tab, slot, wq, wtail); break;
-      default: fill_phase<GWAVE, 1, true, true>(S.fa[1], tk, 1, ctr, tab, slot, wq, wtail); break;
+    WalkQ *to = wq0 + (round & (WALK_FW - 1));  // the walker that takes this item's pairs
+    switch (it->kind & 3) {
+      case 0: devq_fill<2, false>(S.fa[0], tk, ctrs, tab, slot, to, cq); break;
+      case 1: devq_fill<2, true>(S.fa[1], tk, ctrs, tab, slot, to, cq); break;
+      case 2: devq_fill<1, false>(S.fa[0], tk, ctrs, tab, slot, to, cq); break;
+      default: devq_fill<1, true>(S.fa[1], tk, ctrs, tab, slot, to, cq); break;
     }
+    ++round;
+    if (!fill_barrier(ctrs + 1)) break;  // every wave is out of the task before wave 0 resets its state
   }
-  if (lane == 0) __hip_atomic_store(&wq->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (wave == 0 && lane < WALK_FW) __hip_atomic_store(&wq0[lane].done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 }  // namespace
@@ -303,7 +358,7 @@ size_t devq_scratch_rows() { return SQ_ROWS; }
 size_t devq_scratch_cols() { return SQ_COLS; }
 size_t devq_scratch_ring() { return SQ_RING; }
 
-int devq_launch(const SQArgs &S, int grid, int lds, hipStream_t st) {
+int devq_launch(const SQArgs *S, int grid, int lds, hipStream_t st) {
   (void)hipGetLastError();
   hipLaunchKernelGGL(devq_kernel, dim3(grid), dim3(128 * WALK_FW), lds, st, S);
   return hipGetLastError() == hipSuccess ? TA_OK : TA_E_DEVICE;
@@ -411,6 +466,8 @@ struct DevQ {
   uint8_t *tb = nullptr;
   int8_t *table = nullptr;
   int32_t *err = nullptr;
+  SQArgs *dargs = nullptr;  // the kernel's arguments (device copy of hargs)
+  SQArgs hargs{};
   size_t tb_cap = 0;
   RangeAlloc ha, ca, ta, slots;
   uint32_t published = 0;
@@ -430,7 +487,7 @@ void free_all(DevQ *q) {
     if (h) (void)hipHostFree(h);
   for (void *d : {(void *)q->head, (void *)q->mirror, (void *)q->dpairs, (void *)q->dptb, (void *)q->dcodes,
                   (void *)q->wrow, (void *)q->wcol, (void *)q->wring,
-                  (void *)q->fo, (void *)q->tb, (void *)q->table, (void *)q->err})
+                  (void *)q->fo, (void *)q->tb, (void *)q->table, (void *)q->err, (void *)q->dargs})
     if (d) (void)hipFree(d);
   if (q->ev) (void)hipEventDestroy(q->ev);
   if (q->stream) (void)hipStreamDestroy(q->stream);
@@ -458,7 +515,8 @@ FillArgs fill_args(DevQ *q, int local) {
 }
 
 void launch(DevQ *q) {
-  SQArgs S{};
+  SQArgs &S = q->hargs;
+  S = SQArgs{};
   S.fa[0] = fill_args(q, 0);
   S.fa[1] = fill_args(q, 1);
   S.items = q->items; S.tail = q->tail; S.head = q->head; S.mirror = q->mirror;
@@ -470,7 +528,8 @@ void launch(DevQ *q) {
   S.flag = q->flag;
   S.idle_ticks = 200000000;  // 2 s: a safety bound; the host stops an idle kernel itself
   if (hipMemsetAsync(q->head + 32, 0, 4, q->stream) == hipSuccess &&  // halt cleared before the launch
-      devq_launch(S, q->grid, q->lds, q->stream) == TA_OK && hipEventRecord(q->ev, q->stream) == hipSuccess)
+      hipMemcpyAsync(q->dargs, &q->hargs, sizeof(SQArgs), hipMemcpyHostToDevice, q->stream) == hipSuccess &&
+      devq_launch(q->dargs, q->grid, q->lds, q->stream) == TA_OK && hipEventRecord(q->ev, q->stream) == hipSuccess)
     q->launched = true;
 }
 
@@ -517,7 +576,7 @@ DevQ *devq_create(ta_ctx *ctx, int64_t tb_budget) {
   q->lds = devq_lds_bytes(q->Kp);
   q->grid = std::max(1, ctx->cus - SQ_FREE_CUS);
   bool ok = q->lds <= ctx->lds_max;
-  const size_t nw = (size_t)q->grid * WALK_FW;
+  const size_t nw = (size_t)q->grid;  // scratch per workgroup
   ok = ok && (q->items = host_coherent<SQItem>(SQ_CAP)) && (q->tail = host_coherent<uint32_t>(64)) &&
        (q->flag = host_coherent<uint32_t>(NSLOT)) && (q->pairs = host_coherent<PairDev>(NSLOT)) &&
        (q->ptb = host_coherent<PairTB>(NSLOT)) && (q->res = host_coherent<ResultDev>(NSLOT)) &&
@@ -528,7 +587,7 @@ DevQ *devq_create(ta_ctx *ctx, int64_t tb_budget) {
        dev((void **)&q->dcodes, HDATA) &&
        dev((void **)&q->wrow, nw * devq_scratch_rows() * 2) && dev((void **)&q->wcol, nw * devq_scratch_cols() * 2) &&
        dev((void **)&q->wring, nw * devq_scratch_ring() * 4) && dev((void **)&q->fo, NSLOT * sizeof(FillOut)) &&
-       dev((void **)&q->table, 1024) && dev((void **)&q->err, 256);
+       dev((void **)&q->table, 1024) && dev((void **)&q->err, 256) && dev((void **)&q->dargs, sizeof(SQArgs));
   // the plane arena: the traceback budget, at most 16 GB, halved until it fits
   q->tb_cap = (size_t)std::min<int64_t>(tb_budget, (int64_t)16 << 30);
   while (ok && q->tb_cap >= ((size_t)256 << 20) && hipMalloc((void **)&q->tb, q->tb_cap) != hipSuccess) {

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -30,7 +31,7 @@ int main(int argc, char **argv) {
   ta_params base;
   ta_workload_params(w, &base);
   const int R = ta_workload_nrequests(w);
-  struct Req { double at; int first, count, mode; uint64_t ticket; double done; int64_t cells; };
+  struct Req { double at; int first, count, mode; uint64_t ticket; double done, sub; int64_t cells; };
   std::vector<Req> rq(R);
   int64_t total_cells = 0;
   for (int q = 0; q < R; ++q) {
@@ -90,6 +91,7 @@ int main(int argc, char **argv) {
     if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
     ta_params p = base;
     p.mode = rq[q].mode;
+    rq[q].sub = now_s();
     if (ta_submit(ctx, &p, &pairs[rq[q].first], rq[q].count, &rq[q].ticket)) { failed = true; break; }
     submitted.store(q + 1);
   }
@@ -104,14 +106,30 @@ int main(int argc, char **argv) {
     lat[q] = rq[q].done - rq[q].at;
     last = std::max(last, rq[q].done);
   }
+  // the five slowest requests: arrival, submission lag, latency, pairs, mode, longest side
+  std::vector<int> order(R);
+  for (int q = 0; q < R; ++q) order[q] = q;
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return lat[x] > lat[y]; });
+  std::string slow;
+  for (int k = 0; k < std::min(R, 5); ++k) {
+    const Req &r = rq[order[k]];
+    int longest = 0;
+    for (int i = 0; i < r.count; ++i) longest = std::max({longest, pairs[r.first + i].la, pairs[r.first + i].lb});
+    char b[160];
+    snprintf(b, sizeof b, "%s[%d, %.3f, %.2f, %.2f, %d, %d, %d]", k ? ", " : "", order[k], r.at, (r.sub - r.at) * 1e3,
+             lat[order[k]] * 1e3, r.count, r.mode, longest);
+    slow += b;
+  }
   std::sort(lat.begin(), lat.end());
   auto pct = [&](double p) { return lat[std::min(R - 1, (int)(p * (R - 1) + 0.5))]; };
   const double span = rq[R - 1].at;
   printf("{\"requests\": %d, \"pairs\": %d, \"cells\": %lld, \"arrival_span_s\": %.3f, \"makespan_s\": %.3f, "
          "\"offered_gcups\": %.2f, \"served_gcups\": %.2f, \"p50_ms\": %.2f, \"p99_ms\": %.2f, \"max_ms\": %.2f, "
-         "\"budget_gb\": %.1f, \"inflight\": %d, \"max_len\": %d}\n",
+         "\"budget_gb\": %.1f, \"inflight\": %d, \"max_len\": %d, \"p90_ms\": %.2f, "
+         "\"slowest\": [%s], \"slowest_fields\": \"request, arrival_s, submit_lag_ms, latency_ms, pairs, mode, longest\"}\n",
          R, np, (long long)total_cells, span, last, total_cells / span / 1e9, total_cells / last / 1e9,
-         pct(0.5) * 1e3, pct(0.99) * 1e3, lat[R - 1] * 1e3, budget_gb, inflight, max_len ? max_len : 5000);
+         pct(0.5) * 1e3, pct(0.99) * 1e3, lat[R - 1] * 1e3, budget_gb, inflight, max_len ? max_len : 5000,
+         pct(0.9) * 1e3, slow.c_str());
   (void)end;
   ta_workload_destroy(w);
   return 0;
