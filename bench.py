@@ -441,7 +441,10 @@ def main():
         # Served form: open-loop Poisson arrivals (2,000 requests/s, 1-256
         # pairs each, lengths 50-5,000, global/local 50/50) through
         # ta_submit/ta_poll in a child process; latency = completion minus
-        # scheduled arrival.
+        # scheduled arrival.  The default admission path is the device queue
+        # (requests pushed into the running persistent kernel); the batch
+        # path (TWEAKALIGN_DEVICE_QUEUE=0) is measured beside it on the
+        # first 20,000 requests of the same schedule.
         import subprocess
         sb = os.path.join(ROOT, "tweakseq_amd", "tweakalign-serve-bench")
         if os.path.exists(sb) and args.served_requests > 0:
@@ -455,9 +458,18 @@ def main():
                 # fraction is of the GPU's int32 roofline at 9.5 ops per cell
                 # (half global, half local)
                 out["served"]["frac_of_roofline"] = round(out["served"]["served_gcups"] * 9.5 / VALU_PEAK_GOPS, 4)
+                out["served"]["admission"] = "device queue"
                 log("served: p50 %.1f ms, p99 %.1f ms, %.1f GCUPS served of %.1f offered"
                     % (out["served"]["p50_ms"], out["served"]["p99_ms"], out["served"]["served_gcups"],
                        out["served"]["offered_gcups"]))
+            nb = min(args.served_requests, 20000)
+            env = dict(os.environ, TWEAKALIGN_DEVICE_QUEUE="0")
+            r = subprocess.run([sb, str(nb)], capture_output=True, text=True, timeout=300 + nb / 1000, env=env)
+            if r.returncode == 0:
+                out["served_batches"] = json.loads(r.stdout.strip().splitlines()[-1])
+                out["served_batches"]["admission"] = "batches (TWEAKALIGN_DEVICE_QUEUE=0)"
+                log("served through batches (%d requests): p50 %.1f ms, p99 %.1f ms"
+                    % (nb, out["served_batches"]["p50_ms"], out["served_batches"]["p99_ms"]))
         # Measured VALU throughput (instruction-kind microbenchmark) beside the theoretical peak.
         tool = os.path.join(ROOT, "tweakseq_amd", "tweakalign-valu-peak")
         if os.path.exists(tool):
