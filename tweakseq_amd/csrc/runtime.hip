@@ -310,6 +310,42 @@ struct StageHold {
   }
 };
 
+// Host vectors bound for their own device buffers, gathered into one pinned
+// staging block and sent with one asynchronous copy each (a copy from
+// pageable memory is staged by the runtime and waits for the link).  The
+// vectors must live until commit(); the block until the stream drains.
+struct BulkUpload {
+  struct Item { void **dst; const void *src; size_t bytes, at; };
+  std::vector<Item> items;
+  StageHold hold;
+  template <typename T>
+  void add(T **dst, const std::vector<T> &v) {
+    *dst = nullptr;
+    items.push_back(Item{(void **)dst, v.data(), v.size() * sizeof(T), 0});
+  }
+  int commit(hipStream_t st) {
+    size_t total = 0;
+    for (Item &x : items) {
+      x.at = total;
+      total += (x.bytes + 255) & ~(size_t)255;
+    }
+    if (!total) return TA_OK;
+    hold.s = stage_take(total);
+    if (!hold.s.p) return TA_E_NOMEM;
+    for (Item &x : items)
+      if (x.bytes) memcpy(hold.s.p + x.at, x.src, x.bytes);
+    hold.stream = st;
+    for (Item &x : items) {
+      if (!x.bytes) continue;
+      if (!(*x.dst = cache_alloc(x.bytes))) return TA_E_NOMEM;
+      hold.queued = true;
+      if (hipMemcpyAsync(*x.dst, hold.s.p + x.at, x.bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+        return TA_E_DEVICE;
+    }
+    return TA_OK;
+  }
+};
+
 
 }  // namespace
 
@@ -460,6 +496,27 @@ int fill_waves_per_cu(int P, int Kp, bool nib, int want, bool walk = false) {
 // n_force / S_force (wave units): columns and strips of the task's unit; extra
 // columns lead (local couples only), extra strips are pad rows.
 
+// Raw letters to residue codes in place, 16 bytes per thread step, through a
+// 256-entry map (0xff: outside the alphabet, which sets *bad).
+__global__ void encode_kernel(uint4 *codes, size_t words, const uint8_t *map, int32_t *bad) {
+  __shared__ uint8_t m[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) m[i] = map[i];
+  __syncthreads();
+  uint32_t any = 0;
+  for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (size_t)gridDim.x * blockDim.x) {
+    uint4 v = codes[w];
+    uint32_t x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t b0 = m[x[q] & 0xff], b1 = m[(x[q] >> 8) & 0xff], b2 = m[(x[q] >> 16) & 0xff], b3 = m[x[q] >> 24];
+      any |= b0 | b1 | b2 | b3;
+      x[q] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+    }
+    codes[w] = make_uint4(x[0], x[1], x[2], x[3]);
+  }
+  if (__any(any & 0x80) && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
+}
+
 // The row and column codes of every planned task, expanded on the GPU from
 // the uploaded residues: row r of a half is its pair's query residue r (pad
 // past m), column j its reference residue j - off (pad before the leading
@@ -485,6 +542,19 @@ __global__ void expand_codes_kernel(const CodeJob *jobs, int njobs, const PairDe
 
 // Allocates a plan's row and column codes on the device and expands them
 // there (the residues and pair descriptors are already uploaded).
+// Allocates a plan's row and column codes and expands them from a job list
+// already on the device.
+int expand_jobs(const CodeJob *d_jobs, size_t njobs, uint64_t rows_n, uint64_t cols_n, const PairDev *d_pairs,
+                const uint8_t *d_codes, int Kp, uint16_t **d_rows, uint16_t **d_cols, hipStream_t st) {
+  int rc = dalloc(d_rows, (size_t)rows_n);
+  rc = rc ? rc : dalloc(d_cols, (size_t)cols_n);
+  if (rc || !njobs) return rc;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(expand_codes_kernel, dim3((unsigned)std::min<size_t>(njobs, 65536)), dim3(256), 0, st,
+                     d_jobs, (int)njobs, d_pairs, d_codes, Kp - 1, *d_rows, *d_cols);
+  return hipGetLastError() == hipSuccess ? TA_OK : TA_E_DEVICE;
+}
+
 int expand_codes(const std::vector<CodeJob> &jobs, uint64_t rows_n, uint64_t cols_n, const PairDev *d_pairs,
                  const uint8_t *d_codes, int Kp, uint16_t **d_rows, uint16_t **d_cols, hipStream_t st) {
   int rc = dalloc(d_rows, (size_t)rows_n);
@@ -601,35 +671,33 @@ void add_units(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, boo
     }
     gather_by_key(u, std::move(key));
   }
-  std::vector<std::vector<Task>> units;
+  std::vector<Task> flat;  // unit t: flat[NGW t .. NGW t + NGW - 1]
+  flat.reserve(u.size() + NGW);
+  std::vector<uint64_t> key;  // more 8-step blocks first (S * NB < 2^40)
   for (size_t k = 0; k < u.size();) {
     size_t e = k;
     while (e < u.size() && e - k < NGW && (local || u[e].n == u[k].n)) ++e;
     int nq = 0, mq = 0;
     for (size_t i = k; i < e; ++i) { nq = std::max(nq, u[i].n); mq = std::max(mq, u[i].m); }
-    std::vector<Task> q;
+    const size_t first = flat.size();
     for (size_t i = k; i < e; ++i)
-      add_task(pl, q, P, G, trace, hp, u[i].lo, u[i].hi, u[i].single, Kp, nq, strips(mq));
-    while (q.size() < (size_t)NGW) {
-      Task f = q[0];
+      add_task(pl, flat, P, G, trace, hp, u[i].lo, u[i].hi, u[i].single, Kp, nq, strips(mq));
+    for (size_t i = e - k; i < (size_t)NGW; ++i) {
+      Task f = flat[first];
       f.single = 2;
-      q.push_back(f);
+      flat.push_back(f);
     }
-    units.push_back(std::move(q));
+    key.push_back((1ull << 40) - (uint64_t)flat[first].S * flat[first].NB);
     k = e;
   }
-  {  // more 8-step blocks first (S * NB < 2^40)
-    std::vector<uint64_t> key(units.size());
-    for (size_t t = 0; t < units.size(); ++t) key[t] = (1ull << 40) - (uint64_t)units[t][0].S * units[t][0].NB;
-    gather_by_key(units, std::move(key));
-  }
-  for (auto &q : units) dst.insert(dst.end(), q.begin(), q.end());
+  const std::vector<int32_t> o = order_by_key(std::move(key));
+  dst.reserve(dst.size() + flat.size());
+  for (int32_t t : o) dst.insert(dst.end(), flat.begin() + (size_t)t * NGW, flat.begin() + (size_t)(t + 1) * NGW);
 }
 
 // Steps a G-lane group needs for a task of these shapes, in 8-step blocks.
 double group_cost(int mmax, int n) { return (double)ceil_div(mmax, G * R) * ceil_div(n + G, 8); }
 
-double task_cost(const Task &T) { return (double)T.S * T.NB; }
 
 // Whole-wave groups for tasks long enough to set the makespan of a group
 // sweep: more than 0.75x the mean work of a group slot (a list-scheduling
@@ -794,6 +862,14 @@ void ta_batch_destroy(ta_batch *B) {
   delete B;
 }
 
+// A pair the batch runtime takes: lengths in range, letters present, and
+// (global) scores clear of the int32 kernels' -infinity (-2^29).
+static bool pair_ok(const ta_params &p, const ta_pair &q) {
+  if (q.la < 0 || q.lb < 0 || q.la > 1000000 || q.lb > 1000000 || (q.la && !q.a) || (q.lb && !q.b)) return false;
+  return p.mode == TA_LOCAL ||
+         (int64_t)(q.la + q.lb + 2) * -p.gap_extend + 2 * (int64_t)-p.gap_open < (1ll << 28);
+}
+
 int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, ta_batch **out) {
   return batch_create_on(ctx, p, pairs, n, ctx ? ctx->stream : nullptr, out);
 }
@@ -804,6 +880,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   *out = nullptr;
   if (params_check(p)) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
+  PhaseTrace tr("create");
 
   ta_batch *B = new ta_batch();
   B->stream = stream;
@@ -835,99 +912,152 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   int best[32] = {0};
   for (int r = 0; r < B->alph.K; ++r)
     for (int c = 0; c < B->alph.K; ++c) best[r] = std::max<int>(best[r], B->alph.S[r * B->alph.K + c]);
-  // Unique sequences first (by pointer and length), their offsets in the
-  // code array, then their codes and bounds on the host threads.
+  // Unique sequences (by pointer and length) and their offsets in the code
+  // array.  Pointers are split by hash over the host threads; each thread
+  // scans every pair and keeps a table for its share only (a few hundred KB,
+  // cache-resident), mapping a pointer to its latest unique entry, so the
+  // same pointer always meets the same thread, in pair order.
   struct Uniq { const char *s; int32_t len; uint64_t off; int64_t bound; };
-  std::vector<Uniq> uq;
-  // pointer -> latest unique entry: open addressing, linear probing, at
-  // most half full (a node-based map costs more than the encoding here)
-  size_t hcap = 16;
-  while (hcap < 4 * (size_t)n) hcap <<= 1;
-  std::vector<const char *> hkey(hcap);
-  std::vector<int32_t> hval(hcap, -1);
-  std::vector<int32_t> ua((size_t)n), ub((size_t)n);
-  size_t ctotal = 0;
-  auto uniq = [&](const char *s, int32_t len) -> int32_t {
+  for (int32_t k = 0; k < n; ++k)
+    if (!pair_ok(*p, pairs[k])) {
+      delete B;
+      return TA_E_INVALID;
+    }
+  auto mix = [](const char *s) -> uint64_t {
     uint64_t x = (uint64_t)(uintptr_t)s;
     x = (x ^ (x >> 29)) * 0xbf58476d1ce4e5b9ull;
-    size_t h = (size_t)(x ^ (x >> 32)) & (hcap - 1);
-    while (hval[h] >= 0 && hkey[h] != s) h = (h + 1) & (hcap - 1);
-    if (hval[h] >= 0 && uq[hval[h]].len == len) return hval[h];
-    const size_t at = (ctotal + 15) & ~(size_t)15;  // 16-byte aligned starts (walker windows)
-    uq.push_back(Uniq{s, len, at, 0});
-    ctotal = at + (size_t)len;
-    hkey[h] = s;
-    hval[h] = (int32_t)uq.size() - 1;
-    return (int32_t)uq.size() - 1;
+    return x ^ (x >> 32);
   };
-  for (int32_t k = 0; k < n; ++k) {
-    const ta_pair &q = pairs[k];
-    if (q.la < 0 || q.lb < 0 || q.la > 1000000 || q.lb > 1000000 || (q.la && !q.a) || (q.lb && !q.b)) {
-      delete B;
-      return TA_E_INVALID;
+  const int T = n >= 8192 ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
+  std::vector<int32_t> ua((size_t)n), ub((size_t)n);
+  std::vector<std::vector<Uniq>> part(T);
+  parallel_tasks(T, [&](int t) {
+    struct Slot { const char *s; int32_t idx; };
+    size_t cap = 64;
+    std::vector<Slot> tab(cap, Slot{nullptr, -1});
+    std::vector<Uniq> &mine = part[t];
+    auto grow = [&]() {  // keep the table at most half full
+      std::vector<Slot> old(2 * cap, Slot{nullptr, -1});
+      old.swap(tab);
+      cap *= 2;
+      for (const Slot &e : old) {
+        if (e.idx < 0) continue;
+        size_t h = (size_t)mix(e.s) & (cap - 1);
+        while (tab[h].idx >= 0) h = (h + 1) & (cap - 1);
+        tab[h] = e;
+      }
+    };
+    auto uniq = [&](const char *s, int32_t len, uint64_t x) -> int32_t {
+      size_t h = (size_t)x & (cap - 1);
+      while (tab[h].idx >= 0 && tab[h].s != s) h = (h + 1) & (cap - 1);
+      if (tab[h].idx >= 0 && mine[tab[h].idx].len == len) return tab[h].idx;
+      const bool fresh = tab[h].idx < 0;
+      mine.push_back(Uniq{s, len, 0, -1});
+      tab[h] = Slot{s, (int32_t)mine.size() - 1};
+      if (fresh && 2 * mine.size() > cap) grow();
+      return (int32_t)mine.size() - 1;
+    };
+    for (int32_t k = 0; k < n; ++k) {
+      const ta_pair &q = pairs[k];
+      const uint64_t xa = mix(q.a), xb = mix(q.b);
+      if ((int)((xa >> 40) % (uint64_t)T) == t) ua[k] = uniq(q.a, q.la, xa);
+      if ((int)((xb >> 40) % (uint64_t)T) == t) ub[k] = uniq(q.b, q.lb, xb);
     }
-    // Global scores must stay clear of the int32 kernels' -infinity (-2^29).
-    if (!local && (int64_t)(q.la + q.lb + 2) * -p->gap_extend + 2 * (int64_t)-p->gap_open >= (1ll << 28)) {
-      delete B;
-      return TA_E_INVALID;
+  });
+  std::vector<int32_t> base(T + 1, 0);
+  for (int t = 0; t < T; ++t) base[t + 1] = base[t] + (int32_t)part[t].size();
+  std::vector<Uniq> uq;
+  uq.reserve((size_t)base[T]);
+  size_t ctotal = 0;
+  for (int t = 0; t < T; ++t)
+    for (Uniq &e : part[t]) {
+      e.off = (ctotal + 15) & ~(size_t)15;  // 16-byte aligned starts (walker windows)
+      ctotal = e.off + (size_t)e.len;
+      uq.push_back(e);
     }
-    ua[k] = uniq(q.a, q.la);
-    ub[k] = uniq(q.b, q.lb);
-  }
+  if (T > 1)
+    parallel_for((size_t)n, [&](size_t k) {
+      ua[k] += base[(mix(pairs[k].a) >> 40) % (uint64_t)T];
+      ub[k] += base[(mix(pairs[k].b) >> 40) % (uint64_t)T];
+    });
+  tr.mark("dedupe");
   // The walker reads 32-byte windows: 32 bytes of slack after the last
-  // sequence.  Each host thread writes its sequences' codes and the zero
-  // padding after them (so the buffer's pages are first touched in parallel).
+  // sequence.  The raw letters go up as they are, each sequence followed by
+  // a letter that encodes to 0 up to the next 16-byte start (each host thread
+  // copies its sequences, so the pages are first touched in parallel), and
+  // the GPU encodes them in place, flagging letters outside the alphabet.
   const size_t cend = ((ctotal + 15) & ~(size_t)15) + 32;
-  StageHold codes;  // residue codes of the unique sequences, 16-byte aligned starts
+  StageHold codes;  // raw letters of the unique sequences, 16-byte aligned starts
   codes.s = stage_take(cend);
   if (!codes.s.p) {
     delete B;
     return TA_E_NOMEM;
   }
-  // One entry per raw byte: best score << 8 | code, 0xff for a letter
-  // outside the alphabet (codes are below 0x80), so the loop has no branch.
-  uint32_t enc[256];
-  for (int b = 0; b < 256; ++b) {
+  uint8_t map[256], pad_letter = 0;
+  for (int b = 255; b >= 0; --b) {
     const int c = B->alph.map[b];
-    enc[b] = c < 0 ? 0xffu : (uint32_t)best[c] << 8 | (uint32_t)c;
+    map[b] = c < 0 ? 0xff : (uint8_t)c;
+    if (c == 0) pad_letter = (uint8_t)b;
   }
-  std::atomic<int> bad_letter{0};
   parallel_for(uq.size(), [&](size_t u) {
-    Uniq &e = uq[u];
+    const Uniq &e = uq[u];
     uint8_t *dst = codes.s.p + e.off;
-    const unsigned char *src = (const unsigned char *)e.s;
-    uint64_t sum = 0;
-    uint32_t any = 0;
-    for (int32_t i = 0; i < e.len; ++i) {
-      const uint32_t t = enc[src[i]];
-      dst[i] = (uint8_t)t;
-      any |= t;
-      sum += t >> 8;
-    }
-    if (any & 0x80) bad_letter = 1;
-    e.bound = (int64_t)sum;
+    if (e.len) memcpy(dst, e.s, (size_t)e.len);
     const size_t next = u + 1 < uq.size() ? uq[u + 1].off : cend;
-    memset(dst + e.len, 0, next - (e.off + (size_t)e.len));
+    memset(dst + e.len, pad_letter, next - (e.off + (size_t)e.len));
   });
-  if (uq.empty()) memset(codes.s.p, 0, cend);
-  if (bad_letter) {
-    delete B;
-    return TA_E_ALPHABET;
-  }
-  // The codes go up while the host plans: from the pinned block the copy
-  // runs on the DMA engine alone.
+  if (uq.empty()) memset(codes.s.p, pad_letter, cend);
+  tr.mark("letters");
+  // The sum over a sequence's residues of the best score each can reach
+  // (0 if none is positive) bounds any local alignment of it; it decides
+  // whether int16 can saturate, which only a sequence longer than
+  // limit / (best of all letters) can reach, so only such sequences are summed.
+  int maxbest = 0;
+  for (int r = 0; r < B->alph.K; ++r) maxbest = std::max(maxbest, best[r]);
+  const int64_t sat_limit = 32767 - 2L * B->alph.maxS;
+  auto bound = [&](int32_t u) -> int64_t {
+    Uniq &e = uq[u];
+    if ((int64_t)e.len * maxbest <= sat_limit) return (int64_t)e.len * maxbest;
+    if (e.bound < 0) {
+      int64_t sum = 0;
+      const unsigned char *src = (const unsigned char *)e.s;
+      for (int32_t i = 0; i < e.len; ++i) sum += map[src[i]] < 0x80 ? best[map[src[i]]] : 0;
+      e.bound = sum;
+    }
+    return e.bound;
+  };
+  // The letters go up while the host plans: from the pinned block the copy
+  // runs on the DMA engine alone; the encoding kernel follows on the stream.
   hipStream_t st = B->stream;
   int rc = dalloc(&B->d_codes, cend);
+  uint8_t *d_map = nullptr;
+  int32_t *d_bad = nullptr;
+  int32_t bad_letter = 0;
   if (!rc) {
     codes.stream = st;
     codes.queued = true;
     if (hipMemcpyAsync(B->d_codes, codes.s.p, cend, hipMemcpyHostToDevice, st) != hipSuccess) rc = TA_E_DEVICE;
   }
+  rc = rc ? rc : upload(&d_map, std::vector<uint8_t>(map, map + 256), st);
+  rc = rc ? rc : dalloc(&d_bad, 1);
+  if (!rc && hipMemsetAsync(d_bad, 0, sizeof(int32_t), st) != hipSuccess) rc = TA_E_DEVICE;
+  if (!rc) {
+    (void)hipGetLastError();
+    const size_t words = cend / 16;
+    hipLaunchKernelGGL(encode_kernel, dim3((unsigned)std::min<size_t>((words + 255) / 256, 4096)), dim3(256), 0, st,
+                       (uint4 *)B->d_codes, words, d_map, d_bad);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&bad_letter, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess)
+      rc = TA_E_DEVICE;
+  }
   if (rc) {
     (void)hipStreamSynchronize(st);
+    dfree(d_map);
+    dfree(d_bad);
     ta_batch_destroy(B);
     return rc;
   }
+  tr.mark("upload-encode");
   B->hp.resize(n);
   B->fo_init.assign(n, FillOut{0, 0, 0, 0});
   B->pairs_h.resize(n);
@@ -936,7 +1066,6 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     HostPair &h = B->hp[k];
     h.a_off = uq[ua[k]].off;
     h.b_off = uq[ub[k]].off;
-    const int64_t bound_a = uq[ua[k]].bound, bound_b = uq[ub[k]].bound;
     h.la = q.la; h.lb = q.lb;
     B->cells += (int64_t)q.la * q.lb;
     if (q.la == 0 || q.lb == 0) {
@@ -956,7 +1085,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       h.path = 1;  // packed end-cell keys hold the column in 15 bits
     } else if (local) {
       h.path = 2;
-      h.maysat = std::min(bound_a, bound_b) > 32767 - 2L * B->alph.maxS;
+      h.maysat = std::min(bound(ua[k]), bound(ub[k])) > sat_limit;
       if (h.maysat) B->may_saturate = true;
     } else {
       h.path = global_i16_safe(*p, B->alph.maxS, q.la, q.lb) ? 2 : 1;
@@ -968,10 +1097,12 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     B->cig_bytes += (uint64_t)d.cig_cap;
   }
 
+  tr.mark("pairs");
   // Couples: local pairs pad to the longer column count; global couples
   // need equal column counts (so H(m, n) of both sits in the last column).
   Plan &pl = B->plan;
   pl.ptb.assign(n, PairTB{0, 0, 0, 0, 0, 0, 0});
+  pl.cjobs.reserve((size_t)n / 2 + NGW);
   std::vector<int> i16, i32;
   for (int32_t k = 0; k < n; ++k) {
     if (B->hp[k].path == 2) i16.push_back(k);
@@ -1001,6 +1132,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       for (size_t t = 0; t < v.size(); t += 2) couples.push_back({v[t], t + 1 < v.size() ? v[t + 1] : -1});
     }
   }
+  tr.mark("couples");
   // Mean group-slot work decides which tasks get whole-wave groups.
   double work = 0;
   auto shape = [&](int lo, int hi) {
@@ -1024,6 +1156,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   const double mean_slot = work / slots;
   const bool few = B->wave_tasks || couples.size() + i32.size() <= (size_t)ctx->cus * wpc;
   std::vector<Unit> u2, u1;
+  pl.tasks2[0].reserve(couples.size());
+  pl.tasks1[0].reserve(i32.size());
   for (auto &c : couples) {
     const bool single = c.second < 0;
     const int hi = single ? c.first : c.second;
@@ -1041,9 +1175,14 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   }
   add_units(pl, pl.tasks2[1], 2, u2, local, B->trace, B->hp, B->Kp);
   add_units(pl, pl.tasks1[1], 1, u1, local, B->trace, B->hp, B->Kp);
-  auto by_cost = [](const Task &x, const Task &y) { return task_cost(x) > task_cost(y); };
-  std::stable_sort(pl.tasks2[0].begin(), pl.tasks2[0].end(), by_cost);
-  std::stable_sort(pl.tasks1[0].begin(), pl.tasks1[0].end(), by_cost);
+  // costlier (more 8-step blocks, S * NB < 2^40) first, stable
+  auto by_cost = [](std::vector<Task> &v) {
+    std::vector<uint64_t> key(v.size());
+    for (size_t t = 0; t < v.size(); ++t) key[t] = (1ull << 40) - (uint64_t)v[t].S * v[t].NB;
+    gather_by_key(v, std::move(key));
+  };
+  by_cost(pl.tasks2[0]);
+  by_cost(pl.tasks1[0]);
   // With few tasks the run lasts as long as its longest whole-wave task, so
   // tasks of two or more strips get a workgroup's fill waves each (strips
   // pipelined across them), as long as there are at most two per CU.
@@ -1060,6 +1199,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     }
   }
 
+  tr.mark("tasks");
   // Long pairs: strips of all pairs form one queue, largest pairs first.
   {
     std::vector<int> lk;
@@ -1099,40 +1239,49 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     if (const char *e = getenv("TWEAKALIGN_WALK_GUARD")) B->guard_cap = std::max(0, atoi(e));
   }
 
-  // Upload.
-  std::vector<int8_t> table = padded_table(B->alph);
-  rc = rc ? rc : upload(&B->d_table, table, st);
-  if (B->nib && !B->trace) rc = rc ? rc : upload(&B->d_table_nib, nibble_table(B->alph), st);
-  for (int g = 0; g < 2; ++g) {
-    rc = rc ? rc : upload(&B->d_tasks2[g], pl.tasks2[g], st);
-    rc = rc ? rc : upload(&B->d_tasks1[g], pl.tasks1[g], st);
+  tr.mark("long pairs");
+  // Upload: every host-built table in one pinned block.
+  std::vector<int8_t> table = padded_table(B->alph), table_nib;
+  BulkUpload up;
+  up.add(&B->d_table, table);
+  if (B->nib && !B->trace) {
+    table_nib = nibble_table(B->alph);
+    up.add(&B->d_table_nib, table_nib);
   }
-  rc = rc ? rc : upload(&B->d_ptb, pl.ptb, st);
-  rc = rc ? rc : upload(&B->d_pairs, B->pairs_h, st);
-  rc = rc ? rc : expand_codes(pl.cjobs, pl.rows_n, pl.cols_n, B->d_pairs, B->d_codes, B->Kp, &B->d_rowcodes,
-                              &B->d_colcodes, st);
-  rc = rc ? rc : upload(&B->d_fo_init, B->fo_init, st);
+  for (int g = 0; g < 2; ++g) {
+    up.add(&B->d_tasks2[g], pl.tasks2[g]);
+    up.add(&B->d_tasks1[g], pl.tasks1[g]);
+  }
+  up.add(&B->d_ptb, pl.ptb);
+  up.add(&B->d_pairs, B->pairs_h);
+  CodeJob *d_jobs = nullptr;  // freed after the final drain
+  up.add(&d_jobs, pl.cjobs);
+  up.add(&B->d_fo_init, B->fo_init);
+  std::vector<int32_t> order, rest;
   {
     std::vector<uint64_t> key(n);  // longer pairs (la + lb <= 2,000,000) first
     for (int32_t k = 0; k < n; ++k) key[k] = 2000000 - (uint64_t)(B->hp[k].la + B->hp[k].lb);
-    const std::vector<int32_t> order = order_by_key(std::move(key));
-    rc = rc ? rc : upload(&B->d_order, order, st);
+    order = order_by_key(std::move(key));
+    up.add(&B->d_order, order);
     if (B->walk) {  // pairs without a fill task: empty and long pairs
-      std::vector<int32_t> rest;
       for (int32_t k : order)
         if (pl.ptb[k].P != 1 && pl.ptb[k].P != 2) rest.push_back(k);
       B->n_rest = (int32_t)rest.size();
-      if (!rest.empty()) rc = rc ? rc : upload(&B->d_order_rest, rest, st);
+      up.add(&B->d_order_rest, rest);
     }
   }
+  up.add(&B->d_lp, B->longs);
+  up.add(&B->d_litems, B->litems);
+  rc = rc ? rc : up.commit(st);
+  rc = rc ? rc : expand_jobs(d_jobs, pl.cjobs.size(), pl.rows_n, pl.cols_n, B->d_pairs, B->d_codes, B->Kp,
+                             &B->d_rowcodes, &B->d_colcodes, st);
+  tr.mark("uploads");
   rc = rc ? rc : dalloc(&B->d_ring, pl.ring_elems);
   rc = rc ? rc : dalloc(&B->d_tb, pl.tb_bytes);
   rc = rc ? rc : dalloc(&B->d_fo, (size_t)std::max(n, 1));
   rc = rc ? rc : dalloc(&B->d_res, (size_t)std::max(n, 1));
   rc = rc ? rc : dalloc(&B->d_cig, B->cig_bytes);
   rc = rc ? rc : dalloc(&B->d_ctr, 16);
-  rc = rc ? rc : upload(&B->d_lp, B->longs, st);
-  rc = rc ? rc : upload(&B->d_litems, B->litems, st);
   rc = rc ? rc : dalloc(&B->d_lring, B->lring_elems);
   rc = rc ? rc : dalloc(&B->d_lck, B->lck_elems);
   rc = rc ? rc : dalloc(&B->d_lcol, B->lcol_elems);
@@ -1141,9 +1290,16 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   rc = rc ? rc : dalloc(&B->d_wtb, B->wtb_stride * B->long_wgs);
   for (auto &e : B->ev)
     if (!rc && hipEventCreate(&e) != hipSuccess) rc = TA_E_DEVICE;
+  tr.mark("allocations");
   // Drained on every path: the blocks of a failed creation go back to the
-  // cache only after the copies queued into them.
+  // cache only after the copies queued into them (and the letter check has
+  // landed in bad_letter).
   if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = TA_E_DEVICE;
+  dfree(d_map);
+  dfree(d_bad);
+  dfree(d_jobs);
+  tr.mark("drain");
+  if (!rc && bad_letter) rc = TA_E_ALPHABET;
   if (rc) {
     ta_batch_destroy(B);
     return rc;
@@ -1592,8 +1748,10 @@ void fill_result(ta_result &o, const ResultDev &r) {
 int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, int64_t cap, int64_t *used) {
   if (!ctx || !B || (B->n && !out)) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
+  PhaseTrace tr("results");
   Fetched F;
   int rc = fetch_results(B, F);
+  tr.mark("fetch");
   if (rc) return rc;
   const ResultDev *res = F.res;
   const uint32_t *runs = F.runs;
@@ -1617,6 +1775,7 @@ int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, 
     if (out[k].cigar_len > 0)
       format_runs(runs + off[k] / 4, res[k].cigar_len / 4, cigar_buf + out[k].cigar_off);
   });
+  tr.mark("format");
   for (int32_t k = 0; k < B->n; ++k)
     if (out[k].flags & TA_F_WALK_FAILED) return TA_E_WALK;
   return rc;
@@ -1625,8 +1784,10 @@ int ta_batch_results(ta_ctx *ctx, ta_batch *B, ta_result *out, char *cigar_buf, 
 int batch_results_text(ta_ctx *ctx, ta_batch *B, std::vector<ta_result> &out, std::string &text) {
   if (!ctx || !B) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
+  PhaseTrace tr("results");
   Fetched F;
   const int rc = fetch_results(B, F);
+  tr.mark("fetch");
   if (rc) return rc;
   const ResultDev *res = F.res;
   const uint32_t *runs = F.runs;
@@ -1648,6 +1809,7 @@ int batch_results_text(ta_ctx *ctx, ta_batch *B, std::vector<ta_result> &out, st
   parallel_for((size_t)B->n, [&](size_t k) {
     if (out[k].cigar_len > 0) format_runs(runs + off[k] / 4, res[k].cigar_len / 4, &text[out[k].cigar_off]);
   });
+  tr.mark("format");
   return TA_OK;
 }
 

@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <string>
@@ -101,6 +104,39 @@ void parallel_for(size_t n, F f) {
     });
   for (auto &x : th) x.join();
 }
+
+// Runs f(t) for t in [0, T) on T host threads (f(0) inline when T is 1).
+template <class F>
+void parallel_tasks(int T, F f) {
+  if (T <= 1) {
+    f(0);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for (int t = 0; t < T; ++t) th.emplace_back([&f, t] { f(t); });
+  for (auto &x : th) x.join();
+}
+
+// Host-side phase timings on stderr when TWEAKALIGN_TRACE=1: each mark()
+// prints the time since the previous one under the trace's name.
+struct PhaseTrace {
+  const char *name;
+  bool on;
+  std::chrono::steady_clock::time_point t;
+  explicit PhaseTrace(const char *n) : name(n) {
+    const char *e = getenv("TWEAKALIGN_TRACE");
+    on = e && e[0] == '1';
+    if (on) t = std::chrono::steady_clock::now();
+  }
+  void mark(const char *what) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    fprintf(stderr, "[tweakalign] %s %-14s %8.3f ms\n", name, what,
+            std::chrono::duration<double>(now - t).count() * 1e3);
+    t = now;
+  }
+};
 
 // Stops the serving thread (called by ta_destroy).
 void serve_shutdown(ta_ctx *ctx);

@@ -14,7 +14,7 @@ LANES_AUTO, LANES_I16, LANES_I32, LANES_LONG = 0, 16, 32, 64
 F_I32, F_RERUN, F_LONG, F_EMPTY = 0x1, 0x2, 0x4, 0x8
 F_WALK_FAILED = 0x100
 E_PENDING = 1
-E_SPACE, E_WALK = -3, -10
+E_INVALID, E_ALPHABET, E_SPACE, E_WALK = -1, -2, -3, -10
 
 Result = namedtuple("Result", "score a_begin a_end b_begin b_end cigar flags")
 
