@@ -5,6 +5,7 @@
   tweakalign-serve-bench   open-loop serving benchmark
   tweakalign-pair-bench    call -> CIGAR latency of one pair (single-pair path and batch path)
   tweakalign-valu-peak     measured VALU throughput of the fill's instruction kinds
+  tweakalign-gpualign-check  the tweakseq integration class (qt/GpuAlign) against stand-in Qt headers
 
 Usage: python -m tweakseq_amd.build   (or build() from __graft_entry__)
 """
@@ -65,6 +66,14 @@ def build_tools():
         run(["g++", "-O2", "-std=c++17", "-Wall", "-pthread", src, "-o", out, "-L" + HERE, "-ltweakalign",
              "-Wl,-rpath,$ORIGIN"])
     run(["hipcc"] + HIP_FLAGS + [os.path.join(CSRC, "valu_peak.hip"), "-o", os.path.join(HERE, "tweakalign-valu-peak")])
+    # The tweakseq integration class (qt/GpuAlign.*) built against stand-in Qt
+    # headers with its driver: the check that the maintainer-facing code
+    # compiles, renders CIGARs and calls the library.
+    native = os.path.join(ROOT, "tests", "native")
+    run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(native, "fakeqt"), "-I" + os.path.join(ROOT, "include"),
+         "-I" + os.path.join(HERE, "qt"), os.path.join(HERE, "qt", "GpuAlign.cpp"),
+         os.path.join(native, "gpualign_check.cpp"), "-o", os.path.join(HERE, "tweakalign-gpualign-check"),
+         "-L" + HERE, "-ltweakalign", "-Wl,-rpath,$ORIGIN"])
 
 
 def build():
