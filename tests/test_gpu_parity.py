@@ -164,6 +164,25 @@ def test_batched_equals_plain(eng):
         assert batched == plain
 
 
+def test_plain_form_first_on_a_fresh_batch(eng):
+    """The plain form run before any batched run, on a batch whose small
+    device blocks come back from the library's cache holding other batches'
+    data: its counters and error word start at 0 (a stale error word once
+    failed the wait), and it equals the batched form."""
+    rng = random.Random(15)
+    pairs = [(rand_prot(rng, rng.randint(200, 700)), rand_prot(rng, rng.randint(200, 700))) for _ in range(24)]
+    params = ta.engine.make_params(mode=ta.LOCAL)
+    for _ in range(3):
+        eng.align(pairs[:8], params)  # churn the cache's small blocks
+        b = eng.batch(pairs, params)
+        b.run_plain(); b.wait()
+        plain = b.results()
+        b.run(); b.wait()
+        batched = b.results()
+        b.close()
+        assert plain == batched
+
+
 def test_plain_reruns_saturated_pairs(eng):
     """The plain form (one launch per pair) reruns a saturating int16 pair in
     int32 as the batched form does: 3,000 x 'W' scores 33,000."""

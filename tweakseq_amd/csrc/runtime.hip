@@ -310,6 +310,40 @@ struct StageHold {
   }
 };
 
+// Host vectors of destroyed batches, kept for the next batches so that
+// their pages stay mapped (a fresh multi-MB vector pays a page fault per
+// 4 KB, milliseconds per batch): up to 4 per element type, none above 64 MB.
+template <typename T>
+struct VecBin {
+  std::mutex mu;
+  std::vector<std::vector<T>> v;
+  static VecBin &get() {
+    static VecBin *b = new VecBin();
+    return *b;
+  }
+};
+template <typename T>
+void vec_take(std::vector<T> &x) {
+  VecBin<T> &b = VecBin<T>::get();
+  std::lock_guard<std::mutex> g(b.mu);
+  if (!b.v.empty()) {
+    x.swap(b.v.back());
+    b.v.pop_back();
+  }
+  x.clear();
+}
+template <typename T>
+void vec_give(std::vector<T> &x) {
+  const size_t bytes = x.capacity() * sizeof(T);
+  if (bytes < ((size_t)64 << 10) || bytes > ((size_t)64 << 20)) return;
+  VecBin<T> &b = VecBin<T>::get();
+  std::lock_guard<std::mutex> g(b.mu);
+  if (b.v.size() < 4) {
+    b.v.emplace_back();
+    b.v.back().swap(x);
+  }
+}
+
 // Host vectors bound for their own device buffers, gathered into one pinned
 // staging block and sent with one asynchronous copy each (a copy from
 // pageable memory is staged by the runtime and waits for the link).  The
@@ -859,6 +893,15 @@ void ta_batch_destroy(ta_batch *B) {
     if (e) (void)hipEventDestroy(e);
   if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
+  vec_give(B->hp);
+  vec_give(B->fo_init);
+  vec_give(B->pairs_h);
+  vec_give(B->plan.ptb);
+  vec_give(B->plan.cjobs);
+  for (int g = 0; g < 2; ++g) {
+    vec_give(B->plan.tasks2[g]);
+    vec_give(B->plan.tasks1[g]);
+  }
   delete B;
 }
 
@@ -929,12 +972,32 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     return x ^ (x >> 32);
   };
   const int T = n >= 8192 ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
-  std::vector<int32_t> ua((size_t)n), ub((size_t)n);
-  std::vector<std::vector<Uniq>> part(T);
+  // a pointer's thread: the hash's top 32 bits scaled to [0, T) (no division)
+  auto part_of = [T](uint64_t x) { return (int)(((x >> 32) * (uint64_t)T) >> 32); };
+  // The calling thread's buffers, kept between calls: their pages stay
+  // mapped (first-touch page faults cost more than the work here).
+  // (Named through references: the worker threads below must reach the
+  // calling thread's copies, not their own.)
+  static thread_local std::vector<int32_t> tl_ua, tl_ub;
+  static thread_local std::vector<std::vector<Uniq>> tl_part;
+  static thread_local std::vector<Uniq> tl_uq;
+  std::vector<int32_t> &ua = tl_ua, &ub = tl_ub;
+  std::vector<std::vector<Uniq>> &part = tl_part;
+  std::vector<Uniq> &uq = tl_uq;
+  ua.resize((size_t)n);
+  ub.resize((size_t)n);
+  if ((int)part.size() < T) part.resize(T);
+  for (int t = 0; t < T; ++t) part[t].clear();
   parallel_tasks(T, [&](int t) {
     struct Slot { const char *s; int32_t idx; };
+    // sized for this thread's expected share at most a quarter full; the
+    // thread keeps it between calls (the host pool's threads live on)
     size_t cap = 64;
-    std::vector<Slot> tab(cap, Slot{nullptr, -1});
+    while (cap < 8 * (size_t)n / (size_t)T) cap <<= 1;
+    static thread_local std::vector<Slot> tl_tab;
+    std::vector<Slot> &tab = tl_tab;
+    if (tab.size() != cap) tab.assign(cap, Slot{nullptr, -1});
+    else std::fill(tab.begin(), tab.end(), Slot{nullptr, -1});
     std::vector<Uniq> &mine = part[t];
     auto grow = [&]() {  // keep the table at most half full
       std::vector<Slot> old(2 * cap, Slot{nullptr, -1});
@@ -960,13 +1023,13 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     for (int32_t k = 0; k < n; ++k) {
       const ta_pair &q = pairs[k];
       const uint64_t xa = mix(q.a), xb = mix(q.b);
-      if ((int)((xa >> 40) % (uint64_t)T) == t) ua[k] = uniq(q.a, q.la, xa);
-      if ((int)((xb >> 40) % (uint64_t)T) == t) ub[k] = uniq(q.b, q.lb, xb);
+      if (part_of(xa) == t) ua[k] = uniq(q.a, q.la, xa);
+      if (part_of(xb) == t) ub[k] = uniq(q.b, q.lb, xb);
     }
   });
   std::vector<int32_t> base(T + 1, 0);
   for (int t = 0; t < T; ++t) base[t + 1] = base[t] + (int32_t)part[t].size();
-  std::vector<Uniq> uq;
+  uq.clear();
   uq.reserve((size_t)base[T]);
   size_t ctotal = 0;
   for (int t = 0; t < T; ++t)
@@ -977,8 +1040,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     }
   if (T > 1)
     parallel_for((size_t)n, [&](size_t k) {
-      ua[k] += base[(mix(pairs[k].a) >> 40) % (uint64_t)T];
-      ub[k] += base[(mix(pairs[k].b) >> 40) % (uint64_t)T];
+      ua[k] += base[part_of(mix(pairs[k].a))];
+      ub[k] += base[part_of(mix(pairs[k].b))];
     });
   tr.mark("dedupe");
   // The walker reads 32-byte windows: 32 bytes of slack after the last
@@ -1046,9 +1109,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     const size_t words = cend / 16;
     hipLaunchKernelGGL(encode_kernel, dim3((unsigned)std::min<size_t>((words + 255) / 256, 4096)), dim3(256), 0, st,
                        (uint4 *)B->d_codes, words, d_map, d_bad);
-    if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(&bad_letter, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess)
-      rc = TA_E_DEVICE;
+    if (hipGetLastError() != hipSuccess) rc = TA_E_DEVICE;
   }
   if (rc) {
     (void)hipStreamSynchronize(st);
@@ -1058,6 +1119,9 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     return rc;
   }
   tr.mark("upload-encode");
+  vec_take(B->hp);
+  vec_take(B->fo_init);
+  vec_take(B->pairs_h);
   B->hp.resize(n);
   B->fo_init.assign(n, FillOut{0, 0, 0, 0});
   B->pairs_h.resize(n);
@@ -1101,6 +1165,12 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   // Couples: local pairs pad to the longer column count; global couples
   // need equal column counts (so H(m, n) of both sits in the last column).
   Plan &pl = B->plan;
+  vec_take(pl.ptb);
+  vec_take(pl.cjobs);
+  for (int g = 0; g < 2; ++g) {
+    vec_take(pl.tasks2[g]);
+    vec_take(pl.tasks1[g]);
+  }
   pl.ptb.assign(n, PairTB{0, 0, 0, 0, 0, 0, 0});
   pl.cjobs.reserve((size_t)n / 2 + NGW);
   std::vector<int> i16, i32;
@@ -1282,6 +1352,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   rc = rc ? rc : dalloc(&B->d_res, (size_t)std::max(n, 1));
   rc = rc ? rc : dalloc(&B->d_cig, B->cig_bytes);
   rc = rc ? rc : dalloc(&B->d_ctr, 16);
+  if (!rc && hipMemsetAsync(B->d_ctr, 0, 16 * sizeof(int32_t), st) != hipSuccess) rc = TA_E_DEVICE;
   rc = rc ? rc : dalloc(&B->d_lring, B->lring_elems);
   rc = rc ? rc : dalloc(&B->d_lck, B->lck_elems);
   rc = rc ? rc : dalloc(&B->d_lcol, B->lcol_elems);
@@ -1292,9 +1363,11 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     if (!rc && hipEventCreate(&e) != hipSuccess) rc = TA_E_DEVICE;
   tr.mark("allocations");
   // Drained on every path: the blocks of a failed creation go back to the
-  // cache only after the copies queued into them (and the letter check has
-  // landed in bad_letter).
+  // cache only after the copies queued into them.
   if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = TA_E_DEVICE;
+  // the letter check, read once the stream has drained (a copy into
+  // pageable memory queued earlier would have waited for the upload)
+  if (!rc && hipMemcpy(&bad_letter, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) rc = TA_E_DEVICE;
   dfree(d_map);
   dfree(d_bad);
   dfree(d_jobs);
@@ -1896,6 +1969,9 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     B->plain_ready = true;
   }
   HIPCHK(hipEventRecord(B->ev[0], st));
+  // the counters and the error word (d_ctr[10]) start at 0, as init_kernel
+  // leaves them for a batch run
+  HIPCHK(hipMemsetAsync(B->d_ctr, 0, 16 * sizeof(int32_t), st));
   if (B->n > 0)
     HIPCHK(hipMemcpyAsync(B->d_fo, B->d_fo_init, sizeof(FillOut) * B->n, hipMemcpyDeviceToDevice, st));
   std::vector<int> lp_of(B->n, -1);

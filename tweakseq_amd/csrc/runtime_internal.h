@@ -4,8 +4,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -86,6 +88,78 @@ int batch_results_text(ta_ctx *ctx, ta_batch *B, std::vector<ta_result> &out, st
 // hipFree while work runs; device_free returns them to the cache).
 void *device_alloc(size_t bytes);
 void device_free(void *p);
+// Host worker threads kept for the process's life (up to 15, plus the
+// caller): the parallel sections of batch creation and results run on them
+// instead of spawning threads per section.  One job at a time; a caller
+// that finds the pool busy gets `false` and runs its section on threads of
+// its own.
+class HostPool {
+ public:
+  static HostPool &get() {
+    static HostPool *pool = new HostPool();  // never destroyed: its workers outlive static destruction
+    return *pool;
+  }
+  int workers() const { return nw_; }
+  // f(t) for t in [0, T), T <= workers() + 1; f(0) on the caller.
+  bool run(int T, const std::function<void(int)> &f) {
+    std::unique_lock<std::mutex> own(job_mu_, std::try_to_lock);
+    if (!own.owns_lock() || T > nw_ + 1) return false;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      job_ = &f;
+      tasks_ = T;
+      pending_ = T - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    f(0);
+    std::unique_lock<std::mutex> g(mu_);
+    done_.wait(g, [&] { return pending_ == 0; });
+    job_ = nullptr;
+    return true;
+  }
+
+ private:
+  HostPool() {
+    nw_ = (int)std::min(15u, std::max(1u, std::thread::hardware_concurrency()) - 1);
+    for (int w = 1; w <= nw_; ++w) std::thread([this, w] { loop(w); }).detach();
+  }
+  void loop(int w) {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> g(mu_);
+    for (;;) {
+      cv_.wait(g, [&] { return gen_ != seen; });
+      seen = gen_;
+      if (w >= tasks_) continue;
+      const std::function<void(int)> *f = job_;
+      g.unlock();
+      (*f)(w);
+      g.lock();
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  int nw_ = 0;
+  std::mutex job_mu_, mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)> *job_ = nullptr;
+  int tasks_ = 0, pending_ = 0;
+  uint64_t gen_ = 0;
+};
+
+// Runs f(t) for t in [0, T) on T host threads (f(0) inline when T is 1).
+template <class F>
+void parallel_tasks(int T, F f) {
+  if (T <= 1) {
+    f(0);
+    return;
+  }
+  if (HostPool::get().run(T, std::function<void(int)>([&f](int t) { f(t); }))) return;
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for (int t = 0; t < T; ++t) th.emplace_back([&f, t] { f(t); });
+  for (auto &x : th) x.join();
+}
+
 // Runs f(i) for i in [0, n) on up to 16 host threads in contiguous chunks
 // (f must not depend on the order).
 template <class F>
@@ -96,26 +170,9 @@ void parallel_for(size_t n, F f) {
     for (size_t i = 0; i < n; ++i) f(i);
     return;
   }
-  std::vector<std::thread> th;
-  th.reserve(nt);
-  for (size_t t = 0; t < nt; ++t)
-    th.emplace_back([&, t] {
-      for (size_t i = n * t / nt; i < n * (t + 1) / nt; ++i) f(i);
-    });
-  for (auto &x : th) x.join();
-}
-
-// Runs f(t) for t in [0, T) on T host threads (f(0) inline when T is 1).
-template <class F>
-void parallel_tasks(int T, F f) {
-  if (T <= 1) {
-    f(0);
-    return;
-  }
-  std::vector<std::thread> th;
-  th.reserve(T);
-  for (int t = 0; t < T; ++t) th.emplace_back([&f, t] { f(t); });
-  for (auto &x : th) x.join();
+  parallel_tasks((int)nt, [&](int t) {
+    for (size_t i = n * (size_t)t / nt; i < n * ((size_t)t + 1) / nt; ++i) f(i);
+  });
 }
 
 // Host-side phase timings on stderr when TWEAKALIGN_TRACE=1: each mark()
