@@ -21,20 +21,25 @@
 //
 // A join first computes every column pair's match score (int32: nA * nB is
 // bounded so that 24 codes x 11 x nA * nB fits; a thread's rows' counts sit
-// in registers, each column's PB row is a wave-uniform scalar load), then sweeps
-// anti-diagonals: thread t owns rows t+1, t+1+256, ... (2, 4 or 8 each;
-// longer row profiles go in strips of 2,048 rows, each reading the last row
-// of the one before from global memory), keeps H and E of its
-// rows' last cells in registers, and reads the row above (H, F at the
-// previous diagonal, H at the one before) from a three-slot LDS ring of int64
-// values, one barrier per diagonal.  The walk back runs on thread 0 over
-// 64 x 64 tiles of direction bytes that the workgroup stages in LDS, one
-// barrier pair per tile instead of a global load per step.  Scores and
-// direction bytes are diagonal-major, so one diagonal's rows are adjacent.
+// in registers, each column's PB row is a wave-uniform scalar load), then
+// sweeps the DP in the register-strip form of the pairwise fill: a strip is
+// 64 lanes x JR rows, lane l holds its JR rows' H and E in registers and
+// works on column t - l + 1 at step t; the bottom row (H, F) reaches the
+// next lane one step later through a wave_shr:1 DPP move, so a wave needs
+// no barrier at all.  The workgroup's four waves take strips s, s + 4, ...;
+// strip s + 1 reads strip s's last row from an LDS ring once strip s has
+// published (an LDS progress counter per strip slot, workgroup scope,
+// bounded waits) that it stored the columns it needs.  Values are int32
+// where the join's bound allows (most joins), else int64.  Direction bytes
+// go out per lane and step, [strip][8-step block][lane][step][row].  The walk
+// back runs on thread 0 over 64 x 64 tiles of direction bytes that the
+// workgroup stages in LDS, one barrier pair per tile.
+
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <climits>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -42,11 +47,11 @@
 
 namespace {
 
-constexpr int JT = 256;                  // threads per join
-constexpr int JRMAX = 8;                 // DP rows per thread at most (a strip: JT * JR rows)
+constexpr int JT = 256;                  // threads per join: 4 waves
+constexpr int JW = JT / 64;              // waves per join
+constexpr int JRMAX = 8;                 // DP rows per lane at most (a strip: 64 * JR rows)
 constexpr int TW = 64;                   // walk tile: TW x TW direction bytes in LDS
 constexpr int KS = 24;                   // row stride of counts and PB (codes padded with zeros)
-constexpr int64_t JNEG = INT64_MIN / 4;  // -inf
 constexpr int64_t WMAX = 8000000;        // nA * nB bound: 24 codes x 11 x nA * nB fits int32
 
 struct JoinJob {
@@ -54,142 +59,241 @@ struct JoinJob {
   int64_t w;        // nA * nB: gap costs scale
   int64_t cnt_off;  // la x KS: residues of each code in A's columns
   int64_t pb_off;   // lb x KS: PB[j][x] = sum_y S(x, y) cntB[j][y]
-  int64_t m_off;    // match scores M(i, j), diagonal-major: [i + j - 2][i - 1], (la + lb - 1) x la
-  int64_t tb_off;   // direction bytes src | e << 2 | f << 3, diagonal-major: [i + j][i], (la + lb + 1) x (la + 1)
+  int64_t m_off;    // match scores M(i, j) in the sweep's order: [strip][step][lane][row]
+  int64_t tb_off;   // direction bytes src | e << 2 | f << 3, [strip][block][lane][step][row]
   int64_t ops_off;  // la + lb ops, walk order (last column first)
-  int64_t hb_off;   // 4 x (lb + 1) int64: a strip's last row (H, F), two strips' worth
 };
 
 struct JoinArgs {
   const JoinJob *jobs;
   const int32_t *cnt, *pb;
   int32_t *mbuf;
-  int64_t *hbot;
   uint8_t *tb;
   uint8_t *ops;
   int32_t *nops;
-  int32_t K, open, ext;
+  int32_t *err;     // | 1: a wait on another wave's progress gave up
+  int32_t K, open, ext, spin_limit;
 };
 
-// JR: DP rows per thread, the fewest that cover the level's longest row
-// profile (every unused row slot costs its checks on every diagonal).
-template <int JR>
-__global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
-  extern __shared__ int64_t ring[];  // H[3][la + 1], then F[3][la + 1]
-  __shared__ uint8_t tile[TW * TW];
-  __shared__ int32_t walk[4];        // i, j, state, finished
-  const JoinJob J = A.jobs[blockIdx.x];
-  const int la = J.la, lb = J.lb, tid = threadIdx.x;
-  const int64_t go = (int64_t)A.open * J.w, ge = (int64_t)A.ext * J.w;
-  constexpr int RS = JT * JR;  // DP rows per strip
-  const int rs = (la < RS ? la : RS) + 1;
-  int64_t *Hr = ring, *Fr = ring + 3 * rs;
-  const int32_t *cnt = A.cnt + J.cnt_off, *pb = A.pb + J.pb_off;
-  int32_t *mm = A.mbuf + J.m_off;
-  uint8_t *tb = A.tb + J.tb_off;
-  // Both are diagonal-major so that the rows of one diagonal (consecutive
-  // threads) read and write consecutive addresses.
-  const int64_t ms = la, ts = la + 1;
+// 8-step blocks of a strip of a join with lb columns (lane 63 trails lane 0).
+__host__ __device__ inline int join_blocks(int lb) { return (lb + 63 + 7) / 8; }
 
-  // ---- match scores: thread t scores its rows i = t, t + JT, ... (counts
-  // in registers) against every column, whose PB row is the same for the
-  // whole wave (scalar loads); the stores of a column are adjacent.
-  for (int i = tid; i < la; i += JT) {
-    int32_t a[KS];
+// Direction byte of cell (i, j), 1-based, in a join's layout.
+__device__ __forceinline__ int64_t join_tb_at(int i, int j, int JR, int NB) {
+  const int RS = 64 * JR, r = (i - 1) % RS, l = r / JR, t = j - 1 + l;
+  return (((((int64_t)((i - 1) / RS) * NB + (t >> 3)) * 64 + l) * 8 + (t & 7)) * JR + (r % JR));
+}
+
+// wave_shr:1 (lane l takes lane l - 1's value, lane 0 keeps `old`), for the
+// 32- and 64-bit value types.
+__device__ __forceinline__ int32_t shr_lane(int32_t old, int32_t v) {
+  return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int64_t shr_lane(int64_t old, int64_t v) {
+  const int32_t lo = __builtin_amdgcn_update_dpp((int32_t)old, (int32_t)v, 0x138, 0xf, 0xf, false);
+  const int32_t hi = __builtin_amdgcn_update_dpp((int32_t)(old >> 32), (int32_t)(v >> 32), 0x138, 0xf, 0xf, false);
+  return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+
+template <typename T>
+struct JoinNeg;
+template <>
+struct JoinNeg<int32_t> { static constexpr int32_t v = -(1 << 29); };
+template <>
+struct JoinNeg<int64_t> { static constexpr int64_t v = INT64_MIN / 4; };
+
+// Index of the match score of cell (i + 1, j + 1) (0-based i, j) in the order
+// a join's sweep reads them: [strip][step][lane][row], step t = j + lane.
+__device__ __forceinline__ int64_t join_m_at(int i, int j, int JR, int NST) {
+  const int RS = 64 * JR, l = (i % RS) / JR;
+  return (((int64_t)(i / RS) * NST + j + l) * 64 + l) * JR + (i % JR);
+}
+
+// Every join's match scores M(i, j) = cntA[i] . PB[j], tile by tile over
+// the whole GPU (a join's own workgroup would compute them on one CU): a
+// workgroup takes 64 rows x 64 columns of one join, stages their counts and
+// PB rows in LDS, and each thread scores 16 cells.  Tiles: (join, tile index).
+__global__ __launch_bounds__(256) void msa_scores_kernel(const JoinJob *jobs, const int2 *tiles, const int32_t *cnt,
+                                                         const int32_t *pb, int32_t *mbuf, int JR) {
+  __shared__ int32_t ca[64 * KS], pbs[64 * KS];
+  const int2 tl = tiles[blockIdx.x];
+  const JoinJob J = jobs[tl.x];
+  const int ncb = (J.lb + 63) / 64, i0 = (tl.y / ncb) * 64, j0 = (tl.y % ncb) * 64;
+  const int ni = J.la - i0 < 64 ? J.la - i0 : 64, nj = J.lb - j0 < 64 ? J.lb - j0 : 64;
+  for (int x = threadIdx.x; x < 64 * KS; x += 256) {
+    ca[x] = x < ni * KS ? cnt[J.cnt_off + (int64_t)i0 * KS + x] : 0;
+    pbs[x] = x < nj * KS ? pb[J.pb_off + (int64_t)j0 * KS + x] : 0;
+  }
+  __syncthreads();
+  const int r = threadIdx.x & 63, NST = 8 * join_blocks(J.lb);
+  if (r >= ni) return;
+  int32_t a[KS];
 #pragma unroll
-    for (int x = 0; x < KS; ++x) a[x] = cnt[(int64_t)i * KS + x];
-    for (int j = 0; j < lb; ++j) {
-      const int32_t *pj = pb + (int64_t)j * KS;
-      int32_t m = 0;
+  for (int x = 0; x < KS; ++x) a[x] = ca[r * KS + x];
+  int32_t *mm = mbuf + J.m_off;
+  for (int c = threadIdx.x >> 6; c < nj; c += 4) {
+    const int32_t *pj = pbs + c * KS;
+    int32_t m = 0;
 #pragma unroll
-      for (int x = 0; x < KS; ++x) m += a[x] * pj[x];
-      mm[(int64_t)(i + j) * ms + i] = m;
+    for (int x = 0; x < KS; ++x) m += a[x] * pj[x];
+    mm[join_m_at(i0 + r, j0 + c, JR, NST)] = m;
+  }
+}
+
+// JR: DP rows per lane, the fewest that cover the level's longest row
+// profile in one strip per wave where possible; T: int32_t when every join of
+// the level keeps its values within +-2^28, else int64_t.
+template <int JR, typename T>
+__global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
+  extern __shared__ __align__(16) uint8_t jlds[];  // ring H[lb + 2], F[lb + 2] (T)
+  __shared__ uint8_t tile[TW * TW];
+  __shared__ int32_t walk[4];                      // i, j, state, finished
+  __shared__ int32_t prog[8];                      // strip s publishes ((s + 1) << 20) | columns stored
+  constexpr T NEG = JoinNeg<T>::v;
+  const JoinJob J = A.jobs[blockIdx.x];
+  const int la = J.la, lb = J.lb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const T go = (T)((int64_t)A.open * J.w), ge = (T)((int64_t)A.ext * J.w);
+  constexpr int RS = 64 * JR;  // DP rows per strip
+  const int S = (la + RS - 1) / RS, NB = join_blocks(lb);
+  T *ringH = (T *)jlds, *ringF = ringH + (lb + 2);
+  const int32_t *mm = A.mbuf + J.m_off;
+  uint8_t *tb = A.tb + J.tb_off;
+  if (tid < 8) prog[tid] = 0;
+
+  // (the match scores were written by msa_scores_kernel, launched before)
+  const int NST = 8 * NB;  // steps per strip
+  __syncthreads();         // prog
+
+  // ---- the DP, strip by strip on each wave.
+  for (int s = wave; s < S; s += JW) {
+    const int i0 = s * RS + lane * JR + 1;  // this lane's first row
+    T Hl[JR], El[JR];                       // H(i, j - 1), E(i, j - 1)
+#pragma unroll
+    for (int k = 0; k < JR; ++k) {
+      Hl[k] = go + (T)(i0 + k - 1) * ge;  // H(i, 0)
+      El[k] = NEG;
+    }
+    T hd0 = i0 == 1 ? (T)0 : go + (T)(i0 - 2) * ge;  // H(i0 - 1, j - 1) for the next column: H(i0 - 1, 0)
+    T Hbot = 0, Fbot = NEG;                           // this lane's last row at its last column
+    const bool writer = lane == 63 && s + 1 < S;       // stores the strip's last row for strip s + 1
+    // match scores, MD steps ahead (one step ahead left every step waiting
+    // on its load); cells outside the pair read slots nobody wrote, and
+    // those steps are masked
+    constexpr int MD = 4;
+    int32_t mring[MD][JR];
+    const int32_t *ms = mm + ((int64_t)s * NST * 64 + lane) * JR;
+    auto load_m = [&](int t, int32_t (&m)[JR]) {
+      const int tt = t < NST ? t : NST - 1;
+      const int32_t *src = ms + (int64_t)tt * 64 * JR;
+      if constexpr (JR == 2) {
+        const int2 v = *(const int2 *)src;
+        m[0] = v.x; m[1] = v.y;
+      } else {
+#pragma unroll
+        for (int k = 0; k < JR; k += 4) {
+          const int4 v = *(const int4 *)(src + k);
+          m[k] = v.x; m[k + 1] = v.y; m[k + 2] = v.z; m[k + 3] = v.w;
+        }
+      }
+    };
+#pragma unroll
+    for (int d = 0; d < MD; ++d) load_m(d, mring[d]);
+    for (int b = 0; b < NB; ++b) {
+      // strip s - 1 must have stored the ring up to this block's last column
+      if (s > 0) {
+        const int need = (s << 20) | min(8 * b + 8, lb);
+        int spins = 0;
+        while (__hip_atomic_load(&prog[(s - 1) & 7], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+          if (++spins > A.spin_limit) {
+            if (lane == 0) atomicOr(A.err, 1);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      // lane 0's top inputs of the block's 8 steps: row 0 or the ring, read
+      // once the progress allows (not a dependent LDS read per step)
+      T tops[8], topfs[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int c = 8 * b + q + 1;  // lane 0's column at step 8b + q
+        if (s == 0) {
+          tops[q] = go + (T)(c - 1) * ge;  // H(0, c)
+          topfs[q] = NEG;
+        } else {
+          tops[q] = c <= lb ? ringH[c] : (T)0;
+          topfs[q] = c <= lb ? ringF[c] : NEG;
+        }
+      }
+      uint32_t codes[8][JR / 4 > 0 ? JR / 4 : 1];  // direction bytes of the block's steps, packed
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int t = 8 * b + q, j = t - lane + 1;
+        int32_t mcur[JR];
+#pragma unroll
+        for (int k = 0; k < JR; ++k) mcur[k] = mring[q % MD][k];
+        load_m(t + MD, mring[q % MD]);
+        // lane 0's top: row 0 or the ring; the others: the lane above's bottom row
+        const T Hup0 = shr_lane(tops[q], Hbot), Fup0 = shr_lane(topfs[q], Fbot);
+        uint32_t bytes = 0;
+        if (j >= 1 && j <= lb) {
+          T hup = Hup0, fup = Fup0, hd = hd0;
+#pragma unroll
+          for (int k = 0; k < JR; ++k) {
+            const T eo = Hl[k] + go, ee = El[k] + ge;
+            const bool eb = ee > eo;
+            const T E = eb ? ee : eo;
+            const T fo = hup + go, fe = fup + ge;
+            const bool fb = fe > fo;
+            const T F = fb ? fe : fo;
+            T h = hd + (T)mcur[k];
+            int src = 0;
+            if (E > h) { h = E; src = 1; }
+            if (F > h) { h = F; src = 2; }
+            hd = Hl[k];
+            Hl[k] = h;
+            El[k] = E;
+            hup = h;
+            fup = F;
+            bytes |= (uint32_t)(src | ((int)eb << 2) | ((int)fb << 3)) << (8 * (k & 3));
+            if ((k & 3) == 3 || k == JR - 1) {
+              codes[q][k >> 2] = bytes;
+              bytes = 0;
+            }
+          }
+          Hbot = hup;
+          Fbot = fup;
+          hd0 = Hup0;
+          if (writer) {
+            ringH[j] = hup;
+            ringF[j] = fup;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < (JR / 4 > 0 ? JR / 4 : 1); ++k) codes[q][k] = 0;
+        }
+      }
+      // the block's direction bytes: 8 steps x JR rows, contiguous per lane
+      uint8_t *dst = tb + ((((int64_t)s * NB + b) * 64 + lane) * 8) * JR;
+      if constexpr (JR >= 4) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int k = 0; k < JR / 4; ++k) ((uint32_t *)dst)[q * (JR / 4) + k] = codes[q][k];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) ((uint16_t *)dst)[q] = (uint16_t)codes[q][0];
+      }
+      // publish the ring columns stored so far (the writer lane's column)
+      if (s + 1 < S) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        const int done = min(max(8 * b + 8 - 63, 0), lb);
+        if (lane == 63) __hip_atomic_store(&prog[s & 7], ((s + 1) << 20) | done, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
     }
   }
   __syncthreads();
-
-  // ---- the DP in strips of RS rows (one, unless the rows outnumber the
-  // threads' registers), one anti-diagonal d = i + j per barrier.  A strip's
-  // first row reads the previous strip's last row from global memory (two
-  // buffers by strip parity: a one-row strip reads and writes the same row).
-  int64_t *hb = A.hbot + J.hb_off;
-  for (int row0 = 0, sp = 0; row0 < la; row0 += RS, sp ^= 1) {
-    const int rend = la < row0 + RS ? la : row0 + RS;  // rows row0 + 1 .. rend
-    const int64_t *Hin = hb + (int64_t)(sp ^ 1) * 2 * (lb + 1), *Fin = Hin + (lb + 1);
-    int64_t *Hout = hb + (int64_t)sp * 2 * (lb + 1), *Fout = Hout + (lb + 1);
-    int64_t Hl[JR], El[JR];  // H(i, j-1), E(i, j-1) of each owned row
-    int32_t mn[JR];          // match scores of the next diagonal's cells, loaded a diagonal early
-#pragma unroll
-    for (int r = 0; r < JR; ++r) {
-      const int i = row0 + tid + 1 + r * JT;
-      Hl[r] = go + (int64_t)(i - 1) * ge;  // H(i, 0)
-      El[r] = JNEG;
-      mn[r] = (i <= rend && row0 + 2 - i >= 1) ? mm[(int64_t)row0 * ms + (i - 1)] : 0;
-    }
-    for (int d = row0 + 2; d <= rend + lb; ++d) {
-      const int cur = d % 3, prv = (d + 2) % 3, pp = (d + 1) % 3;  // diagonals d, d-1, d-2
-      int32_t mc[JR];
-#pragma unroll
-      for (int r = 0; r < JR; ++r) {
-        const int i = row0 + tid + 1 + r * JT, j1 = d + 1 - i;
-        mc[r] = mn[r];
-        mn[r] = (i <= rend && j1 >= 1 && j1 <= lb) ? mm[(int64_t)(d - 1) * ms + (i - 1)] : 0;
-      }
-      // Loads of every owned row first (all in flight together), then the
-      // cells: a row's ring stores would otherwise order the next row's loads.
-      int64_t Hup[JR], Fup[JR], Hdg[JR];
-#pragma unroll
-      for (int r = 0; r < JR; ++r) {
-        const int li = tid + 1 + r * JT, i = row0 + li;  // row in the strip, in the matrix
-        const int j = d - i;
-        Hup[r] = Fup[r] = Hdg[r] = 0;
-        if (i > rend || j < 1 || j > lb) continue;
-        if (i == 1) {  // row 0 is the boundary
-          Hup[r] = go + (int64_t)(j - 1) * ge;
-          Fup[r] = JNEG;
-          Hdg[r] = j == 1 ? 0 : go + (int64_t)(j - 2) * ge;
-        } else if (li == 1) {  // the previous strip's last row
-          Hup[r] = Hin[j];
-          Fup[r] = Fin[j];
-          Hdg[r] = j == 1 ? go + (int64_t)(i - 2) * ge : Hin[j - 1];
-        } else {
-          Hup[r] = Hr[prv * rs + li - 1];
-          Fup[r] = Fr[prv * rs + li - 1];
-          Hdg[r] = j == 1 ? go + (int64_t)(i - 2) * ge : Hr[pp * rs + li - 1];
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < JR; ++r) {
-        const int li = tid + 1 + r * JT, i = row0 + li;
-        const int j = d - i;
-        if (i > rend || j < 1 || j > lb) continue;
-        const int64_t eo = Hl[r] + go, ee = El[r] + ge;
-        const bool eb = ee > eo;
-        const int64_t E = eb ? ee : eo;
-        const int64_t fo = Hup[r] + go, fe = Fup[r] + ge;
-        const bool fb = fe > fo;
-        const int64_t F = fb ? fe : fo;
-        int64_t h = Hdg[r] + mc[r];
-        int src = 0;
-        if (E > h) { h = E; src = 1; }
-        if (F > h) { h = F; src = 2; }
-        Hr[cur * rs + li] = h;
-        Fr[cur * rs + li] = F;
-        if (i == rend) { Hout[j] = h; Fout[j] = F; }
-        Hl[r] = h;
-        El[r] = E;
-        tb[(int64_t)d * ts + i] = (uint8_t)(src | ((int)eb << 2) | ((int)fb << 3));
-      }
-      // The ring is the only memory shared within a diagonal: an LDS-only
-      // barrier leaves the next diagonal's score loads in flight across it
-      // (the direction bytes and a strip's last row are ordered by the full
-      // barriers after the strip).
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-      __builtin_amdgcn_s_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    }
-    __syncthreads();
-  }
 
   // ---- walk back from (la, lb): the pairwise state machine (H, E 'D',
   // F 'I').  The workgroup stages the TW x TW tile of direction bytes ending
@@ -201,12 +305,13 @@ __global__ __launch_bounds__(JT) void msa_join_kernel(JoinArgs A) {
   for (;;) {
     const int wi = walk[0], wj = walk[1];
     if (walk[3]) break;
-    const int i0 = wi - (TW - 1) > 0 ? wi - (TW - 1) : 0, j0 = wj - (TW - 1) > 0 ? wj - (TW - 1) : 0;
+    const int i0 = wi - (TW - 1) > 1 ? wi - (TW - 1) : 1, j0 = wj - (TW - 1) > 1 ? wj - (TW - 1) : 1;
     const int ni = wi - i0 + 1, nj = wj - j0 + 1;
-    for (int c = tid; c < ni * nj; c += JT) {
-      const int r = c / nj, q = c - r * nj;
-      tile[r * TW + q] = tb[(int64_t)(i0 + r + j0 + q) * ts + (i0 + r)];
-    }
+    if (wi >= 1 && wj >= 1)
+      for (int c = tid; c < ni * nj; c += JT) {
+        const int r = c / nj, q = c - r * nj;
+        tile[r * TW + q] = tb[join_tb_at(i0 + r, j0 + q, JR, NB)];
+      }
     __syncthreads();
     if (tid == 0) {
       int i = wi, j = wj, state = walk[2];
@@ -259,6 +364,8 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
   int rc = ta_matrix(p->alphabet, p->match, p->mismatch, S, &K);
   if (rc) return rc;
   if (K > KS) return TA_E_INVALID;
+  int64_t max_abs_s = 1;
+  for (int32_t x = 0; x < K * K; ++x) max_abs_s = std::max<int64_t>(max_abs_s, S[x] < 0 ? -S[x] : S[x]);
   std::vector<uint8_t> codes((size_t)off[nseq]);
   for (int32_t s = 0; s < nseq; ++s)
     if ((rc = ta_encode(p->alphabet, seqs[s], lens[s], codes.data() + off[s]))) return rc;
@@ -348,8 +455,15 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       if (level[nseq + t] == L) ts.push_back(t);
     std::vector<JoinJob> jobs(ts.size());
     std::vector<int32_t> hcnt, hpb;
-    int64_t tb_total = 0, ops_total = 0, m_total = 0, cnt_total = 0, pb_total = 0, hb_total = 0;
-    int max_la = 0;
+    int64_t tb_total = 0, ops_total = 0, m_total = 0, cnt_total = 0, pb_total = 0;
+    int max_la = 0, max_lb = 0;
+    bool wide = false;  // some join's DP values need int64
+    for (size_t q = 0; q < ts.size(); ++q) {
+      max_la = std::max(max_la, prof[ja[ts[q]]].len);
+      max_lb = std::max(max_lb, prof[jb[ts[q]]].len);
+    }
+    // rows per lane: the fewest that give every wave of the join a strip
+    const int jr = max_la <= 2 * JT ? 2 : max_la <= 4 * JT ? 4 : JRMAX;
     // sizes and offsets, then each join's counts and PB on the host threads
     for (size_t q = 0; q < ts.size(); ++q) {
       const Profile &PA = prof[ja[ts[q]]], &PB = prof[jb[ts[q]]];
@@ -357,20 +471,20 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       J.la = PA.len; J.lb = PB.len;
       J.w = (int64_t)PA.rows.size() * (int64_t)PB.rows.size();
       // joins whose column scores could leave int32
-      if (J.w > WMAX || (int64_t)PA.len * (PA.len + PB.len + 1) > INT32_MAX) return TA_E_INVALID;
+      if (J.w > WMAX || (int64_t)PA.len * PB.len > INT32_MAX) return TA_E_INVALID;
+      // int32 DP when every value, and -2^29 plus a gap cost, stays clear of overflow
+      const int64_t step = std::max<int64_t>(max_abs_s, (int64_t)-p->gap_open - p->gap_extend) * J.w;
+      if ((int64_t)(J.la + J.lb + 2) * step >= (1ll << 28)) wide = true;
       J.cnt_off = cnt_total;
       cnt_total += (int64_t)PA.len * KS;
       J.pb_off = pb_total;
       pb_total += (int64_t)PB.len * KS;
-      J.m_off = m_total;
-      m_total += (int64_t)(PA.len + PB.len - 1) * PA.len;
+      J.m_off = m_total;  // the sweep's order: strips x steps x 64 lanes x jr rows (16-byte aligned)
+      m_total += ((int64_t)(PA.len + 64 * jr - 1) / (64 * jr)) * 8 * join_blocks(PB.len) * 64 * jr;
       J.tb_off = tb_total;
-      tb_total += (int64_t)(PA.len + PB.len + 1) * (PA.len + 1);
+      tb_total += (((int64_t)(PA.len + 64 * jr - 1) / (64 * jr)) * join_blocks(PB.len) * 64 * 8 * jr + 15) & ~15ll;
       J.ops_off = ops_total;
       ops_total += PA.len + PB.len;
-      J.hb_off = hb_total;
-      hb_total += 4 * (int64_t)(PB.len + 1);
-      max_la = std::max(max_la, PA.len);
     }
     hcnt.assign((size_t)cnt_total, 0);
     hpb.assign((size_t)pb_total, 0);
@@ -398,19 +512,20 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
           for (int32_t x = 0; x < K; ++x) pbj[(size_t)j * KS + x] += n * S[x * K + y];
         }
     });
-    const int jr = max_la <= 2 * JT ? 2 : max_la <= 4 * JT ? 4 : JRMAX;
-    const size_t lds = sizeof(int64_t) * 6 * (size_t)(std::min(max_la, JT * jr) + 1);
+    const size_t lds = (wide ? sizeof(int64_t) : sizeof(int32_t)) * 2 * (size_t)(max_lb + 2);
     JoinJob *d_jobs = (JoinJob *)device_alloc(sizeof(JoinJob) * jobs.size());
     int32_t *d_cnt = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hcnt.size()));
     int32_t *d_pb = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hpb.size()));
     int32_t *d_m = (int32_t *)device_alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(1, m_total));
-    int64_t *d_hb = (int64_t *)device_alloc(sizeof(int64_t) * (size_t)std::max<int64_t>(1, hb_total));
+    int32_t *d_err = (int32_t *)device_alloc(sizeof(int32_t));
     uint8_t *d_tb = (uint8_t *)device_alloc((size_t)std::max<int64_t>(1, tb_total));
     uint8_t *d_ops = (uint8_t *)device_alloc((size_t)std::max<int64_t>(1, ops_total));
     int32_t *d_nops = (int32_t *)device_alloc(sizeof(int32_t) * jobs.size());
     std::vector<uint8_t> hops((size_t)std::max<int64_t>(1, ops_total));
     std::vector<int32_t> hnops(jobs.size());
-    rc = (d_jobs && d_cnt && d_pb && d_m && d_hb && d_tb && d_ops && d_nops) ? TA_OK : TA_E_NOMEM;
+    int32_t herr = 0;
+    rc = (d_jobs && d_cnt && d_pb && d_m && d_err && d_tb && d_ops && d_nops) ? TA_OK : TA_E_NOMEM;
+    if (!rc && hipMemsetAsync(d_err, 0, sizeof(int32_t), st) != hipSuccess) rc = TA_E_DEVICE;
     if (!rc && (hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JoinJob) * jobs.size(), hipMemcpyHostToDevice, st) ||
                 (!hcnt.empty() &&
                  hipMemcpyAsync(d_cnt, hcnt.data(), sizeof(int32_t) * hcnt.size(), hipMemcpyHostToDevice, st)) ||
@@ -419,19 +534,40 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       rc = TA_E_DEVICE;
     if (!rc) {
       JoinArgs a;
-      a.jobs = d_jobs; a.cnt = d_cnt; a.pb = d_pb; a.mbuf = d_m; a.hbot = d_hb; a.tb = d_tb; a.ops = d_ops;
-      a.nops = d_nops;
+      a.jobs = d_jobs; a.cnt = d_cnt; a.pb = d_pb; a.mbuf = d_m; a.tb = d_tb; a.ops = d_ops;
+      a.nops = d_nops; a.err = d_err;
       a.K = K; a.open = p->gap_open; a.ext = p->gap_extend;  // K <= KS
+      a.spin_limit = 1 << 22;
       (void)hipGetLastError();
-      void (*kern)(JoinArgs) = jr == 2 ? msa_join_kernel<2> : jr == 4 ? msa_join_kernel<4> : msa_join_kernel<JRMAX>;
-      hipLaunchKernelGGL(kern, dim3((unsigned)jobs.size()), dim3(JT), lds, st, a);
-      if (hipGetLastError() != hipSuccess ||
+      // the match scores over the whole GPU, then the joins
+      std::vector<int2> tiles;
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const int nt = ((jobs[q].la + 63) / 64) * ((jobs[q].lb + 63) / 64);
+        for (int t = 0; t < nt; ++t) tiles.push_back(make_int2((int)q, t));
+      }
+      int2 *d_tiles = (int2 *)device_alloc(sizeof(int2) * tiles.size());
+      if (!d_tiles || hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, st))
+        rc = TA_E_DEVICE;
+      if (!rc) hipLaunchKernelGGL(msa_scores_kernel, dim3((unsigned)tiles.size()), dim3(256), 0, st, d_jobs, d_tiles, d_cnt,
+                                  d_pb, d_m, jr);
+      void (*kern)(JoinArgs) =
+          wide ? (jr == 2 ? msa_join_kernel<2, int64_t> : jr == 4 ? msa_join_kernel<4, int64_t> : msa_join_kernel<JRMAX, int64_t>)
+               : (jr == 2 ? msa_join_kernel<2, int32_t> : jr == 4 ? msa_join_kernel<4, int32_t> : msa_join_kernel<JRMAX, int32_t>);
+      if (!rc) hipLaunchKernelGGL(kern, dim3((unsigned)jobs.size()), dim3(JT), lds, st, a);
+      if (rc || hipGetLastError() != hipSuccess ||
           hipMemcpyAsync(hops.data(), d_ops, (size_t)ops_total, hipMemcpyDeviceToHost, st) ||
           hipMemcpyAsync(hnops.data(), d_nops, sizeof(int32_t) * jobs.size(), hipMemcpyDeviceToHost, st) ||
+          hipMemcpyAsync(&herr, d_err, sizeof(int32_t), hipMemcpyDeviceToHost, st) ||
           hipStreamSynchronize(st))
         rc = TA_E_DEVICE;
+      (void)hipStreamSynchronize(st);
+      device_free(d_tiles);
+      if (!rc && herr) {
+        fprintf(stderr, "tweakalign: a wait on another wave's progress gave up (msa join)\n");
+        rc = TA_E_DEVICE;
+      }
     }
-    device_free(d_jobs); device_free(d_cnt); device_free(d_pb); device_free(d_m); device_free(d_hb);
+    device_free(d_jobs); device_free(d_cnt); device_free(d_pb); device_free(d_m); device_free(d_err);
     device_free(d_tb); device_free(d_ops); device_free(d_nops);
     if (rc) return rc;
     // apply each join's ops (walk order: read backwards)
