@@ -19,10 +19,11 @@
 //     direction bytes back from (la, lb) to ops; the host applies the ops to
 //     the profiles' rows.
 //
-// A join first computes every column pair's match score (int32: nA * nB is
-// bounded so that 24 codes x 11 x nA * nB fits; a thread's rows' counts sit
-// in registers, each column's PB row is a wave-uniform scalar load), then
-// sweeps the DP in the register-strip form of the pairwise fill: a strip is
+// Each level first scores every column pair of every join over the whole GPU
+// (msa_scores_kernel, 64 x 64-cell tiles; int32: nA * nB is bounded so that
+// 24 codes x 11 x nA * nB fits), in the order the sweeps read them.  Then a
+// join's workgroup sweeps the DP in the register-strip form of the pairwise
+// fill: a strip is
 // 64 lanes x JR rows, lane l holds its JR rows' H and E in registers and
 // works on column t - l + 1 at step t; the bottom row (H, F) reaches the
 // next lane one step later through a wave_shr:1 DPP move, so a wave needs
