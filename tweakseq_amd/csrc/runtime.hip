@@ -1027,17 +1027,29 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       if (part_of(xb) == t) ub[k] = uniq(q.b, q.lb, xb);
     }
   });
+  // Global order: partition by partition; each sequence starts 16-byte
+  // aligned (walker windows).  Offsets and the flat array, per partition on
+  // the threads, from the partitions' counts and byte totals.
   std::vector<int32_t> base(T + 1, 0);
-  for (int t = 0; t < T; ++t) base[t + 1] = base[t] + (int32_t)part[t].size();
-  uq.clear();
-  uq.reserve((size_t)base[T]);
-  size_t ctotal = 0;
-  for (int t = 0; t < T; ++t)
-    for (Uniq &e : part[t]) {
-      e.off = (ctotal + 15) & ~(size_t)15;  // 16-byte aligned starts (walker windows)
-      ctotal = e.off + (size_t)e.len;
-      uq.push_back(e);
+  std::vector<size_t> bbase(T + 1, 0);
+  for (int t = 0; t < T; ++t) {
+    base[t + 1] = base[t] + (int32_t)part[t].size();
+    size_t bytes = 0;
+    for (const Uniq &e : part[t]) bytes += ((size_t)e.len + 15) & ~(size_t)15;
+    bbase[t + 1] = bbase[t] + bytes;
+  }
+  uq.resize((size_t)base[T]);
+  parallel_tasks(T, [&](int t) {
+    size_t at = bbase[t];
+    Uniq *dst = uq.data() + base[t];
+    for (const Uniq &e : part[t]) {
+      *dst = e;
+      dst->off = at;
+      at += ((size_t)e.len + 15) & ~(size_t)15;
+      ++dst;
     }
+  });
+  const size_t ctotal = bbase[T];
   if (T > 1)
     parallel_for((size_t)n, [&](size_t k) {
       ua[k] += base[part_of(mix(pairs[k].a))];
