@@ -315,6 +315,64 @@ def main():
         out["align_batch_e2e_ms"] = round(min(e2e) * 1e3, 1)
         log("ta_align_batch (host strings -> host CIGARs): %.1f ms, %.1f GCUPS"
             % (out["align_batch_e2e_ms"], out["align_batch_e2e_gcups"]))
+        # The same, as a stream of batches from one host thread: batch k + 1
+        # is built (letters, planning, uploads) while batch k runs, and
+        # batch k's CIGAR text is formatted while k + 1 runs -- two contexts,
+        # so the two batches sit on two streams and a creation's final drain
+        # does not wait for the other's kernels.  Letters in, CIGAR text out,
+        # every batch's results checked against the one-call form.
+        ctxs = [eng.ctx, ta.Engine(0)]
+        nb = 8
+        want_used = used.value
+        bufs = [ctypes.create_string_buffer(cap) for _ in range(2)]
+        ress = [(ta.engine.CResult * w3.npairs)() for _ in range(2)]
+        L = ta.lib()
+
+        def cx(k):
+            c = ctxs[k % 2]
+            return c if isinstance(c, ctypes.c_void_p) else c.ctx
+
+        def create(k):
+            h = ctypes.c_void_p()
+            rc = L.ta_batch_create(cx(k), ctypes.byref(w3.params), ps.arr, w3.npairs, ctypes.byref(h))
+            if rc:
+                raise RuntimeError("ta_batch_create failed: %d" % rc)
+            return h
+
+        pipe = []
+        for rep in range(2):
+            t0 = time.perf_counter()
+            prev = None
+            for k in range(nb):
+                h = create(k)
+                if L.ta_batch_run(cx(k), h):
+                    raise RuntimeError("ta_batch_run failed")
+                if prev is not None:
+                    kp, hp = prev
+                    u = ctypes.c_int64()
+                    if L.ta_batch_wait(cx(kp), hp) or L.ta_batch_results(cx(kp), hp, ress[kp % 2], bufs[kp % 2], cap,
+                                                                         ctypes.byref(u)):
+                        raise RuntimeError("pipelined batch failed")
+                    if u.value != want_used:
+                        raise RuntimeError("pipelined batch gave other CIGARs")
+                    L.ta_batch_destroy(hp)
+                prev = (k, h)
+            kp, hp = prev
+            u = ctypes.c_int64()
+            if L.ta_batch_wait(cx(kp), hp) or L.ta_batch_results(cx(kp), hp, ress[kp % 2], bufs[kp % 2], cap,
+                                                                 ctypes.byref(u)):
+                raise RuntimeError("pipelined batch failed")
+            L.ta_batch_destroy(hp)
+            pipe.append(time.perf_counter() - t0)
+        same = bufs[0].raw[:want_used] == buf.raw[:want_used] and bufs[1].raw[:want_used] == buf.raw[:want_used]
+        ctxs[1].close()
+        out["align_batch_pipelined"] = {
+            "gcups": round(nb * cells / min(pipe) / 1e9, 1), "ms_per_batch": round(min(pipe) / nb * 1e3, 2),
+            "batches": nb, "cigar_text_equals_one_call": same,
+            "form": "one host thread, batch k+1 created while batch k runs (two contexts / streams); "
+                    "letters in, CIGAR text out"}
+        log("pipelined batches (host strings -> host CIGARs): %.2f ms per batch, %.1f GCUPS"
+            % (out["align_batch_pipelined"]["ms_per_batch"], out["align_batch_pipelined"]["gcups"]))
         # Plain form: one fill + one traceback launch per pair, on a prefix.
         npl = min(args.plain_pairs, w3.npairs)
         bp = eng.batch(w3.pairs_struct(0, npl), w3.params)

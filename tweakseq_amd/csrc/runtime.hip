@@ -966,6 +966,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       delete B;
       return TA_E_INVALID;
     }
+  tr.mark("validate");
   auto mix = [](const char *s) -> uint64_t {
     uint64_t x = (uint64_t)(uintptr_t)s;
     x = (x ^ (x >> 29)) * 0xbf58476d1ce4e5b9ull;
@@ -1027,6 +1028,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       if (part_of(xb) == t) ub[k] = uniq(q.b, q.lb, xb);
     }
   });
+  tr.mark("dedupe hash");
   // Global order: partition by partition; each sequence starts 16-byte
   // aligned (walker windows).  Offsets and the flat array, per partition on
   // the threads, from the partitions' counts and byte totals.
@@ -1050,6 +1052,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     }
   });
   const size_t ctotal = bbase[T];
+  tr.mark("dedupe offsets");
   if (T > 1)
     parallel_for((size_t)n, [&](size_t k) {
       ua[k] += base[part_of(mix(pairs[k].a))];
