@@ -228,6 +228,35 @@ def test_workload_local_batch_prefix(eng):
     check(eng, pairs, "65,536-pair local batch, first 1,500", mode=ta.LOCAL, gap_open=-11, gap_extend=-1)
 
 
+def test_local_batch_in_eight_lane_units(eng, monkeypatch):
+    """A local batch with more couples than resident fill waves, planned with
+    8-lane units (64-row strips, 8 tasks per wave) beside 16-lane units and
+    whole-wave tasks: by default (the smaller couples) and with every couple
+    that is not whole-wave in 8-lane units.  Both plans differ from the one
+    without 8-lane units, and every pair equals the oracle."""
+    rng = random.Random(71)
+    pairs = []
+    for _ in range(5000):
+        a = rand_prot(rng, rng.randint(20, 300))
+        pairs.append((a, mutate(rng, a, psub=0.25, pindel=0.04)))
+    params = ta.engine.make_params(mode=ta.LOCAL, gap_open=-11, gap_extend=-1)
+    want = oracle_results(params, pairs)
+    tasks = {}
+    for label, limit in (("none", "0"), ("default", None), ("all", "1000000")):
+        if limit is None:
+            monkeypatch.delenv("TWEAKALIGN_EIGHT_LANE", raising=False)
+        else:
+            monkeypatch.setenv("TWEAKALIGN_EIGHT_LANE", limit)
+        b = eng.batch(pairs, params)
+        tasks[label] = b.info()["tasks"]
+        b.run()
+        b.wait()
+        got = b.results()
+        b.close()
+        compare(got, want, params, pairs, "8-lane units: " + label)
+    assert tasks["default"] != tasks["none"] and tasks["all"] != tasks["default"], tasks
+
+
 @pytest.mark.parametrize("mode", [ta.GLOBAL, ta.LOCAL])
 def test_shared_buffers_and_prefixes(eng, mode):
     """Pairs whose sequences point into the same buffers, at the same and at

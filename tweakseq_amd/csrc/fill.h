@@ -71,9 +71,11 @@ struct Lane<1> {
 };
 
 struct FillArgs {
-  const Task *tasks[2];  // [0] whole-wave tasks (GG = 64), [1] wave units of NGW G-lane tasks
-  int32_t ntasks[2];     // tasks, wave units
+  const Task *tasks[3];  // [0] whole-wave tasks (GG = 64), [1] wave units of NGW G-lane tasks,
+                         // [2] wave units of 8 8-lane tasks (local packed couples)
+  int32_t ntasks[3];     // tasks, wave units, wave units
   int32_t *ctr;          // work-queue heads ctr[0], ctr[1]
+  int32_t *ctr8;         // work-queue head of tasks[2]
   const int8_t *table;   // Kp x Kp, row/column Kp-1 = pad
   int32_t Kp;
   const uint16_t *rowcodes;
@@ -370,10 +372,10 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
                                            const int8_t *tab, int8_t *slot, WalkQ *wq, int &wtail,
                                            CoopQ *cq = nullptr) {
   static_assert(!COOP || (GG == GWAVE && TB), "cooperative tasks are whole-wave, with walks");
+  constexpr bool NIB = P == 2 && !TB;  // 4-bit profile slot
   using L = Lane<P>;
   using V = typename L::V;
   constexpr int ST = GG * R;  // rows per strip
-  constexpr bool NIB = P == 2 && !TB;  // 4-bit profile slot
   const int lane = threadIdx.x & 63;
   const int l = group_lane<GG>(lane);
   const int Kp = A.Kp;
@@ -931,7 +933,7 @@ __global__ __launch_bounds__(TB ? 128 * WALK_FW : 64) void fill_kernel(FillArgs 
   int8_t *slot = (int8_t *)lds + 1024 + threadIdx.x * stride;
   int wtail = 0;
   // Cooperative tasks first (a workgroup each), then long tasks (a wave
-  // each), then G-lane groups take the rest.
+  // each), then G-lane groups, then 8-lane groups take the rest.
   int ncoop = 0;
   if constexpr (TB) {
     if (A.walk && A.ncoop) {
@@ -941,7 +943,11 @@ __global__ __launch_bounds__(TB ? 128 * WALK_FW : 64) void fill_kernel(FillArgs 
   }
   if (A.ntasks[0] > ncoop)
     fill_phase<GWAVE, P, LOCAL, TB>(A, A.tasks[0] + ncoop, A.ntasks[0] - ncoop, A.ctr, tab, slot, wq, wtail);
-  if (A.ntasks[1]) fill_phase<G, P, LOCAL, TB>(A, A.tasks[1], A.ntasks[1], A.ctr + 1, tab, slot, wq, wtail);
+  if (A.ntasks[1])
+    fill_phase<G, P, LOCAL, TB>(A, A.tasks[1], A.ntasks[1], A.ctr + 1, tab, slot, wq, wtail);
+  if constexpr (LOCAL && P == 2) {
+    if (A.ntasks[2]) fill_phase<8, P, LOCAL, TB>(A, A.tasks[2], A.ntasks[2], A.ctr8, tab, slot, wq, wtail);
+  }
   if constexpr (TB) {
     if (A.walk && lane == 0) __hip_atomic_store(&wq->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }

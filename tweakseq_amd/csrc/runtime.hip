@@ -504,7 +504,9 @@ struct CodeJob {
 };
 
 struct Plan {
-  std::vector<Task> tasks2[2], tasks1[2];  // [0] whole-wave groups, [1] G-lane groups (NGW per wave unit)
+  // [0] whole-wave groups, [1] G-lane groups (NGW per wave unit), [2] 8-lane
+  // groups (8 per wave unit; local packed couples only, tasks1[2] stays empty)
+  std::vector<Task> tasks2[3], tasks1[3];
   std::vector<CodeJob> cjobs;
   uint64_t rows_n = 0, cols_n = 0;        // rowcodes / colcodes entries planned
   std::vector<PairTB> ptb;
@@ -687,16 +689,17 @@ struct Unit {
   int m, n;  // rows and columns (maxima over the two halves)
 };
 
-// Wave units: NGW G-lane tasks of one shape that one wave runs in lockstep.
-// Local couples are ordered by (strips, columns) and take their unit's
-// column count as leading pad columns; global couples need equal column
-// counts (H(m, n) is read at the last column), so they are grouped by
-// columns.  A unit short of NGW tasks is filled with copies of its first task
-// that write nothing.  Units go out largest first, flattened (NGW tasks per
-// unit).
-void add_units(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, bool local, bool trace,
+// Wave units: GWAVE / GG tasks of GG lanes and one shape that one wave runs
+// in lockstep.  Local couples are ordered by (strips, columns) and take their
+// unit's column count as leading pad columns; global couples need equal
+// column counts (H(m, n) is read at the last column), so they are grouped by
+// columns.  A unit short of its tasks is filled with copies of its first
+// task that write nothing.  Units go out largest first, flattened (GWAVE /
+// GG tasks per unit).
+void add_units(Plan &pl, std::vector<Task> &dst, int P, int GG, std::vector<Unit> u, bool local, bool trace,
                const std::vector<HostPair> &hp, int Kp) {
-  auto strips = [](int m) { return ceil_div(std::max(m, 1), G * R); };
+  const int NG = GWAVE / GG;
+  auto strips = [GG](int m) { return ceil_div(std::max(m, 1), GG * R); };
   {  // local: more strips first, then more columns; global: columns first
     std::vector<uint64_t> key(u.size());
     for (size_t t = 0; t < u.size(); ++t) {
@@ -705,18 +708,18 @@ void add_units(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, boo
     }
     gather_by_key(u, std::move(key));
   }
-  std::vector<Task> flat;  // unit t: flat[NGW t .. NGW t + NGW - 1]
-  flat.reserve(u.size() + NGW);
+  std::vector<Task> flat;  // unit t: flat[NG t .. NG t + NG - 1]
+  flat.reserve(u.size() + NG);
   std::vector<uint64_t> key;  // more 8-step blocks first (S * NB < 2^40)
   for (size_t k = 0; k < u.size();) {
     size_t e = k;
-    while (e < u.size() && e - k < NGW && (local || u[e].n == u[k].n)) ++e;
+    while (e < u.size() && e - k < (size_t)NG && (local || u[e].n == u[k].n)) ++e;
     int nq = 0, mq = 0;
     for (size_t i = k; i < e; ++i) { nq = std::max(nq, u[i].n); mq = std::max(mq, u[i].m); }
     const size_t first = flat.size();
     for (size_t i = k; i < e; ++i)
-      add_task(pl, flat, P, G, trace, hp, u[i].lo, u[i].hi, u[i].single, Kp, nq, strips(mq));
-    for (size_t i = e - k; i < (size_t)NGW; ++i) {
+      add_task(pl, flat, P, GG, trace, hp, u[i].lo, u[i].hi, u[i].single, Kp, nq, strips(mq));
+    for (size_t i = e - k; i < (size_t)NG; ++i) {
       Task f = flat[first];
       f.single = 2;
       flat.push_back(f);
@@ -726,11 +729,27 @@ void add_units(Plan &pl, std::vector<Task> &dst, int P, std::vector<Unit> u, boo
   }
   const std::vector<int32_t> o = order_by_key(std::move(key));
   dst.reserve(dst.size() + flat.size());
-  for (int32_t t : o) dst.insert(dst.end(), flat.begin() + (size_t)t * NGW, flat.begin() + (size_t)(t + 1) * NGW);
+  for (int32_t t : o) dst.insert(dst.end(), flat.begin() + (size_t)t * NG, flat.begin() + (size_t)(t + 1) * NG);
 }
 
 // Steps a G-lane group needs for a task of these shapes, in 8-step blocks.
 double group_cost(int mmax, int n) { return (double)ceil_div(mmax, G * R) * ceil_div(n + G, 8); }
+
+// The same for an 8-lane group (64-row strips, a 7-column skew).
+double cost8(int mmax, int n) { return (double)ceil_div(mmax, 8 * R) * ceil_div(n + 8, 8); }
+
+// Local packed couples whose 8-lane sweep takes at most this fraction of the
+// mean work of a group slot go into 8-lane units (8 tasks per wave): 64-row
+// strips and a 7-column skew compute fewer pad cells than 128-row strips and
+// a 15-column skew (65,536-pair log-normal batch: 1.118x the real cells
+// instead of 1.164x), while the 16-lane units keep the tasks whose 8-lane
+// units would be long enough to end the run late.  A list-scheduling model
+// of that batch gives the shortest run between 0.4x and 0.9x; 8-lane units
+// of every task end it ~50% late.
+double eight_lane_limit() {
+  const char *e = getenv("TWEAKALIGN_EIGHT_LANE");  // fraction; "0" turns 8-lane units off
+  return e ? atof(e) : 0.5;
+}
 
 
 // Whole-wave groups for tasks long enough to set the makespan of a group
@@ -791,7 +810,7 @@ struct ta_batch {
   bool nib = false;
   int fill_waves = 0;             // resident fill waves per CU (0: default_waves)
   bool wave_tasks = false;        // whole-wave groups for every multi-strip task
-  Task *d_tasks2[2] = {nullptr, nullptr}, *d_tasks1[2] = {nullptr, nullptr};
+  Task *d_tasks2[3] = {nullptr, nullptr, nullptr}, *d_tasks1[3] = {nullptr, nullptr, nullptr};
   uint16_t *d_rowcodes = nullptr, *d_colcodes = nullptr;
   uint32_t *d_ring = nullptr;
   uint8_t *d_tb = nullptr;
@@ -880,7 +899,7 @@ void ta_batch_destroy(ta_batch *B) {
   // holds even where the run's stream is gone.
   if (B->ev[3]) (void)hipEventSynchronize(B->ev[3]);
   dfree(B->d_codes); dfree(B->d_table); dfree(B->d_table_nib);
-  for (int g = 0; g < 2; ++g) { dfree(B->d_tasks2[g]); dfree(B->d_tasks1[g]); }
+  for (int g = 0; g < 3; ++g) { dfree(B->d_tasks2[g]); dfree(B->d_tasks1[g]); }
   dfree(B->d_rowcodes); dfree(B->d_colcodes); dfree(B->d_ring); dfree(B->d_tb);
   dfree(B->d_ptb); dfree(B->d_pairs); dfree(B->d_fo); dfree(B->d_res); dfree(B->d_cig);
   dfree(B->d_ctr); dfree(B->d_fo_init); dfree(B->d_order); dfree(B->d_order_rest);
@@ -898,7 +917,7 @@ void ta_batch_destroy(ta_batch *B) {
   vec_give(B->pairs_h);
   vec_give(B->plan.ptb);
   vec_give(B->plan.cjobs);
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < 3; ++g) {
     vec_give(B->plan.tasks2[g]);
     vec_give(B->plan.tasks1[g]);
   }
@@ -1182,7 +1201,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   Plan &pl = B->plan;
   vec_take(pl.ptb);
   vec_take(pl.cjobs);
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < 3; ++g) {
     vec_take(pl.tasks2[g]);
     vec_take(pl.tasks1[g]);
   }
@@ -1240,15 +1259,18 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   const double slots = (double)ctx->cus * wpc * NGW;
   const double mean_slot = work / slots;
   const bool few = B->wave_tasks || couples.size() + i32.size() <= (size_t)ctx->cus * wpc;
-  std::vector<Unit> u2, u1;
+  std::vector<Unit> u2, u1, u8;
   pl.tasks2[0].reserve(couples.size());
   pl.tasks1[0].reserve(i32.size());
+  const double lim8 = local && !few ? eight_lane_limit() * mean_slot : -1.0;
   for (auto &c : couples) {
     const bool single = c.second < 0;
     const int hi = single ? c.first : c.second;
     auto sh = shape(c.first, hi);
     if (pick_gg(sh.first, sh.second, mean_slot, few) == GWAVE)
       add_task(pl, pl.tasks2[0], 2, GWAVE, B->trace, B->hp, c.first, hi, single, B->Kp);
+    else if (cost8(sh.first, sh.second) <= lim8)
+      u8.push_back(Unit{c.first, hi, single, sh.first, sh.second});
     else
       u2.push_back(Unit{c.first, hi, single, sh.first, sh.second});
   }
@@ -1258,8 +1280,9 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     else
       u1.push_back(Unit{k, k, true, B->hp[k].la, B->hp[k].lb});
   }
-  add_units(pl, pl.tasks2[1], 2, u2, local, B->trace, B->hp, B->Kp);
-  add_units(pl, pl.tasks1[1], 1, u1, local, B->trace, B->hp, B->Kp);
+  add_units(pl, pl.tasks2[1], 2, G, u2, local, B->trace, B->hp, B->Kp);
+  add_units(pl, pl.tasks2[2], 2, 8, u8, local, B->trace, B->hp, B->Kp);
+  add_units(pl, pl.tasks1[1], 1, G, u1, local, B->trace, B->hp, B->Kp);
   // costlier (more 8-step blocks, S * NB < 2^40) first, stable
   auto by_cost = [](std::vector<Task> &v) {
     std::vector<uint64_t> key(v.size());
@@ -1333,7 +1356,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     table_nib = nibble_table(B->alph);
     up.add(&B->d_table_nib, table_nib);
   }
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < 3; ++g) {
     up.add(&B->d_tasks2[g], pl.tasks2[g]);
     up.add(&B->d_tasks1[g], pl.tasks1[g]);
   }
@@ -1419,8 +1442,8 @@ int ta_batch_info(ta_batch *B, int64_t *cells, int64_t *tb_bytes, int32_t *tasks
   if (tb_bytes)
     *tb_bytes = (int64_t)(B->plan.tb_bytes + B->wtb_stride * B->long_wgs + 4 * B->lck_elems + 8 * B->lcol_elems);
   if (tasks)
-    *tasks = (int32_t)(B->plan.tasks2[0].size() + B->plan.tasks2[1].size() + B->plan.tasks1[0].size() +
-                       B->plan.tasks1[1].size() + B->litems.size());
+    *tasks = (int32_t)(B->plan.tasks2[0].size() + B->plan.tasks2[1].size() + B->plan.tasks2[2].size() +
+                       B->plan.tasks1[0].size() + B->plan.tasks1[1].size() + B->litems.size());
   return TA_OK;
 }
 
@@ -1444,14 +1467,17 @@ TbArgs tb_args(ta_batch *B) {
 // `walk`: the batch's own fill (planes in d_tb); each workgroup's walker wave
 // walks the pairs its fill wave finishes.
 template <int P>
-int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntasks[2], uint16_t *rows,
+int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[3], const int ntasks[3], uint16_t *rows,
                 uint16_t *cols, uint32_t *ring, uint8_t *tb, int32_t *ctr, int max_groups, bool walk = false) {
-  if (ntasks[0] + ntasks[1] == 0) return TA_OK;
+  if (ntasks[0] + ntasks[1] + ntasks[2] == 0) return TA_OK;
   const bool nib = P == 2 && !B->trace;  // score-only packed kernels keep 4-bit scores
   FillArgs a;
-  a.tasks[0] = d_tasks[0]; a.tasks[1] = d_tasks[1];
-  a.ntasks[0] = ntasks[0]; a.ntasks[1] = ntasks[1];
+  for (int g = 0; g < 3; ++g) {
+    a.tasks[g] = d_tasks[g];
+    a.ntasks[g] = ntasks[g];
+  }
   a.ctr = ctr;
+  a.ctr8 = B->d_ctr + (P == 2 ? 14 : 15);  // 8-lane units come only from the batch plan
   a.table = nib ? B->d_table_nib : B->d_table; a.Kp = B->Kp;
   a.bias = -B->alph.minS;
   a.rowcodes = rows; a.colcodes = cols; a.ring = ring; a.tb = tb; a.out = B->d_fo;
@@ -1466,7 +1492,7 @@ int launch_fill(ta_ctx *ctx, ta_batch *B, Task *const d_tasks[2], const int ntas
   a.spin_limit = B->spin_limit * 16;
   const int lds = walk ? walk_lds_bytes(P, B->Kp) : fill_lds_bytes(P, B->Kp, nib);
   const int per_cu = fill_waves_per_cu(P, B->Kp, nib, waves_wanted(B), walk);
-  int groups = ntasks[0] + ntasks[1];  // one wave per whole-wave task or wave unit
+  int groups = ntasks[0] + ntasks[1] + ntasks[2];  // one wave per whole-wave task or wave unit
   if (max_groups > 0) groups = std::min(groups, max_groups);
   groups = std::max(1, std::min(groups, ctx->cus * per_cu));
   if (walk) {  // WALK_FW fill waves per workgroup; a cooperative task takes one
@@ -1575,7 +1601,7 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
       u.push_back(Unit{k, k, true, B->hp[k].la, B->hp[k].lb});
     }
   }
-  add_units(pl, pl.tasks1[1], 1, u, B->p.mode == TA_LOCAL, B->trace, B->hp, B->Kp);
+  add_units(pl, pl.tasks1[1], 1, G, u, B->p.mode == TA_LOCAL, B->trace, B->hp, B->Kp);
   for (Task &t : pl.tasks1[1]) t.flags_or = TA_F_RERUN;
   std::vector<Task> all(pl.tasks1[0]);
   all.insert(all.end(), pl.tasks1[1].begin(), pl.tasks1[1].end());
@@ -1597,8 +1623,8 @@ int rerun_saturated(ta_ctx *ctx, ta_batch *B) {
   }
   HIPCHK(hipMemcpyAsync(B->d_ptb, ptb.data(), sizeof(PairTB) * B->n, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(B->d_ctr + 4, 0, 2 * sizeof(int32_t), st));
-  Task *const tk[2] = {B->d_tasks_r, B->d_tasks_r + pl.tasks1[0].size()};
-  const int nt[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / NGW};
+  Task *const tk[3] = {B->d_tasks_r, B->d_tasks_r + pl.tasks1[0].size(), nullptr};
+  const int nt[3] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / NGW, 0};
   return launch_fill<1>(ctx, B, tk, nt, B->d_rowcodes_r, B->d_colcodes_r, B->d_ring_r, B->d_tb_r, B->d_ctr + 4, 0);
 }
 
@@ -1616,8 +1642,8 @@ int launch_init(ta_batch *B) {
 
 int launch_fills(ta_ctx *ctx, ta_batch *B) {
   const Plan &pl = B->plan;
-  const int n2[2] = {(int)pl.tasks2[0].size(), (int)pl.tasks2[1].size() / NGW};
-  const int n1[2] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / NGW};
+  const int n2[3] = {(int)pl.tasks2[0].size(), (int)pl.tasks2[1].size() / NGW, (int)pl.tasks2[2].size() / 8};
+  const int n1[3] = {(int)pl.tasks1[0].size(), (int)pl.tasks1[1].size() / NGW, 0};
   int rc = launch_fill<2>(ctx, B, B->d_tasks2, n2, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb, B->d_ctr, 0,
                           B->walk);
   if (!rc) rc = launch_fill<1>(ctx, B, B->d_tasks1, n1, B->d_rowcodes, B->d_colcodes, B->d_ring, B->d_tb,
@@ -2005,8 +2031,8 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
     if (h.path) {
       HIPCHK(hipMemsetAsync(B->d_ctr + 6, 0, 2 * sizeof(int32_t), st));
       const bool wave = B->plain_tasks[NGW * k].S && pick_gg(h.la, h.lb, 0.0, true) == GWAVE;
-      Task *const tk[2] = {B->d_plain_tasks + NGW * k, B->d_plain_tasks + NGW * k};
-      const int nt[2] = {wave ? 1 : 0, wave ? 0 : 1};
+      Task *const tk[3] = {B->d_plain_tasks + NGW * k, B->d_plain_tasks + NGW * k, nullptr};
+      const int nt[3] = {wave ? 1 : 0, wave ? 0 : 1, 0};
       int rc = h.path == 1 ? launch_fill<1>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1)
                            : launch_fill<2>(ctx, B, tk, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1);
       if (rc) return rc;
@@ -2018,7 +2044,7 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
         HIPCHK(hipStreamSynchronize(st));
         if (fo.flags & TA_F_RERUN) {
           HIPCHK(hipMemsetAsync(B->d_ctr + 6, 0, 2 * sizeof(int32_t), st));
-          Task *const rt[2] = {B->d_plain_rtasks + B->plain_rerun[k], B->d_plain_rtasks + B->plain_rerun[k]};
+          Task *const rt[3] = {B->d_plain_rtasks + B->plain_rerun[k], B->d_plain_rtasks + B->plain_rerun[k], nullptr};
           rc = launch_fill<1>(ctx, B, rt, nt, B->d_prow, B->d_pcol, B->d_pring, B->d_ptbuf, B->d_ctr + 6, 1);
           if (rc) return rc;
           ptb = B->d_pptb_r;
