@@ -101,14 +101,17 @@ struct FillArgs {
 // one workgroup (one CU), so a workgroup-scope release / acquire orders the
 // fill's plane and result stores before the walker's loads, and neither wave
 // ever waits on a wave outside its workgroup: the walker waits only for its
-// fill wave, the fill wave only for ring space the walker frees.
+// fill wave, the fill wave only for ring space the walker frees.  Once its
+// queues are empty the fill wave walks too, from the same ring, with the LDS
+// of its profile slots as walker slices (the walks of the last tasks then
+// have up to three times the lanes).
 constexpr int WQ_CAP = 64;  // ring entries (pair ids)
 struct WalkQ {
   int32_t ids[WQ_CAP];
-  int32_t tail;  // written by the fill wave
-  int32_t head;  // written by the walker
-  int32_t done;  // fill wave finished (tail is final)
-  int32_t pad_;
+  int32_t tail;   // entries pushed (the fill wave)
+  int32_t head;   // entries read (every consumer adds the ones it read)
+  int32_t done;   // fill wave finished (tail is final)
+  int32_t claim;  // entries claimed (consumers, by compare-and-swap)
 };
 constexpr int WALK_FW = 4;      // fill waves per workgroup when walking (one per SIMD)
 constexpr int WALK_LANES = 32;  // walker lanes per fill wave
@@ -161,28 +164,45 @@ __device__ __forceinline__ void walk_push(WalkQ *q, int &tail, const int (&sg)[N
 
 // Walker side: each lane walks one pair at a time and takes the next
 // handed-over pair as soon as its walk ends, until the fill wave is done and
-// the ring is empty.
+// the ring is empty.  Lane 0 claims as many entries as the wave has idle
+// lanes (a compare-and-swap on `claim`: the fill wave may consume the same
+// ring once it is done filling); the lanes (lane 0 among them) run the loop.
 template <int P>
 __device__ __forceinline__ void walk_loop(const FillArgs &A, WalkQ *q, const int8_t *tab, uint32_t *wlds) {
   const int wl = threadIdx.x & 63;
   uint32_t *slice = wlds + wl * WSTRIDE;
   Walker<P> w;
   bool active = false;
-  int head = 0;
   for (;;) {
     const uint64_t idle = __ballot(!active);
     if (idle) {
+      // done before tail: once done is seen, the tail read after it is final
       const int done = __hip_atomic_load(&q->done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-      const int tail = __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-      const int avail = tail - head;
-      if (avail > 0) {
+      int base = 0, take = 0;
+      if (wl == 0) {
         const int nidle = __popcll(idle);
-        const int take = avail < nidle ? avail : nidle;
+        int c = __hip_atomic_load(&q->claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (;;) {
+          const int t = __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const int k = t - c < nidle ? t - c : nidle;
+          if (k <= 0) break;
+          // on failure c is reloaded with the current claim count
+          if (__hip_atomic_compare_exchange_strong(&q->claim, &c, c + k, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP)) {
+            base = c;
+            take = k;
+            break;
+          }
+        }
+      }
+      base = __shfl(base, 0, 64);
+      take = __shfl(take, 0, 64);
+      if (take > 0) {
         // rank of this lane among the idle ones
         const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
         if (!active && rank < take) {
-          const int p = q->ids[(head + rank) & (WQ_CAP - 1)];
+          const int p = q->ids[(base + rank) & (WQ_CAP - 1)];
           PairDev pd;
           FillOut f;
           ResultDev r;
@@ -191,9 +211,8 @@ __device__ __forceinline__ void walk_loop(const FillArgs &A, WalkQ *q, const int
             active = true;
           }
         }
-        head += take;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // ids read before the slots are freed
-        if (wl == 0) __hip_atomic_store(&q->head, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (wl == 0) __hip_atomic_fetch_add(&q->head, take, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       } else if (done && idle == __ballot(true)) {
         return;  // done was set after the last push, so tail is final; no walk left
       }
@@ -919,7 +938,7 @@ __global__ __launch_bounds__(TB ? 128 * WALK_FW : 64) void fill_kernel(FillArgs 
       WalkQ *wq0 = (WalkQ *)(lds + base + WALK_FW * WALK_LANES * WSTRIDE * 4);
       wq = wq0 + f;
       cq = (CoopQ *)(wq0 + WALK_FW);
-      if (wave < WALK_FW && lane == 0) { wq->tail = 0; wq->head = 0; wq->done = 0; }
+      if (wave < WALK_FW && lane == 0) { wq->tail = 0; wq->head = 0; wq->done = 0; wq->claim = 0; }
       if (threadIdx.x == 0) { cq->task = 0; cq->epoch = 0; cq->arrived = 0; }
     }
   }
@@ -949,7 +968,12 @@ __global__ __launch_bounds__(TB ? 128 * WALK_FW : 64) void fill_kernel(FillArgs 
     if (A.ntasks[2]) fill_phase<8, P, LOCAL, TB>(A, A.tasks[2], A.ntasks[2], A.ctr8, tab, slot, wq, wtail);
   }
   if constexpr (TB) {
-    if (A.walk && lane == 0) __hip_atomic_store(&wq->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (A.walk) {
+      if (lane == 0) __hip_atomic_store(&wq->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      // then walk beside the walker wave, in the LDS of this wave's profile slots
+      const int nl = min(64, 64 * stride / (WSTRIDE * 4));
+      if (lane < nl) walk_loop<P>(A, wq, tab, (uint32_t *)(lds + 1024 + wave * 64 * stride));
+    }
   }
 }
 
