@@ -232,8 +232,9 @@ def test_local_batch_in_eight_lane_units(eng, monkeypatch):
     """A local batch with more couples than resident fill waves, planned with
     8-lane units (64-row strips, 8 tasks per wave) beside 16-lane units and
     whole-wave tasks: by default (the smaller couples) and with every couple
-    that is not whole-wave in 8-lane units.  Both plans differ from the one
-    without 8-lane units, and every pair equals the oracle."""
+    that is not whole-wave in 8-lane units.  Both plans write fewer
+    direction-plane bytes than the one without 8-lane units (fewer pad
+    cells), and every pair equals the oracle."""
     rng = random.Random(71)
     pairs = []
     for _ in range(5000):
@@ -241,20 +242,20 @@ def test_local_batch_in_eight_lane_units(eng, monkeypatch):
         pairs.append((a, mutate(rng, a, psub=0.25, pindel=0.04)))
     params = ta.engine.make_params(mode=ta.LOCAL, gap_open=-11, gap_extend=-1)
     want = oracle_results(params, pairs)
-    tasks = {}
+    planes = {}
     for label, limit in (("none", "0"), ("default", None), ("all", "1000000")):
         if limit is None:
             monkeypatch.delenv("TWEAKALIGN_EIGHT_LANE", raising=False)
         else:
             monkeypatch.setenv("TWEAKALIGN_EIGHT_LANE", limit)
         b = eng.batch(pairs, params)
-        tasks[label] = b.info()["tasks"]
+        planes[label] = b.info()["tb_bytes"]
         b.run()
         b.wait()
         got = b.results()
         b.close()
         compare(got, want, params, pairs, "8-lane units: " + label)
-    assert tasks["default"] != tasks["none"] and tasks["all"] != tasks["default"], tasks
+    assert planes["default"] < planes["none"] and planes["all"] < planes["default"], planes
 
 
 @pytest.mark.parametrize("mode", [ta.GLOBAL, ta.LOCAL])

@@ -1214,11 +1214,15 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   }
   // More strips first, then more columns, then lower pair index (the lists
   // hold ascending pair indices).
+  // Local pairs are ranked by 64-row strips (those of 8-lane units), which
+  // keeps them in order of 128-row strips too.
+  const int rank_rows = local ? 8 * R : STRIP;
+  static_assert((1000000 + 8 * R - 1) / (8 * R) < (1 << 15), "strip count exceeds its key field");
   auto sort_bigger = [&](std::vector<int> &v) {
     std::vector<uint64_t> key(v.size());
     for (size_t t = 0; t < v.size(); ++t) {
       const HostPair &h = B->hp[v[t]];
-      key[t] = (STRIPS_TOP - (uint64_t)ceil_div(h.la, STRIP)) << 20 | (LEN_TOP - (uint64_t)h.lb);
+      key[t] = ((1ull << 15) - (uint64_t)ceil_div(h.la, rank_rows)) << 20 | (LEN_TOP - (uint64_t)h.lb);
     }
     gather_by_key(v, std::move(key));
   };
