@@ -4,12 +4,11 @@
 
 namespace ta {
 
-// A DP group is 16 lanes (one DPP row; a wave64 runs 4 groups) or the whole
-// 64-lane wave (long tasks).  A group owns one task (a pair, or a couple of
-// pairs packed into int16 halves) and sweeps it strip by strip.  The fill
-// also runs 8-lane groups (two per row, interleaved); with G = 8 the
-// 65,536-pair local batch wastes fewer pad rows but balances worse (twice the
-// work per wave unit) and hides fewer of its walks: 9.0 ms against 8.0.
+// A DP group is 16 lanes (one DPP row; a wave64 runs 4 groups), 8 lanes (two
+// per row, interleaved; a wave runs 8: the smaller local couples of a large
+// batch, whose 64-row strips waste fewer pad rows) or the whole 64-lane wave
+// (long tasks).  A group owns one task (a pair, or a couple of pairs packed
+// into int16 halves) and sweeps it strip by strip.
 constexpr int G = 16;
 constexpr int GWAVE = 64;
 constexpr int NGW = GWAVE / G;  // groups per wave
