@@ -160,6 +160,27 @@ int ta_batch_info(ta_batch *b, int64_t *cells, int64_t *tb_bytes, int32_t *tasks
 /* Waits for the batch's last run, then frees it. */
 void ta_batch_destroy(ta_batch *b);
 
+/* Resident sequences: each encoded once into device memory of the
+ * context's GPU and aligned by any number of later calls without being sent
+ * again (the rows an editor realigns over and over, the sequences of a
+ * guide tree's all-pairs scores).  The reference writes every sequence to
+ * the external aligner's input file for each run
+ * (tweakseq/UI/SeqEditMainWin.cpp:1616-1663).  A handle stays valid, for the
+ * alphabet it was added with, until ta_seq_reset; the store grows as needed
+ * and is freed by ta_seq_reset and ta_destroy. */
+typedef int64_t ta_seq;
+typedef struct {
+  ta_seq a, b;
+} ta_seq_pair;
+int ta_seq_add(ta_ctx *ctx, int32_t alphabet, const char *letters, int32_t len, ta_seq *out);
+                                      /* TA_E_ALPHABET: a letter outside the alphabet */
+int ta_seq_reset(ta_ctx *ctx);        /* forgets every resident sequence, frees the store */
+/* ta_align_batch on resident sequences: same results and return codes;
+ * TA_E_INVALID for an unknown handle or one of another alphabet than
+ * p->alphabet. */
+int ta_align_resident(ta_ctx *ctx, const ta_params *p, const ta_seq_pair *pairs, int32_t n,
+                      ta_result *out, char *cigar_buf, int64_t cap, int64_t *used);
+
 /* Serving: continuous batching.  ta_submit copies the pairs and returns at
  * once; requests that arrive while others run are admitted as traceback
  * memory (budget set by ta_serve_config) frees up, and run concurrently.

@@ -457,6 +457,7 @@ void ta_destroy(ta_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->pair_stage) (void)hipHostFree(c->pair_stage);
+  if (c->seq_arena) (void)hipFree(c->seq_arena);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -805,6 +806,7 @@ struct ta_batch {
   Plan plan;
   // device
   uint8_t *d_codes = nullptr;
+  bool codes_borrowed = false;    // d_codes is the context's resident-sequence arena
   int8_t *d_table = nullptr;
   int8_t *d_table_nib = nullptr;  // packed couples: 4-bit biased scores
   bool nib = false;
@@ -898,6 +900,7 @@ void ta_batch_destroy(ta_batch *B) {
   // ev[3]) must have finished first.  The event is the batch's own, so this
   // holds even where the run's stream is gone.
   if (B->ev[3]) (void)hipEventSynchronize(B->ev[3]);
+  if (B->codes_borrowed) B->d_codes = nullptr;
   dfree(B->d_codes); dfree(B->d_table); dfree(B->d_table_nib);
   for (int g = 0; g < 3; ++g) { dfree(B->d_tasks2[g]); dfree(B->d_tasks1[g]); }
   dfree(B->d_rowcodes); dfree(B->d_colcodes); dfree(B->d_ring); dfree(B->d_tb);
@@ -937,8 +940,8 @@ int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
 }
 
 int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, hipStream_t stream,
-                    ta_batch **out, const ta_tuning *force) {
-  if (!ctx || !p || !out || n < 0 || (n > 0 && !pairs)) return TA_E_INVALID;
+                    ta_batch **out, const ta_tuning *force, const ta_seq_pair *res) {
+  if (!ctx || !p || !out || n < 0 || (n > 0 && !pairs && !res)) return TA_E_INVALID;
   *out = nullptr;
   if (params_check(p)) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
@@ -960,7 +963,16 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     tu = *force;
   } else {
     std::lock_guard<std::mutex> g(ctx->tune_mu);
-    auto it = ctx->tune.find(tune_key(p, ta_tune_bucket(pairs, n)));
+    int bucket = 0;
+    if (res) {  // the bucket from the resident sequences' lengths
+      std::vector<ta_pair> lens((size_t)n);
+      for (int32_t k = 0; k < n; ++k)
+        lens[k] = ta_pair{nullptr, ctx->seqs[res[k].a].len, nullptr, ctx->seqs[res[k].b].len};
+      bucket = ta_tune_bucket(lens.data(), n);
+    } else {
+      bucket = ta_tune_bucket(pairs, n);
+    }
+    auto it = ctx->tune.find(tune_key(p, bucket));
     if (it != ctx->tune.end()) tu = it->second;
   }
   B->fill_waves = tu.waves_per_cu;
@@ -980,11 +992,24 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   // cache-resident), mapping a pointer to its latest unique entry, so the
   // same pointer always meets the same thread, in pair order.
   struct Uniq { const char *s; int32_t len; uint64_t off; int64_t bound; };
-  for (int32_t k = 0; k < n; ++k)
-    if (!pair_ok(*p, pairs[k])) {
+  // Resident sequences: known handles of this alphabet, lengths in range.
+  auto pair_at = [&](int32_t k) -> ta_pair {
+    if (!res) return pairs[k];
+    const int32_t la = ctx->seqs[res[k].a].len, lb = ctx->seqs[res[k].b].len;
+    return ta_pair{la ? "" : nullptr, la, lb ? "" : nullptr, lb};  // letters are on the device
+  };
+  for (int32_t k = 0; k < n; ++k) {
+    if (res && (res[k].a < 0 || res[k].b < 0 || res[k].a >= (int64_t)ctx->seqs.size() ||
+                res[k].b >= (int64_t)ctx->seqs.size() || ctx->seqs[res[k].a].alphabet != p->alphabet ||
+                ctx->seqs[res[k].b].alphabet != p->alphabet)) {
       delete B;
       return TA_E_INVALID;
     }
+    if (!pair_ok(*p, pair_at(k))) {
+      delete B;
+      return TA_E_INVALID;
+    }
+  }
   tr.mark("validate");
   auto mix = [](const char *s) -> uint64_t {
     uint64_t x = (uint64_t)(uintptr_t)s;
@@ -1004,80 +1029,83 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   std::vector<int32_t> &ua = tl_ua, &ub = tl_ub;
   std::vector<std::vector<Uniq>> &part = tl_part;
   std::vector<Uniq> &uq = tl_uq;
-  ua.resize((size_t)n);
-  ub.resize((size_t)n);
-  if ((int)part.size() < T) part.resize(T);
-  for (int t = 0; t < T; ++t) part[t].clear();
-  parallel_tasks(T, [&](int t) {
-    struct Slot { const char *s; int32_t idx; };
-    // sized for this thread's expected share at most a quarter full; the
-    // thread keeps it between calls (the host pool's threads live on)
-    size_t cap = 64;
-    while (cap < 8 * (size_t)n / (size_t)T) cap <<= 1;
-    static thread_local std::vector<Slot> tl_tab;
-    std::vector<Slot> &tab = tl_tab;
-    if (tab.size() != cap) tab.assign(cap, Slot{nullptr, -1});
-    else std::fill(tab.begin(), tab.end(), Slot{nullptr, -1});
-    std::vector<Uniq> &mine = part[t];
-    auto grow = [&]() {  // keep the table at most half full
-      std::vector<Slot> old(2 * cap, Slot{nullptr, -1});
-      old.swap(tab);
-      cap *= 2;
-      for (const Slot &e : old) {
-        if (e.idx < 0) continue;
-        size_t h = (size_t)mix(e.s) & (cap - 1);
-        while (tab[h].idx >= 0) h = (h + 1) & (cap - 1);
-        tab[h] = e;
+  size_t ctotal = 0;  // bytes of the unique sequences' codes
+  if (!res) {
+    ua.resize((size_t)n);
+    ub.resize((size_t)n);
+    if ((int)part.size() < T) part.resize(T);
+    for (int t = 0; t < T; ++t) part[t].clear();
+    parallel_tasks(T, [&](int t) {
+      struct Slot { const char *s; int32_t idx; };
+      // sized for this thread's expected share at most a quarter full; the
+      // thread keeps it between calls (the host pool's threads live on)
+      size_t cap = 64;
+      while (cap < 8 * (size_t)n / (size_t)T) cap <<= 1;
+      static thread_local std::vector<Slot> tl_tab;
+      std::vector<Slot> &tab = tl_tab;
+      if (tab.size() != cap) tab.assign(cap, Slot{nullptr, -1});
+      else std::fill(tab.begin(), tab.end(), Slot{nullptr, -1});
+      std::vector<Uniq> &mine = part[t];
+      auto grow = [&]() {  // keep the table at most half full
+        std::vector<Slot> old(2 * cap, Slot{nullptr, -1});
+        old.swap(tab);
+        cap *= 2;
+        for (const Slot &e : old) {
+          if (e.idx < 0) continue;
+          size_t h = (size_t)mix(e.s) & (cap - 1);
+          while (tab[h].idx >= 0) h = (h + 1) & (cap - 1);
+          tab[h] = e;
+        }
+      };
+      auto uniq = [&](const char *s, int32_t len, uint64_t x) -> int32_t {
+        size_t h = (size_t)x & (cap - 1);
+        while (tab[h].idx >= 0 && tab[h].s != s) h = (h + 1) & (cap - 1);
+        if (tab[h].idx >= 0 && mine[tab[h].idx].len == len) return tab[h].idx;
+        const bool fresh = tab[h].idx < 0;
+        mine.push_back(Uniq{s, len, 0, -1});
+        tab[h] = Slot{s, (int32_t)mine.size() - 1};
+        if (fresh && 2 * mine.size() > cap) grow();
+        return (int32_t)mine.size() - 1;
+      };
+      for (int32_t k = 0; k < n; ++k) {
+        const ta_pair &q = pairs[k];
+        const uint64_t xa = mix(q.a), xb = mix(q.b);
+        if (part_of(xa) == t) ua[k] = uniq(q.a, q.la, xa);
+        if (part_of(xb) == t) ub[k] = uniq(q.b, q.lb, xb);
       }
-    };
-    auto uniq = [&](const char *s, int32_t len, uint64_t x) -> int32_t {
-      size_t h = (size_t)x & (cap - 1);
-      while (tab[h].idx >= 0 && tab[h].s != s) h = (h + 1) & (cap - 1);
-      if (tab[h].idx >= 0 && mine[tab[h].idx].len == len) return tab[h].idx;
-      const bool fresh = tab[h].idx < 0;
-      mine.push_back(Uniq{s, len, 0, -1});
-      tab[h] = Slot{s, (int32_t)mine.size() - 1};
-      if (fresh && 2 * mine.size() > cap) grow();
-      return (int32_t)mine.size() - 1;
-    };
-    for (int32_t k = 0; k < n; ++k) {
-      const ta_pair &q = pairs[k];
-      const uint64_t xa = mix(q.a), xb = mix(q.b);
-      if (part_of(xa) == t) ua[k] = uniq(q.a, q.la, xa);
-      if (part_of(xb) == t) ub[k] = uniq(q.b, q.lb, xb);
-    }
-  });
-  tr.mark("dedupe hash");
-  // Global order: partition by partition; each sequence starts 16-byte
-  // aligned (walker windows).  Offsets and the flat array, per partition on
-  // the threads, from the partitions' counts and byte totals.
-  std::vector<int32_t> base(T + 1, 0);
-  std::vector<size_t> bbase(T + 1, 0);
-  for (int t = 0; t < T; ++t) {
-    base[t + 1] = base[t] + (int32_t)part[t].size();
-    size_t bytes = 0;
-    for (const Uniq &e : part[t]) bytes += ((size_t)e.len + 15) & ~(size_t)15;
-    bbase[t + 1] = bbase[t] + bytes;
-  }
-  uq.resize((size_t)base[T]);
-  parallel_tasks(T, [&](int t) {
-    size_t at = bbase[t];
-    Uniq *dst = uq.data() + base[t];
-    for (const Uniq &e : part[t]) {
-      *dst = e;
-      dst->off = at;
-      at += ((size_t)e.len + 15) & ~(size_t)15;
-      ++dst;
-    }
-  });
-  const size_t ctotal = bbase[T];
-  tr.mark("dedupe offsets");
-  if (T > 1)
-    parallel_for((size_t)n, [&](size_t k) {
-      ua[k] += base[part_of(mix(pairs[k].a))];
-      ub[k] += base[part_of(mix(pairs[k].b))];
     });
-  tr.mark("dedupe");
+    tr.mark("dedupe hash");
+    // Global order: partition by partition; each sequence starts 16-byte
+    // aligned (walker windows).  Offsets and the flat array, per partition on
+    // the threads, from the partitions' counts and byte totals.
+    std::vector<int32_t> base(T + 1, 0);
+    std::vector<size_t> bbase(T + 1, 0);
+    for (int t = 0; t < T; ++t) {
+      base[t + 1] = base[t] + (int32_t)part[t].size();
+      size_t bytes = 0;
+      for (const Uniq &e : part[t]) bytes += ((size_t)e.len + 15) & ~(size_t)15;
+      bbase[t + 1] = bbase[t] + bytes;
+    }
+    uq.resize((size_t)base[T]);
+    parallel_tasks(T, [&](int t) {
+      size_t at = bbase[t];
+      Uniq *dst = uq.data() + base[t];
+      for (const Uniq &e : part[t]) {
+        *dst = e;
+        dst->off = at;
+        at += ((size_t)e.len + 15) & ~(size_t)15;
+        ++dst;
+      }
+    });
+    ctotal = bbase[T];
+    tr.mark("dedupe offsets");
+    if (T > 1)
+      parallel_for((size_t)n, [&](size_t k) {
+        ua[k] += base[part_of(mix(pairs[k].a))];
+        ub[k] += base[part_of(mix(pairs[k].b))];
+      });
+    tr.mark("dedupe");
+  }
   // The walker reads 32-byte windows: 32 bytes of slack after the last
   // sequence.  The raw letters go up as they are, each sequence followed by
   // a letter that encodes to 0 up to the next 16-byte start (each host thread
@@ -1085,10 +1113,12 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   // the GPU encodes them in place, flagging letters outside the alphabet.
   const size_t cend = ((ctotal + 15) & ~(size_t)15) + 32;
   StageHold codes;  // raw letters of the unique sequences, 16-byte aligned starts
-  codes.s = stage_take(cend);
-  if (!codes.s.p) {
-    delete B;
-    return TA_E_NOMEM;
+  if (!res) {
+    codes.s = stage_take(cend);
+    if (!codes.s.p) {
+      delete B;
+      return TA_E_NOMEM;
+    }
   }
   uint8_t map[256], pad_letter = 0;
   for (int b = 255; b >= 0; --b) {
@@ -1096,14 +1126,16 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     map[b] = c < 0 ? 0xff : (uint8_t)c;
     if (c == 0) pad_letter = (uint8_t)b;
   }
-  parallel_for(uq.size(), [&](size_t u) {
-    const Uniq &e = uq[u];
-    uint8_t *dst = codes.s.p + e.off;
-    if (e.len) memcpy(dst, e.s, (size_t)e.len);
-    const size_t next = u + 1 < uq.size() ? uq[u + 1].off : cend;
-    memset(dst + e.len, pad_letter, next - (e.off + (size_t)e.len));
-  });
-  if (uq.empty()) memset(codes.s.p, pad_letter, cend);
+  if (!res) {
+    parallel_for(uq.size(), [&](size_t u) {
+      const Uniq &e = uq[u];
+      uint8_t *dst = codes.s.p + e.off;
+      if (e.len) memcpy(dst, e.s, (size_t)e.len);
+      const size_t next = u + 1 < uq.size() ? uq[u + 1].off : cend;
+      memset(dst + e.len, pad_letter, next - (e.off + (size_t)e.len));
+    });
+    if (uq.empty()) memset(codes.s.p, pad_letter, cend);
+  }
   tr.mark("letters");
   // The sum over a sequence's residues of the best score each can reach
   // (0 if none is positive) bounds any local alignment of it; it decides
@@ -1123,22 +1155,35 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     }
     return e.bound;
   };
+  // The same bound for a resident sequence, from its residue counts.
+  auto bound_res = [&](ta_seq h) -> int64_t {
+    const int32_t *cnt = ctx->seq_counts.data() + 32 * (size_t)h;
+    int64_t sum = 0;
+    for (int r = 0; r < B->alph.K; ++r) sum += (int64_t)cnt[r] * best[r];
+    return sum;
+  };
   // The letters go up while the host plans: from the pinned block the copy
   // runs on the DMA engine alone; the encoding kernel follows on the stream.
   hipStream_t st = B->stream;
-  int rc = dalloc(&B->d_codes, cend);
+  int rc = TA_OK;
   uint8_t *d_map = nullptr;
   int32_t *d_bad = nullptr;
   int32_t bad_letter = 0;
-  if (!rc) {
+  if (res) {  // resident sequences: the codes are on the device already
+    B->d_codes = ctx->seq_arena;
+    B->codes_borrowed = true;
+  } else {
+    rc = dalloc(&B->d_codes, cend);
+  }
+  if (!rc && !res) {
     codes.stream = st;
     codes.queued = true;
     if (hipMemcpyAsync(B->d_codes, codes.s.p, cend, hipMemcpyHostToDevice, st) != hipSuccess) rc = TA_E_DEVICE;
   }
-  rc = rc ? rc : upload(&d_map, std::vector<uint8_t>(map, map + 256), st);
-  rc = rc ? rc : dalloc(&d_bad, 1);
-  if (!rc && hipMemsetAsync(d_bad, 0, sizeof(int32_t), st) != hipSuccess) rc = TA_E_DEVICE;
-  if (!rc) {
+  rc = rc || res ? rc : upload(&d_map, std::vector<uint8_t>(map, map + 256), st);
+  rc = rc || res ? rc : dalloc(&d_bad, 1);
+  if (!rc && !res && hipMemsetAsync(d_bad, 0, sizeof(int32_t), st) != hipSuccess) rc = TA_E_DEVICE;
+  if (!rc && !res) {
     (void)hipGetLastError();
     const size_t words = cend / 16;
     hipLaunchKernelGGL(encode_kernel, dim3((unsigned)std::min<size_t>((words + 255) / 256, 4096)), dim3(256), 0, st,
@@ -1160,10 +1205,10 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   B->fo_init.assign(n, FillOut{0, 0, 0, 0});
   B->pairs_h.resize(n);
   for (int32_t k = 0; k < n; ++k) {
-    const ta_pair &q = pairs[k];
+    const ta_pair q = pair_at(k);
     HostPair &h = B->hp[k];
-    h.a_off = uq[ua[k]].off;
-    h.b_off = uq[ub[k]].off;
+    h.a_off = res ? ctx->seqs[res[k].a].off : uq[ua[k]].off;
+    h.b_off = res ? ctx->seqs[res[k].b].off : uq[ub[k]].off;
     h.la = q.la; h.lb = q.lb;
     B->cells += (int64_t)q.la * q.lb;
     if (q.la == 0 || q.lb == 0) {
@@ -1183,7 +1228,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       h.path = 1;  // packed end-cell keys hold the column in 15 bits
     } else if (local) {
       h.path = 2;
-      h.maysat = std::min(bound(ua[k]), bound(ub[k])) > sat_limit;
+      h.maysat = (res ? std::min(bound_res(res[k].a), bound_res(res[k].b)) : std::min(bound(ua[k]), bound(ub[k]))) >
+                 sat_limit;
       if (h.maysat) B->may_saturate = true;
     } else {
       h.path = global_i16_safe(*p, B->alph.maxS, q.la, q.lb) ? 2 : 1;
@@ -1409,7 +1455,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = TA_E_DEVICE;
   // the letter check, read once the stream has drained (a copy into
   // pageable memory queued earlier would have waited for the upload)
-  if (!rc && hipMemcpy(&bad_letter, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) rc = TA_E_DEVICE;
+  if (!rc && d_bad && hipMemcpy(&bad_letter, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = TA_E_DEVICE;
   dfree(d_map);
   dfree(d_bad);
   dfree(d_jobs);
@@ -1935,6 +1982,74 @@ int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_
                    char *cigar_buf, int64_t cap, int64_t *used) {
   ta_batch *B = nullptr;
   int rc = ta_batch_create(ctx, p, pairs, n, &B);
+  if (rc) return rc;
+  rc = ta_batch_run(ctx, B);
+  if (!rc) rc = ta_batch_wait(ctx, B);
+  if (!rc) rc = ta_batch_results(ctx, B, out, cigar_buf, cap, used);
+  ta_batch_destroy(B);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Resident sequences: codes encoded on the host once, appended to the
+// context's device arena (grown by copying into a larger one: no batch holds
+// the arena between calls, since ta_align_resident returns only after its
+// batch is freed), aligned by handle.
+int ta_seq_add(ta_ctx *ctx, int32_t alphabet, const char *letters, int32_t len, ta_seq *out) {
+  if (!ctx || !out || len < 0 || len > 1000000 || (len > 0 && !letters)) return TA_E_INVALID;
+  if (alphabet != TA_PROTEIN && alphabet != TA_DNA) return TA_E_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  const Alphabet A = make_alphabet(alphabet, 1, -1);
+  // the codes, code 0 up to the next 16-byte start, then the walkers' slack
+  const size_t span = ((size_t)len + 15) & ~(size_t)15;
+  std::vector<uint8_t> codes(span + 32, 0);
+  std::vector<int32_t> cnt(32, 0);
+  for (int32_t i = 0; i < len; ++i) {
+    const int c = A.map[(unsigned char)letters[i]];
+    if (c < 0) return TA_E_ALPHABET;
+    codes[i] = (uint8_t)c;
+    ++cnt[c];
+  }
+  const size_t off = ctx->seq_used, need = off + span + 32;
+  if (need > ctx->seq_cap) {
+    const size_t cap = std::max(need, std::max(2 * ctx->seq_cap, (size_t)16 << 20));
+    uint8_t *d = nullptr;
+    if (hipMalloc((void **)&d, cap) != hipSuccess) {
+      (void)hipGetLastError();
+      return TA_E_NOMEM;
+    }
+    if (off && hipMemcpy(d, ctx->seq_arena, off, hipMemcpyDeviceToDevice) != hipSuccess) {
+      (void)hipFree(d);
+      return TA_E_DEVICE;
+    }
+    if (ctx->seq_arena) (void)hipFree(ctx->seq_arena);
+    ctx->seq_arena = d;
+    ctx->seq_cap = cap;
+  }
+  HIPCHK(hipMemcpy(ctx->seq_arena + off, codes.data(), span + 32, hipMemcpyHostToDevice));
+  ctx->seq_used = off + span;
+  ctx->seqs.push_back(ta_ctx::SeqRec{off, len, alphabet});
+  ctx->seq_counts.insert(ctx->seq_counts.end(), cnt.begin(), cnt.end());
+  *out = (ta_seq)(ctx->seqs.size() - 1);
+  return TA_OK;
+}
+
+int ta_seq_reset(ta_ctx *ctx) {
+  if (!ctx) return TA_E_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  if (ctx->seq_arena) HIPCHK(hipFree(ctx->seq_arena));
+  ctx->seq_arena = nullptr;
+  ctx->seq_cap = ctx->seq_used = 0;
+  ctx->seqs.clear();
+  ctx->seq_counts.clear();
+  return TA_OK;
+}
+
+int ta_align_resident(ta_ctx *ctx, const ta_params *p, const ta_seq_pair *pairs, int32_t n, ta_result *out,
+                      char *cigar_buf, int64_t cap, int64_t *used) {
+  if (!ctx || n < 0 || (n > 0 && (!pairs || !out))) return TA_E_INVALID;
+  ta_batch *B = nullptr;
+  int rc = batch_create_on(ctx, p, nullptr, n, ctx->stream, &B, nullptr, pairs);
   if (rc) return rc;
   rc = ta_batch_run(ctx, B);
   if (!rc) rc = ta_batch_wait(ctx, B);

@@ -32,6 +32,19 @@ struct ta_ctx {
   // ta_destroy.
   void *pair_stage = nullptr;
   size_t pair_stage_cap = 0;
+  // Resident sequences (ta_seq_add): residue codes in one device arena, each
+  // sequence 16-byte aligned and padded with code 0, 32 bytes of slack after
+  // the last (the walkers' windows); per sequence its offset, length,
+  // alphabet and the count of each residue code (the int16 saturation bound
+  // of a batch depends on its scores).
+  struct SeqRec {
+    uint64_t off;
+    int32_t len, alphabet;
+  };
+  uint8_t *seq_arena = nullptr;
+  size_t seq_cap = 0, seq_used = 0;
+  std::vector<SeqRec> seqs;
+  std::vector<int32_t> seq_counts;  // 32 per sequence
 };
 
 // Finds the device address of ta_align_pair_static's scratch array on
@@ -77,8 +90,9 @@ inline int tune_key(const ta_params *p, int bucket) {
 // ta_batch_create on a given stream (the serving loop gives each in-flight
 // batch its own).
 // `force`: use this tuning instead of the context's table (the autotuner).
+// `res`: the pairs are resident sequences (ta_seq_add) and `pairs` is unused.
 int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, hipStream_t stream,
-                    ta_batch **out, const ta_tuning *force = nullptr);
+                    ta_batch **out, const ta_tuning *force = nullptr, const ta_seq_pair *res = nullptr);
 // Direction-plane bytes a batch of these pairs would allocate (admission).
 int64_t batch_tb_estimate(const ta_params *p, const ta_pair *pairs, int32_t n);
 // Results of a finished batch with the CIGAR text in one string

@@ -30,6 +30,10 @@ class Pair(ctypes.Structure):
                 ("b", ctypes.c_void_p), ("lb", ctypes.c_int32)]
 
 
+class SeqPair(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_int64), ("b", ctypes.c_int64)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("lanes", ctypes.c_int32), ("wave_tasks", ctypes.c_int32)]
 
@@ -73,6 +77,10 @@ def lib():
         "ta_tune_get": ([vp, P(Params), i32, P(Tuning)], ctypes.c_int),
         "ta_batch_info": ([vp, P(i64), P(i64), P(i32)], ctypes.c_int),
         "ta_batch_destroy": ([vp], None),
+        "ta_seq_add": ([vp, i32, ctypes.c_char_p, i32, P(i64)], ctypes.c_int),
+        "ta_seq_reset": ([vp], ctypes.c_int),
+        "ta_align_resident": ([vp, P(Params), P(SeqPair), i32, P(CResult), ctypes.c_char_p, i64, P(i64)],
+                              ctypes.c_int),
         "ta_serve_config": ([vp, i64, i32], ctypes.c_int),
         "ta_submit": ([vp, P(Params), P(Pair), i32, P(ctypes.c_uint64)], ctypes.c_int),
         "ta_poll": ([vp, ctypes.c_uint64, P(CResult), ctypes.c_char_p, i64, P(i64)], ctypes.c_int),
@@ -218,6 +226,7 @@ class Engine:
         h = ctypes.c_void_p()
         _check(lib().ta_create(ctypes.byref(h), device), "ta_create")
         self.ctx = h
+        self._seq_len = {}  # resident handle -> length
 
     def batch(self, pairs, params):
         return Batch(self, pairs, params)
@@ -230,6 +239,32 @@ class Engine:
             return b.results()
         finally:
             b.close()
+
+    def seq_add(self, letters, alphabet=PROTEIN):
+        """A resident sequence (ta_seq_add): encoded once on the device; its handle."""
+        s = letters.encode() if isinstance(letters, str) else bytes(letters)
+        h = ctypes.c_int64()
+        _check(lib().ta_seq_add(self.ctx, alphabet, s, len(s), ctypes.byref(h)), "ta_seq_add")
+        self._seq_len[h.value] = len(s)
+        return h.value
+
+    def seq_reset(self):
+        _check(lib().ta_seq_reset(self.ctx), "ta_seq_reset")
+        self._seq_len = {}
+
+    def align_resident(self, handle_pairs, params):
+        """ta_align_resident on (handle a, handle b) pairs; a list of Results."""
+        n = len(handle_pairs)
+        arr = (SeqPair * max(n, 1))(*[SeqPair(a, b) for a, b in handle_pairs])
+        res = (CResult * max(n, 1))()
+        # a run of length k takes at most k + 1 <= 2k characters
+        cap = (sum(2 * (self._seq_len.get(a, 0) + self._seq_len.get(b, 0)) for a, b in handle_pairs) + 16
+               if params.traceback else 0)
+        buf = ctypes.create_string_buffer(cap) if cap else None
+        used = ctypes.c_int64()
+        _check(lib().ta_align_resident(self.ctx, ctypes.byref(params), arr, n, res, buf, cap, ctypes.byref(used)),
+               "ta_align_resident")
+        return _collect(res, buf, n, params.traceback)
 
     def align_pair(self, a, b, params):
         """One pair through ta_align_pair (the low-latency path); a Result."""
