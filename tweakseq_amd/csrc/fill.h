@@ -391,10 +391,10 @@ __device__ __forceinline__ void fill_phase(const FillArgs &A, const Task *tasks,
                                            const int8_t *tab, int8_t *slot, WalkQ *wq, int &wtail,
                                            CoopQ *cq = nullptr) {
   static_assert(!COOP || (GG == GWAVE && TB), "cooperative tasks are whole-wave, with walks");
-  constexpr bool NIB = P == 2 && !TB;  // 4-bit profile slot
   using L = Lane<P>;
   using V = typename L::V;
   constexpr int ST = GG * R;  // rows per strip
+  constexpr bool NIB = P == 2 && !TB;  // 4-bit profile slot
   const int lane = threadIdx.x & 63;
   const int l = group_lane<GG>(lane);
   const int Kp = A.Kp;

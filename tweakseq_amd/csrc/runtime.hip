@@ -944,6 +944,13 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   if (!ctx || !p || !out || n < 0 || (n > 0 && !pairs && !res)) return TA_E_INVALID;
   *out = nullptr;
   if (params_check(p)) return TA_E_INVALID;
+  if (res) {  // resident sequences: known handles of this alphabet
+    const int64_t ns = (int64_t)ctx->seqs.size();
+    for (int32_t k = 0; k < n; ++k)
+      if (res[k].a < 0 || res[k].b < 0 || res[k].a >= ns || res[k].b >= ns ||
+          ctx->seqs[res[k].a].alphabet != p->alphabet || ctx->seqs[res[k].b].alphabet != p->alphabet)
+        return TA_E_INVALID;
+  }
   HIPCHK(hipSetDevice(ctx->device));
   PhaseTrace tr("create");
 
@@ -992,24 +999,18 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   // cache-resident), mapping a pointer to its latest unique entry, so the
   // same pointer always meets the same thread, in pair order.
   struct Uniq { const char *s; int32_t len; uint64_t off; int64_t bound; };
-  // Resident sequences: known handles of this alphabet, lengths in range.
+  // A pair as the checks and the planner see it (resident sequences: the
+  // lengths; their letters are on the device).
   auto pair_at = [&](int32_t k) -> ta_pair {
     if (!res) return pairs[k];
     const int32_t la = ctx->seqs[res[k].a].len, lb = ctx->seqs[res[k].b].len;
-    return ta_pair{la ? "" : nullptr, la, lb ? "" : nullptr, lb};  // letters are on the device
+    return ta_pair{la ? "" : nullptr, la, lb ? "" : nullptr, lb};
   };
-  for (int32_t k = 0; k < n; ++k) {
-    if (res && (res[k].a < 0 || res[k].b < 0 || res[k].a >= (int64_t)ctx->seqs.size() ||
-                res[k].b >= (int64_t)ctx->seqs.size() || ctx->seqs[res[k].a].alphabet != p->alphabet ||
-                ctx->seqs[res[k].b].alphabet != p->alphabet)) {
-      delete B;
-      return TA_E_INVALID;
-    }
+  for (int32_t k = 0; k < n; ++k)
     if (!pair_ok(*p, pair_at(k))) {
       delete B;
       return TA_E_INVALID;
     }
-  }
   tr.mark("validate");
   auto mix = [](const char *s) -> uint64_t {
     uint64_t x = (uint64_t)(uintptr_t)s;
@@ -1319,8 +1320,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     auto sh = shape(c.first, hi);
     if (pick_gg(sh.first, sh.second, mean_slot, few) == GWAVE)
       add_task(pl, pl.tasks2[0], 2, GWAVE, B->trace, B->hp, c.first, hi, single, B->Kp);
-    else if (cost8(sh.first, sh.second) <= lim8)
-      u8.push_back(Unit{c.first, hi, single, sh.first, sh.second});
+    else if (cost8(sh.first, sh.second) <= lim8 && ceil_div(sh.first, 8 * R) <= (int)STRIPS_TOP)
+      u8.push_back(Unit{c.first, hi, single, sh.first, sh.second});  // (strip counts fit the unit sort key)
     else
       u2.push_back(Unit{c.first, hi, single, sh.first, sh.second});
   }
