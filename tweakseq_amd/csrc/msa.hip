@@ -360,6 +360,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
     if (lens[s] <= 0 || !seqs[s]) return TA_E_INVALID;
     off[s + 1] = off[s] + lens[s];
   }
+  PhaseTrace tr("msa");  // TWEAKALIGN_TRACE=1: host phase times on stderr
   int8_t S[32 * 32];
   int32_t K = 0;
   int rc = ta_matrix(p->alphabet, p->match, p->mismatch, S, &K);
@@ -371,6 +372,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
   for (int32_t s = 0; s < nseq; ++s)
     if ((rc = ta_encode(p->alphabet, seqs[s], lens[s], codes.data() + off[s]))) return rc;
   if (hipSetDevice(ctx->device) != hipSuccess) return TA_E_DEVICE;
+  tr.mark("encode");
 
   // 1. pair scores on the GPU
   const int32_t nc = 2 * nseq - 1;
@@ -388,11 +390,14 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
     ta_batch *B = nullptr;
     std::vector<ta_result> res(pairs.size());
     rc = batch_create_on(ctx, &q, pairs.data(), (int32_t)pairs.size(), ctx->stream, &B);
+    tr.mark("score create");
     if (!rc) rc = ta_batch_run(ctx, B);
     if (!rc) rc = ta_batch_wait(ctx, B);
+    tr.mark("score run");
     if (!rc) rc = ta_batch_results(ctx, B, res.data(), nullptr, 0, nullptr);
     if (B) ta_batch_destroy(B);
     if (rc) return rc;
+    tr.mark("score results");
     size_t k = 0;
     for (int32_t a = 0; a < nseq; ++a)
       for (int32_t b = a + 1; b < nseq; ++b, ++k) {
@@ -401,6 +406,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       }
   }
 
+  tr.mark("score sums");
   // 2. UPGMA joins (exact), and each join's tree level.  Row a keeps its
   // best partner b > a (largest mean, ties to the smaller b); the join is the
   // best row's (ties to the smaller a) -- the lexicographically first pair of
@@ -442,6 +448,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
     }
   }
 
+  tr.mark("upgma");
   // 3. joins level by level
   std::vector<Profile> prof(nc);
   for (int32_t s = 0; s < nseq; ++s) {
@@ -450,12 +457,15 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
     prof[s].cell.assign(codes.begin() + off[s], codes.begin() + off[s + 1]);
   }
   hipStream_t st = ctx->stream;
+  // The levels' count tables, kept by the calling thread between levels and
+  // calls: fresh multi-MB vectors pay a page fault per 4 KB.
+  static thread_local std::vector<int32_t> tl_cnt, tl_pb;
+  std::vector<int32_t> &hcnt = tl_cnt, &hpb = tl_pb;
   for (int32_t L = 1; L <= top; ++L) {
     std::vector<int32_t> ts;
     for (int32_t t = 0; t + 1 < nseq; ++t)
       if (level[nseq + t] == L) ts.push_back(t);
     std::vector<JoinJob> jobs(ts.size());
-    std::vector<int32_t> hcnt, hpb;
     int64_t tb_total = 0, ops_total = 0, m_total = 0, cnt_total = 0, pb_total = 0;
     int max_la = 0, max_lb = 0;
     bool wide = false;  // some join's DP values need int64
@@ -487,12 +497,15 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       J.ops_off = ops_total;
       ops_total += PA.len + PB.len;
     }
-    hcnt.assign((size_t)cnt_total, 0);
-    hpb.assign((size_t)pb_total, 0);
+    // each join's tables are cleared by the thread that fills them
+    if (hcnt.size() < (size_t)cnt_total) hcnt.resize((size_t)cnt_total);
+    if (hpb.size() < (size_t)pb_total) hpb.resize((size_t)pb_total);
     parallel_for(ts.size(), [&](size_t q) {
       const Profile &PA = prof[ja[ts[q]]], &PB = prof[jb[ts[q]]];
       const JoinJob &J = jobs[q];
       int32_t *ca = hcnt.data() + J.cnt_off;
+      std::fill(ca, ca + (size_t)PA.len * KS, 0);
+      std::fill(hpb.data() + J.pb_off, hpb.data() + J.pb_off + (size_t)PB.len * KS, 0);
       for (size_t r = 0; r < PA.rows.size(); ++r)
         for (int32_t i = 0; i < PA.len; ++i) {
           const int c = PA.cell[r * PA.len + i];
@@ -513,10 +526,11 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
           for (int32_t x = 0; x < K; ++x) pbj[(size_t)j * KS + x] += n * S[x * K + y];
         }
     });
+    tr.mark("level prep");
     const size_t lds = (wide ? sizeof(int64_t) : sizeof(int32_t)) * 2 * (size_t)(max_lb + 2);
     JoinJob *d_jobs = (JoinJob *)device_alloc(sizeof(JoinJob) * jobs.size());
-    int32_t *d_cnt = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hcnt.size()));
-    int32_t *d_pb = (int32_t *)device_alloc(sizeof(int32_t) * std::max<size_t>(1, hpb.size()));
+    int32_t *d_cnt = (int32_t *)device_alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(1, cnt_total));
+    int32_t *d_pb = (int32_t *)device_alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(1, pb_total));
     int32_t *d_m = (int32_t *)device_alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(1, m_total));
     int32_t *d_err = (int32_t *)device_alloc(sizeof(int32_t));
     uint8_t *d_tb = (uint8_t *)device_alloc((size_t)std::max<int64_t>(1, tb_total));
@@ -528,10 +542,10 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
     rc = (d_jobs && d_cnt && d_pb && d_m && d_err && d_tb && d_ops && d_nops) ? TA_OK : TA_E_NOMEM;
     if (!rc && hipMemsetAsync(d_err, 0, sizeof(int32_t), st) != hipSuccess) rc = TA_E_DEVICE;
     if (!rc && (hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JoinJob) * jobs.size(), hipMemcpyHostToDevice, st) ||
-                (!hcnt.empty() &&
-                 hipMemcpyAsync(d_cnt, hcnt.data(), sizeof(int32_t) * hcnt.size(), hipMemcpyHostToDevice, st)) ||
-                (!hpb.empty() &&
-                 hipMemcpyAsync(d_pb, hpb.data(), sizeof(int32_t) * hpb.size(), hipMemcpyHostToDevice, st))))
+                (cnt_total > 0 &&
+                 hipMemcpyAsync(d_cnt, hcnt.data(), sizeof(int32_t) * cnt_total, hipMemcpyHostToDevice, st)) ||
+                (pb_total > 0 &&
+                 hipMemcpyAsync(d_pb, hpb.data(), sizeof(int32_t) * pb_total, hipMemcpyHostToDevice, st))))
       rc = TA_E_DEVICE;
     if (!rc) {
       JoinArgs a;
@@ -571,6 +585,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
     device_free(d_jobs); device_free(d_cnt); device_free(d_pb); device_free(d_m); device_free(d_err);
     device_free(d_tb); device_free(d_ops); device_free(d_nops);
     if (rc) return rc;
+    tr.mark("level gpu");
     // apply each join's ops (walk order: read backwards)
     for (size_t q = 0; q < ts.size(); ++q) {
       const int32_t t = ts[q], c = nseq + t;
@@ -599,6 +614,7 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       PA = Profile();
       PB = Profile();
     }
+    tr.mark("level apply");
   }
 
   // 4. every residue's column

@@ -709,28 +709,35 @@ void add_units(Plan &pl, std::vector<Task> &dst, int P, int GG, std::vector<Unit
     }
     gather_by_key(u, std::move(key));
   }
-  std::vector<Task> flat;  // unit t: flat[NG t .. NG t + NG - 1]
-  flat.reserve(u.size() + NG);
-  std::vector<uint64_t> key;  // more 8-step blocks first (S * NB < 2^40)
+  // The units' extents and shapes first, then their tasks straight into dst
+  // in order of cost (more 8-step blocks first, S * NB < 2^40).
+  struct Span { uint32_t k, e; int nq, mq; };
+  std::vector<Span> spans;
+  spans.reserve(u.size() / NG + 1);
+  std::vector<uint64_t> key;
+  key.reserve(u.size() / NG + 1);
   for (size_t k = 0; k < u.size();) {
     size_t e = k;
     while (e < u.size() && e - k < (size_t)NG && (local || u[e].n == u[k].n)) ++e;
     int nq = 0, mq = 0;
     for (size_t i = k; i < e; ++i) { nq = std::max(nq, u[i].n); mq = std::max(mq, u[i].m); }
-    const size_t first = flat.size();
-    for (size_t i = k; i < e; ++i)
-      add_task(pl, flat, P, GG, trace, hp, u[i].lo, u[i].hi, u[i].single, Kp, nq, strips(mq));
-    for (size_t i = e - k; i < (size_t)NG; ++i) {
-      Task f = flat[first];
-      f.single = 2;
-      flat.push_back(f);
-    }
-    key.push_back((1ull << 40) - (uint64_t)flat[first].S * flat[first].NB);
+    spans.push_back(Span{(uint32_t)k, (uint32_t)e, nq, mq});
+    key.push_back((1ull << 40) - (uint64_t)strips(mq) * (uint64_t)ceil_div(nq + GG, 8));
     k = e;
   }
   const std::vector<int32_t> o = order_by_key(std::move(key));
-  dst.reserve(dst.size() + flat.size());
-  for (int32_t t : o) dst.insert(dst.end(), flat.begin() + (size_t)t * NG, flat.begin() + (size_t)(t + 1) * NG);
+  dst.reserve(dst.size() + spans.size() * NG);
+  for (int32_t t : o) {
+    const Span &sp = spans[t];
+    const size_t first = dst.size();
+    for (uint32_t i = sp.k; i < sp.e; ++i)
+      add_task(pl, dst, P, GG, trace, hp, u[i].lo, u[i].hi, u[i].single, Kp, sp.nq, strips(sp.mq));
+    for (uint32_t i = sp.e - sp.k; i < (uint32_t)NG; ++i) {  // fillers: copies that write nothing
+      Task f = dst[first];
+      f.single = 2;
+      dst.push_back(f);
+    }
+  }
 }
 
 // Steps a G-lane group needs for a task of these shapes, in 8-step blocks.
@@ -1279,12 +1286,19 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     for (size_t t = 0; t < i16.size(); t += 2)
       couples.push_back({i16[t], t + 1 < i16.size() ? i16[t + 1] : -1});
   } else {
-    std::map<int, std::vector<int>> byn;
-    for (int k : i16) byn[B->hp[k].lb].push_back(k);
-    for (auto &kv : byn) {
-      auto &v = kv.second;
-      sort_bigger(v);
-      for (size_t t = 0; t < v.size(); t += 2) couples.push_back({v[t], t + 1 < v.size() ? v[t + 1] : -1});
+    // Groups of equal columns in ascending order, each ranked as above; the
+    // members of a group pair off in turn, the odd one out alone.  One sort
+    // over all pairs (the key leads with the column count).
+    std::vector<uint64_t> key(i16.size());
+    for (size_t t = 0; t < i16.size(); ++t) {
+      const HostPair &h = B->hp[i16[t]];
+      key[t] = (uint64_t)h.lb << 35 | ((1ull << 15) - (uint64_t)ceil_div(h.la, rank_rows)) << 20;
+    }
+    gather_by_key(i16, std::move(key));
+    for (size_t t = 0; t < i16.size();) {
+      const bool two = t + 1 < i16.size() && B->hp[i16[t + 1]].lb == B->hp[i16[t]].lb;
+      couples.push_back({i16[t], two ? i16[t + 1] : -1});
+      t += two ? 2 : 1;
     }
   }
   tr.mark("couples");
@@ -1331,9 +1345,11 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     else
       u1.push_back(Unit{k, k, true, B->hp[k].la, B->hp[k].lb});
   }
+  tr.mark("task kinds");
   add_units(pl, pl.tasks2[1], 2, G, u2, local, B->trace, B->hp, B->Kp);
   add_units(pl, pl.tasks2[2], 2, 8, u8, local, B->trace, B->hp, B->Kp);
   add_units(pl, pl.tasks1[1], 1, G, u1, local, B->trace, B->hp, B->Kp);
+  tr.mark("wave units");
   // costlier (more 8-step blocks, S * NB < 2^40) first, stable
   auto by_cost = [](std::vector<Task> &v) {
     std::vector<uint64_t> key(v.size());
