@@ -610,36 +610,57 @@ int expand_codes(const std::vector<CodeJob> &jobs, uint64_t rows_n, uint64_t col
   return rc;
 }
 
-void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const std::vector<HostPair> &hp,
-              int lo, int hi, bool single, int Kp, int n_force = 0, int S_force = 0) {
+// Where a task's row codes, column codes, ring and direction planes start in
+// the plan's arrays, or (task_span) how many entries / bytes it takes there.
+struct TaskAt {
+  uint64_t row, col, ring, tb;
+};
+// Row codes: S * ST entries; column codes: 8 NB + 8 (a multiple of 8, so
+// every task's columns start 16-byte aligned); ring: two arrays of
+// 8 NB + 2 GG + 16 when there is more than one strip.
+TaskAt task_span(int P, int GG, bool trace, int n, int S) {
+  const int NB = ceil_div(n + GG, 8);
+  return TaskAt{(uint64_t)S * GG * R, (uint64_t)(8 * NB + 8), S > 1 ? 2ull * (8 * NB + 2 * GG + 16) : 0,
+                trace ? (uint64_t)S * tb_strip_bytes(P, NB, GG) : 0};
+}
+
+// One task (pair lo / hi; hi == lo for a single) of n columns and S strips
+// at the given offsets: the task, its code-expansion job (expanded on the
+// GPU) and the pairs' direction-plane records.
+Task make_task(const TaskAt &at, CodeJob &job, std::vector<PairTB> &ptb, int P, int GG, bool trace,
+               const std::vector<HostPair> &hp, int lo, int hi, bool single, int n, int S) {
   const HostPair &A = hp[lo], &B = hp[hi];
-  const int ST = GG * R;
   Task T{};
   T.pair[0] = lo; T.pair[1] = hi;
   T.m[0] = A.la; T.m[1] = B.la;
-  T.n = n_force ? n_force : std::max(A.lb, B.lb);
+  T.n = n;
   T.off[0] = T.n - A.lb; T.off[1] = T.n - B.lb;
-  T.S = S_force ? S_force : ceil_div(std::max(A.la, B.la), ST);
+  T.S = S;
   T.NB = ceil_div(T.n + GG, 8);
   T.single = single ? 1 : 0;
-  // Row codes: S * ST entries; column codes: 8 NB + 8 (a multiple of 8, so
-  // every task's columns start 16-byte aligned).  Expanded on the GPU.
-  T.row_off = pl.rows_n;
-  T.col_off = pl.cols_n;
-  const int ncol = 8 * T.NB + 8;
-  pl.cjobs.push_back(CodeJob{T.row_off, T.col_off, T.S * ST, ncol, lo, hi, T.off[0], T.off[1]});
-  pl.rows_n += (uint64_t)T.S * ST;
-  pl.cols_n += (uint64_t)ncol;
+  T.row_off = at.row;
+  T.col_off = at.col;
+  job = CodeJob{T.row_off, T.col_off, T.S * GG * R, 8 * T.NB + 8, lo, hi, T.off[0], T.off[1]};
   T.ring_len = T.S > 1 ? 8 * T.NB + 2 * GG + 16 : 0;
-  T.ring_off = pl.ring_elems;
-  pl.ring_elems += 2ull * T.ring_len;
-  T.tb_off = trace ? pl.tb_bytes : 0;
-  if (trace) pl.tb_bytes += (uint64_t)T.S * tb_strip_bytes(P, T.NB, GG);
+  T.ring_off = at.ring;
+  T.tb_off = trace ? at.tb : 0;
   for (int h = 0; h < (single ? 1 : 2); ++h) {
-    PairTB &d = pl.ptb[h ? hi : lo];
+    PairTB &d = ptb[h ? hi : lo];
     d.tb_off = T.tb_off; d.P = P; d.half = h; d.NB = T.NB; d.off = T.off[h]; d.GG = GG;
   }
-  dst.push_back(T);
+  return T;
+}
+
+// make_task at the plan's running offsets, appended to dst.
+void add_task(Plan &pl, std::vector<Task> &dst, int P, int GG, bool trace, const std::vector<HostPair> &hp,
+              int lo, int hi, bool single, int Kp, int n_force = 0, int S_force = 0) {
+  const HostPair &A = hp[lo], &B = hp[hi];
+  const int n = n_force ? n_force : std::max(A.lb, B.lb);
+  const int S = S_force ? S_force : ceil_div(std::max(A.la, B.la), GG * R);
+  const TaskAt at{pl.rows_n, pl.cols_n, pl.ring_elems, pl.tb_bytes}, sz = task_span(P, GG, trace, n, S);
+  pl.cjobs.emplace_back();
+  dst.push_back(make_task(at, pl.cjobs.back(), pl.ptb, P, GG, trace, hp, lo, hi, single, n, S));
+  pl.rows_n += sz.row; pl.cols_n += sz.col; pl.ring_elems += sz.ring; pl.tb_bytes += sz.tb;
 }
 
 // Positions 0..n-1 in ascending order of key, equal keys in position order:
@@ -726,18 +747,42 @@ void add_units(Plan &pl, std::vector<Task> &dst, int P, int GG, std::vector<Unit
     k = e;
   }
   const std::vector<int32_t> o = order_by_key(std::move(key));
-  dst.reserve(dst.size() + spans.size() * NG);
-  for (int32_t t : o) {
-    const Span &sp = spans[t];
-    const size_t first = dst.size();
-    for (uint32_t i = sp.k; i < sp.e; ++i)
-      add_task(pl, dst, P, GG, trace, hp, u[i].lo, u[i].hi, u[i].single, Kp, sp.nq, strips(sp.mq));
-    for (uint32_t i = sp.e - sp.k; i < (uint32_t)NG; ++i) {  // fillers: copies that write nothing
-      Task f = dst[first];
-      f.single = 2;
-      dst.push_back(f);
-    }
+  // Offsets in unit order (every task of a unit has the unit's shape), then
+  // the tasks, jobs and plane records on the host threads: each unit's
+  // entries, and its pairs' records, are its own.
+  struct UnitAt { TaskAt at, sz; size_t dst, job; };
+  std::vector<UnitAt> ua(o.size());
+  size_t nd = dst.size(), nj = pl.cjobs.size();
+  for (size_t x = 0; x < o.size(); ++x) {
+    const Span &sp = spans[o[x]];
+    const uint64_t cnt = sp.e - sp.k;
+    UnitAt &a = ua[x];
+    a.at = TaskAt{pl.rows_n, pl.cols_n, pl.ring_elems, pl.tb_bytes};
+    a.sz = task_span(P, GG, trace, sp.nq, strips(sp.mq));
+    a.dst = nd;
+    a.job = nj;
+    pl.rows_n += cnt * a.sz.row; pl.cols_n += cnt * a.sz.col;
+    pl.ring_elems += cnt * a.sz.ring; pl.tb_bytes += cnt * a.sz.tb;
+    nd += NG;
+    nj += cnt;
   }
+  dst.resize(nd);
+  pl.cjobs.resize(nj);
+  parallel_for(o.size(), [&](size_t x) {
+    const Span &sp = spans[o[x]];
+    const UnitAt &a = ua[x];
+    for (uint32_t i = sp.k; i < sp.e; ++i) {
+      const uint64_t q = i - sp.k;
+      const TaskAt at{a.at.row + q * a.sz.row, a.at.col + q * a.sz.col, a.at.ring + q * a.sz.ring,
+                      a.at.tb + q * a.sz.tb};
+      dst[a.dst + q] = make_task(at, pl.cjobs[a.job + q], pl.ptb, P, GG, trace, hp, u[i].lo, u[i].hi, u[i].single,
+                                 sp.nq, strips(sp.mq));
+    }
+    for (uint32_t q = sp.e - sp.k; q < (uint32_t)NG; ++q) {  // fillers: copies that write nothing
+      dst[a.dst + q] = dst[a.dst];
+      dst[a.dst + q].single = 2;
+    }
+  });
 }
 
 // Steps a G-lane group needs for a task of these shapes, in 8-step blocks.
