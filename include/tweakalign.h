@@ -166,15 +166,16 @@ void ta_batch_destroy(ta_batch *b);
  * guide tree's all-pairs scores).  The reference writes every sequence to
  * the external aligner's input file for each run
  * (tweakseq/UI/SeqEditMainWin.cpp:1616-1663).  A handle stays valid, for the
- * alphabet it was added with, until ta_seq_reset; the store grows as needed
- * and is freed by ta_seq_reset and ta_destroy. */
+ * alphabet it was added with, until ta_seq_reset; the store grows as needed,
+ * and ta_seq_reset and ta_destroy free it once no batch or served request
+ * still reads it. */
 typedef int64_t ta_seq;
 typedef struct {
   ta_seq a, b;
 } ta_seq_pair;
 int ta_seq_add(ta_ctx *ctx, int32_t alphabet, const char *letters, int32_t len, ta_seq *out);
                                       /* TA_E_ALPHABET: a letter outside the alphabet */
-int ta_seq_reset(ta_ctx *ctx);        /* forgets every resident sequence, frees the store */
+int ta_seq_reset(ta_ctx *ctx);        /* forgets every resident sequence */
 /* ta_align_batch on resident sequences: same results and return codes;
  * TA_E_INVALID for an unknown handle or one of another alphabet than
  * p->alphabet. */
@@ -197,6 +198,12 @@ int ta_poll(ta_ctx *ctx, uint64_t ticket, ta_result *out, char *cigar_buf, int64
                                                         written, ticket retired) but a walk of the
                                                         request gave up (TA_F_WALK_FAILED) */
 int ta_cancel(ta_ctx *ctx, uint64_t ticket);
+/* ta_submit on resident sequences (ta_seq_add): the handles are resolved
+ * when the call returns, so the store may grow or be reset while the request
+ * waits; results through ta_poll.  Such requests run in batches of their own
+ * (no letters to copy); no counterpart in the reference. */
+int ta_submit_resident(ta_ctx *ctx, const ta_params *p, const ta_seq_pair *pairs, int32_t n,
+                       uint64_t *ticket);
 
 /* Autotuning (no counterpart in the reference).  A tuning is the launch
  * shape the batch runtime uses for one kind of batch: mode, traceback,

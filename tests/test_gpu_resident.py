@@ -121,3 +121,44 @@ def test_resident_errors(eng):
     same(got, oracle(prot, [("", "ACDEFGHIK"), ("ACDEFGHIK", "")]))
     got = eng.align_resident([(d, d)], dna)
     same(got, oracle(dna, [("ACGTACGT", "ACGTACGT")]))
+
+
+def test_served_resident_requests_equal_oracle(eng):
+    """Resident requests through ta_submit_resident / ta_poll, interleaved
+    with letter requests of both modes; the store grows, and is then reset,
+    while requests wait (their handles were resolved at submission, and they
+    hold the arena they read)."""
+    import time
+    rng = random.Random(23)
+    eng.seq_reset()
+    seqs = []
+    for _ in range(30):
+        base = "".join(rng.choice(AA) for _ in range(rng.randint(30, 400)))
+        seqs += [base, mutate(rng, base)]
+    hs = [eng.seq_add(s) for s in seqs]
+    local = ta.engine.make_params(mode=ta.LOCAL, gap_open=-11, gap_extend=-1)
+    glob = ta.engine.make_params(mode=ta.GLOBAL, gap_open=-10, gap_extend=-1)
+    tickets = []
+    for q in range(30):
+        params = local if q % 2 else glob
+        idx = [(rng.randrange(60), rng.randrange(60)) for _ in range(rng.randint(1, 12))]
+        if q % 3 == 0:
+            t, n = eng.submit([(seqs[i], seqs[j]) for i, j in idx], params)
+        else:
+            t, n = eng.submit_resident([(hs[i], hs[j]) for i, j in idx], params)
+        tickets.append((t, n, params, idx))
+        if q == 10:  # past the first arena while requests wait
+            big = "A" * 1000000
+            for _ in range(18):
+                eng.seq_add(big)
+    eng.seq_reset()
+    for t, n, params, idx in tickets:
+        deadline = time.time() + 60
+        got = eng.poll(t, n, params.traceback)
+        while got is None:
+            assert time.time() < deadline, "request %d still pending" % t
+            time.sleep(0.001)
+            got = eng.poll(t, n, params.traceback)
+        same(got, oracle(params, [(seqs[i], seqs[j]) for i, j in idx]))
+    with pytest.raises(RuntimeError, match="ta_submit_resident"):
+        eng.submit_resident([(hs[0], hs[1])], local)  # the handles are gone

@@ -457,7 +457,7 @@ void ta_destroy(ta_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->pair_stage) (void)hipHostFree(c->pair_stage);
-  if (c->seq_arena) (void)hipFree(c->seq_arena);
+  c->seq_arena.reset();  // freed here unless a batch still holds it (none after serve_shutdown)
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -858,7 +858,7 @@ struct ta_batch {
   Plan plan;
   // device
   uint8_t *d_codes = nullptr;
-  bool codes_borrowed = false;    // d_codes is the context's resident-sequence arena
+  std::shared_ptr<uint8_t> codes_owner;  // set when d_codes is a resident-sequence arena (held, not freed)
   int8_t *d_table = nullptr;
   int8_t *d_table_nib = nullptr;  // packed couples: 4-bit biased scores
   bool nib = false;
@@ -952,7 +952,7 @@ void ta_batch_destroy(ta_batch *B) {
   // ev[3]) must have finished first.  The event is the batch's own, so this
   // holds even where the run's stream is gone.
   if (B->ev[3]) (void)hipEventSynchronize(B->ev[3]);
-  if (B->codes_borrowed) B->d_codes = nullptr;
+  if (B->codes_owner) B->d_codes = nullptr;
   dfree(B->d_codes); dfree(B->d_table); dfree(B->d_table_nib);
   for (int g = 0; g < 3; ++g) { dfree(B->d_tasks2[g]); dfree(B->d_tasks1[g]); }
   dfree(B->d_rowcodes); dfree(B->d_colcodes); dfree(B->d_ring); dfree(B->d_tb);
@@ -992,17 +992,11 @@ int ta_batch_create(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
 }
 
 int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, hipStream_t stream,
-                    ta_batch **out, const ta_tuning *force, const ta_seq_pair *res) {
+                    ta_batch **out, const ta_tuning *force, const ResidentBatch *res) {
   if (!ctx || !p || !out || n < 0 || (n > 0 && !pairs && !res)) return TA_E_INVALID;
   *out = nullptr;
   if (params_check(p)) return TA_E_INVALID;
-  if (res) {  // resident sequences: known handles of this alphabet
-    const int64_t ns = (int64_t)ctx->seqs.size();
-    for (int32_t k = 0; k < n; ++k)
-      if (res[k].a < 0 || res[k].b < 0 || res[k].a >= ns || res[k].b >= ns ||
-          ctx->seqs[res[k].a].alphabet != p->alphabet || ctx->seqs[res[k].b].alphabet != p->alphabet)
-        return TA_E_INVALID;
-  }
+  if (res && ((int64_t)res->la.size() != n || !res->arena)) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
   PhaseTrace tr("create");
 
@@ -1025,8 +1019,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     int bucket = 0;
     if (res) {  // the bucket from the resident sequences' lengths
       std::vector<ta_pair> lens((size_t)n);
-      for (int32_t k = 0; k < n; ++k)
-        lens[k] = ta_pair{nullptr, ctx->seqs[res[k].a].len, nullptr, ctx->seqs[res[k].b].len};
+      for (int32_t k = 0; k < n; ++k) lens[k] = ta_pair{nullptr, res->la[k], nullptr, res->lb[k]};
       bucket = ta_tune_bucket(lens.data(), n);
     } else {
       bucket = ta_tune_bucket(pairs, n);
@@ -1055,7 +1048,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   // lengths; their letters are on the device).
   auto pair_at = [&](int32_t k) -> ta_pair {
     if (!res) return pairs[k];
-    const int32_t la = ctx->seqs[res[k].a].len, lb = ctx->seqs[res[k].b].len;
+    const int32_t la = res->la[k], lb = res->lb[k];
     return ta_pair{la ? "" : nullptr, la, lb ? "" : nullptr, lb};
   };
   for (int32_t k = 0; k < n; ++k)
@@ -1208,13 +1201,6 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
     }
     return e.bound;
   };
-  // The same bound for a resident sequence, from its residue counts.
-  auto bound_res = [&](ta_seq h) -> int64_t {
-    const int32_t *cnt = ctx->seq_counts.data() + 32 * (size_t)h;
-    int64_t sum = 0;
-    for (int r = 0; r < B->alph.K; ++r) sum += (int64_t)cnt[r] * best[r];
-    return sum;
-  };
   // The letters go up while the host plans: from the pinned block the copy
   // runs on the DMA engine alone; the encoding kernel follows on the stream.
   hipStream_t st = B->stream;
@@ -1223,8 +1209,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   int32_t *d_bad = nullptr;
   int32_t bad_letter = 0;
   if (res) {  // resident sequences: the codes are on the device already
-    B->d_codes = ctx->seq_arena;
-    B->codes_borrowed = true;
+    B->codes_owner = res->arena;
+    B->d_codes = res->arena.get();
   } else {
     rc = dalloc(&B->d_codes, cend);
   }
@@ -1260,8 +1246,8 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   for (int32_t k = 0; k < n; ++k) {
     const ta_pair q = pair_at(k);
     HostPair &h = B->hp[k];
-    h.a_off = res ? ctx->seqs[res[k].a].off : uq[ua[k]].off;
-    h.b_off = res ? ctx->seqs[res[k].b].off : uq[ub[k]].off;
+    h.a_off = res ? res->a_off[k] : uq[ua[k]].off;
+    h.b_off = res ? res->b_off[k] : uq[ub[k]].off;
     h.la = q.la; h.lb = q.lb;
     B->cells += (int64_t)q.la * q.lb;
     if (q.la == 0 || q.lb == 0) {
@@ -1281,8 +1267,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
       h.path = 1;  // packed end-cell keys hold the column in 15 bits
     } else if (local) {
       h.path = 2;
-      h.maysat = (res ? std::min(bound_res(res[k].a), bound_res(res[k].b)) : std::min(bound(ua[k]), bound(ub[k]))) >
-                 sat_limit;
+      h.maysat = (res ? std::min(res->a_bound[k], res->b_bound[k]) : std::min(bound(ua[k]), bound(ub[k]))) > sat_limit;
       if (h.maysat) B->may_saturate = true;
     } else {
       h.path = global_i16_safe(*p, B->alph.maxS, q.la, q.lb) ? 2 : 1;
@@ -2072,23 +2057,23 @@ int ta_seq_add(ta_ctx *ctx, int32_t alphabet, const char *letters, int32_t len, 
     codes[i] = (uint8_t)c;
     ++cnt[c];
   }
+  std::lock_guard<std::mutex> g(ctx->seq_mu);
   const size_t off = ctx->seq_used, need = off + span + 32;
   if (need > ctx->seq_cap) {
+    // a larger arena with the old contents; the old one is freed once no
+    // batch holds it
     const size_t cap = std::max(need, std::max(2 * ctx->seq_cap, (size_t)16 << 20));
     uint8_t *d = nullptr;
     if (hipMalloc((void **)&d, cap) != hipSuccess) {
       (void)hipGetLastError();
       return TA_E_NOMEM;
     }
-    if (off && hipMemcpy(d, ctx->seq_arena, off, hipMemcpyDeviceToDevice) != hipSuccess) {
-      (void)hipFree(d);
-      return TA_E_DEVICE;
-    }
-    if (ctx->seq_arena) (void)hipFree(ctx->seq_arena);
-    ctx->seq_arena = d;
+    std::shared_ptr<uint8_t> arena(d, [](uint8_t *q) { (void)hipFree(q); });
+    if (off && hipMemcpy(d, ctx->seq_arena.get(), off, hipMemcpyDeviceToDevice) != hipSuccess) return TA_E_DEVICE;
+    ctx->seq_arena = std::move(arena);
     ctx->seq_cap = cap;
   }
-  HIPCHK(hipMemcpy(ctx->seq_arena + off, codes.data(), span + 32, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->seq_arena.get() + off, codes.data(), span + 32, hipMemcpyHostToDevice));
   ctx->seq_used = off + span;
   ctx->seqs.push_back(ta_ctx::SeqRec{off, len, alphabet});
   ctx->seq_counts.insert(ctx->seq_counts.end(), cnt.begin(), cnt.end());
@@ -2098,20 +2083,54 @@ int ta_seq_add(ta_ctx *ctx, int32_t alphabet, const char *letters, int32_t len, 
 
 int ta_seq_reset(ta_ctx *ctx) {
   if (!ctx) return TA_E_INVALID;
-  HIPCHK(hipSetDevice(ctx->device));
-  if (ctx->seq_arena) HIPCHK(hipFree(ctx->seq_arena));
-  ctx->seq_arena = nullptr;
+  std::lock_guard<std::mutex> g(ctx->seq_mu);
+  ctx->seq_arena.reset();  // freed now, or when the last batch that holds it is
   ctx->seq_cap = ctx->seq_used = 0;
   ctx->seqs.clear();
   ctx->seq_counts.clear();
   return TA_OK;
 }
 
+int resident_resolve(ta_ctx *ctx, const ta_params *p, const ta_seq_pair *pairs, int32_t n, ResidentBatch &out) {
+  if (!ctx || !p || n < 0 || (n > 0 && !pairs) || params_check(p)) return TA_E_INVALID;
+  const Alphabet A = make_alphabet(p->alphabet, p->match, p->mismatch);
+  int best[32] = {0};  // best score of each residue against any letter (0 if none is positive)
+  for (int r = 0; r < A.K; ++r)
+    for (int c = 0; c < A.K; ++c) best[r] = std::max<int>(best[r], A.S[r * A.K + c]);
+  std::lock_guard<std::mutex> g(ctx->seq_mu);
+  const int64_t ns = (int64_t)ctx->seqs.size();
+  auto bound = [&](ta_seq h) {
+    const int32_t *cnt = ctx->seq_counts.data() + 32 * (size_t)h;
+    int64_t sum = 0;
+    for (int r = 0; r < A.K; ++r) sum += (int64_t)cnt[r] * best[r];
+    return sum;
+  };
+  out = ResidentBatch();
+  out.a_off.resize(n); out.b_off.resize(n);
+  out.la.resize(n); out.lb.resize(n);
+  out.a_bound.resize(n); out.b_bound.resize(n);
+  for (int32_t k = 0; k < n; ++k) {
+    const ta_seq a = pairs[k].a, b = pairs[k].b;
+    if (a < 0 || b < 0 || a >= ns || b >= ns || ctx->seqs[a].alphabet != p->alphabet ||
+        ctx->seqs[b].alphabet != p->alphabet)
+      return TA_E_INVALID;
+    out.a_off[k] = ctx->seqs[a].off; out.b_off[k] = ctx->seqs[b].off;
+    out.la[k] = ctx->seqs[a].len; out.lb[k] = ctx->seqs[b].len;
+    out.a_bound[k] = bound(a); out.b_bound[k] = bound(b);
+  }
+  out.arena = ctx->seq_arena;
+  if (n > 0 && !out.arena) return TA_E_INVALID;
+  return TA_OK;
+}
+
 int ta_align_resident(ta_ctx *ctx, const ta_params *p, const ta_seq_pair *pairs, int32_t n, ta_result *out,
                       char *cigar_buf, int64_t cap, int64_t *used) {
   if (!ctx || n < 0 || (n > 0 && (!pairs || !out))) return TA_E_INVALID;
+  ResidentBatch rb;
+  int rc = resident_resolve(ctx, p, pairs, n, rb);
+  if (rc) return rc;
   ta_batch *B = nullptr;
-  int rc = batch_create_on(ctx, p, nullptr, n, ctx->stream, &B, nullptr, pairs);
+  rc = batch_create_on(ctx, p, nullptr, n, ctx->stream, &B, nullptr, &rb);
   if (rc) return rc;
   rc = ta_batch_run(ctx, B);
   if (!rc) rc = ta_batch_wait(ctx, B);

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -36,16 +37,33 @@ struct ta_ctx {
   // sequence 16-byte aligned and padded with code 0, 32 bytes of slack after
   // the last (the walkers' windows); per sequence its offset, length,
   // alphabet and the count of each residue code (the int16 saturation bound
-  // of a batch depends on its scores).
+  // of a batch depends on its scores).  Batches hold the arena they read
+  // (shared), so growing or resetting the store never frees an arena a
+  // served request still reads.  seq_mu guards the store: serving threads
+  // read it while the caller adds.
   struct SeqRec {
     uint64_t off;
     int32_t len, alphabet;
   };
-  uint8_t *seq_arena = nullptr;
+  std::mutex seq_mu;
+  std::shared_ptr<uint8_t> seq_arena;
   size_t seq_cap = 0, seq_used = 0;
   std::vector<SeqRec> seqs;
   std::vector<int32_t> seq_counts;  // 32 per sequence
 };
+
+// Pairs of resident sequences resolved when a call takes them (the store may
+// grow or be reset afterwards): the arena the codes are in and, per pair,
+// code offsets, lengths and the int16 saturation bound of each side under the
+// call's scores.
+struct ResidentBatch {
+  std::shared_ptr<uint8_t> arena;
+  std::vector<uint64_t> a_off, b_off;
+  std::vector<int32_t> la, lb;
+  std::vector<int64_t> a_bound, b_bound;
+};
+// TA_E_INVALID for an unknown handle or one of another alphabet than p's.
+int resident_resolve(ta_ctx *ctx, const ta_params *p, const ta_seq_pair *pairs, int32_t n, ResidentBatch &out);
 
 // Finds the device address of ta_align_pair_static's scratch array on
 // `device` (current), once per device; later calls return at once.
@@ -90,9 +108,10 @@ inline int tune_key(const ta_params *p, int bucket) {
 // ta_batch_create on a given stream (the serving loop gives each in-flight
 // batch its own).
 // `force`: use this tuning instead of the context's table (the autotuner).
-// `res`: the pairs are resident sequences (ta_seq_add) and `pairs` is unused.
+// `res`: the pairs are resident sequences (resident_resolve) and `pairs` is
+// unused.
 int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, hipStream_t stream,
-                    ta_batch **out, const ta_tuning *force = nullptr, const ta_seq_pair *res = nullptr);
+                    ta_batch **out, const ta_tuning *force = nullptr, const ResidentBatch *res = nullptr);
 // Direction-plane bytes a batch of these pairs would allocate (admission).
 int64_t batch_tb_estimate(const ta_params *p, const ta_pair *pairs, int32_t n);
 // Results of a finished batch with the CIGAR text in one string

@@ -45,6 +45,9 @@ struct Request {
   std::string cigar;
   int dq = -1;                 // runs in the device queue: -1 not known yet, 0 no, 1 yes
   DevqJob *job = nullptr;      // its job there while in flight
+  bool resident = false;       // pairs of resident sequences (ta_submit_resident), resolved in rb
+  ResidentBatch rb;
+  size_t npairs() const { return resident ? rb.la.size() : a.size(); }
 };
 
 struct Group {
@@ -188,16 +191,33 @@ void Serve::dq_loop() {
 }
 
 // One batch from build to per-request results (a worker, without the lock).
+// A group is all letters or all resident sequences of one arena.
 void Serve::run_group(Group &g, hipStream_t st) {
   std::vector<ta_pair> pairs;
-  for (Request *r : g.reqs)
-    for (size_t k = 0; k < r->a.size(); ++k)
-      pairs.push_back({r->a[k].data(), (int32_t)r->a[k].size(), r->b[k].data(), (int32_t)r->b[k].size()});
-  const int32_t np = (int32_t)pairs.size();
+  ResidentBatch rb;
+  const bool resident = g.reqs[0]->resident;
+  if (resident) {
+    rb.arena = g.reqs[0]->rb.arena;
+    for (Request *r : g.reqs) {
+      const ResidentBatch &x = r->rb;
+      rb.a_off.insert(rb.a_off.end(), x.a_off.begin(), x.a_off.end());
+      rb.b_off.insert(rb.b_off.end(), x.b_off.begin(), x.b_off.end());
+      rb.la.insert(rb.la.end(), x.la.begin(), x.la.end());
+      rb.lb.insert(rb.lb.end(), x.lb.begin(), x.lb.end());
+      rb.a_bound.insert(rb.a_bound.end(), x.a_bound.begin(), x.a_bound.end());
+      rb.b_bound.insert(rb.b_bound.end(), x.b_bound.begin(), x.b_bound.end());
+    }
+  } else {
+    for (Request *r : g.reqs)
+      for (size_t k = 0; k < r->a.size(); ++k)
+        pairs.push_back({r->a[k].data(), (int32_t)r->a[k].size(), r->b[k].data(), (int32_t)r->b[k].size()});
+  }
+  const int32_t np = resident ? (int32_t)rb.la.size() : (int32_t)pairs.size();
   ta_batch *B = nullptr;
   std::vector<ta_result> res;
   std::string text;
-  int rc = batch_create_on(ctx, &g.reqs[0]->p, pairs.data(), np, st, &B);
+  int rc = resident ? batch_create_on(ctx, &g.reqs[0]->p, nullptr, np, st, &B, nullptr, &rb)
+                    : batch_create_on(ctx, &g.reqs[0]->p, pairs.data(), np, st, &B);
   if (!rc) rc = ta_batch_run(ctx, B);
   if (!rc) rc = ta_batch_wait(ctx, B);
   if (!rc) rc = batch_results_text(ctx, B, res, text);
@@ -205,7 +225,8 @@ void Serve::run_group(Group &g, hipStream_t st) {
   std::lock_guard<std::mutex> lk(mu);
   int32_t k = 0;
   for (Request *r : g.reqs) {
-    const int32_t n = (int32_t)r->a.size();
+    const int32_t n = (int32_t)r->npairs();
+    r->rb = ResidentBatch();  // the arena is no longer read by this request
     if (r->state == RState::Cancelled) {  // cancelled while in flight: retire now
       k += n;
       all.erase(r->ticket);
@@ -288,7 +309,11 @@ void Serve::loop() {
           it = pending.erase(it);
           continue;
         }
-        if (!g.reqs.empty() && !same_params(r->p, g.reqs[0]->p)) { ++it; continue; }
+        if (!g.reqs.empty() && (!same_params(r->p, g.reqs[0]->p) || r->resident != g.reqs[0]->resident ||
+                                (r->resident && r->rb.arena != g.reqs[0]->rb.arena))) {
+          ++it;
+          continue;
+        }
         if (used + g.tb + r->tb_bytes > budget) break;
         g.tb += r->tb_bytes;
         r->state = RState::InFlight;
@@ -379,6 +404,35 @@ int ta_submit(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_t n, 
     r->a.emplace_back(pairs[k].a, (size_t)pairs[k].la);
     r->b.emplace_back(pairs[k].b, (size_t)pairs[k].lb);
   }
+  std::lock_guard<std::mutex> g(s->mu);
+  if (r->tb_bytes > s->budget) {
+    delete r;
+    return TA_E_BUDGET;
+  }
+  r->ticket = s->next_ticket++;
+  s->all[r->ticket] = r;
+  s->pending.push_back(r);
+  *ticket = r->ticket;
+  ++s->changes;
+  s->cv.notify_all();
+  return TA_OK;
+}
+
+int ta_submit_resident(ta_ctx *ctx, const ta_params *p, const ta_seq_pair *pairs, int32_t n, uint64_t *ticket) {
+  if (!ctx || !p || !ticket || n < 0 || (n > 0 && !pairs)) return TA_E_INVALID;
+  Request *r = new Request();
+  const int rc = resident_resolve(ctx, p, pairs, n, r->rb);
+  if (rc) {
+    delete r;
+    return rc;
+  }
+  r->p = *p;
+  r->resident = true;
+  r->dq = 0;  // batches only: the device queue stages letters from host memory
+  std::vector<ta_pair> lens((size_t)n);
+  for (int32_t k = 0; k < n; ++k) lens[k] = ta_pair{nullptr, r->rb.la[k], nullptr, r->rb.lb[k]};
+  r->tb_bytes = batch_tb_estimate(p, lens.data(), n);
+  Serve *s = serve_of(ctx);
   std::lock_guard<std::mutex> g(s->mu);
   if (r->tb_bytes > s->budget) {
     delete r;

@@ -81,6 +81,7 @@ def lib():
         "ta_seq_reset": ([vp], ctypes.c_int),
         "ta_align_resident": ([vp, P(Params), P(SeqPair), i32, P(CResult), ctypes.c_char_p, i64, P(i64)],
                               ctypes.c_int),
+        "ta_submit_resident": ([vp, P(Params), P(SeqPair), i32, P(ctypes.c_uint64)], ctypes.c_int),
         "ta_serve_config": ([vp, i64, i32], ctypes.c_int),
         "ta_submit": ([vp, P(Params), P(Pair), i32, P(ctypes.c_uint64)], ctypes.c_int),
         "ta_poll": ([vp, ctypes.c_uint64, P(CResult), ctypes.c_char_p, i64, P(i64)], ctypes.c_int),
@@ -343,6 +344,15 @@ class Engine:
         t = ctypes.c_uint64()
         _check(lib().ta_submit(self.ctx, ctypes.byref(params), p.arr, p.n, ctypes.byref(t)), "ta_submit")
         return t.value, p.n
+
+    def submit_resident(self, handle_pairs, params):
+        """ta_submit_resident on (handle a, handle b) pairs; (ticket, n) as submit."""
+        n = len(handle_pairs)
+        arr = (SeqPair * max(n, 1))(*[SeqPair(a, b) for a, b in handle_pairs])
+        t = ctypes.c_uint64()
+        _check(lib().ta_submit_resident(self.ctx, ctypes.byref(params), arr, n, ctypes.byref(t)),
+               "ta_submit_resident")
+        return t.value, n
 
     def poll(self, ticket, n, traceback=True):
         """None while pending, else the list of Results."""
