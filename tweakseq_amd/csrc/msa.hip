@@ -110,6 +110,24 @@ __device__ __forceinline__ int64_t join_m_at(int i, int j, int JR, int NST) {
   return (((int64_t)(i / RS) * NST + j + l) * 64 + l) * JR + (i % JR);
 }
 
+// PB of B's columns from their residue counts, in place, a thread per
+// column: PB[j][x] = sum_y S(x, y) cntB[j][y] (K x K per column, which the
+// host used to spend most of a level's preparation on).
+__global__ __launch_bounds__(256) void msa_pb_kernel(int32_t *pb, int64_t rows, const int8_t *S, int K) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= rows) return;
+  int32_t *row = pb + j * KS;
+  int32_t c[KS];
+#pragma unroll
+  for (int y = 0; y < KS; ++y) c[y] = y < K ? row[y] : 0;
+  for (int x = 0; x < K; ++x) {
+    int32_t v = 0;
+#pragma unroll
+    for (int y = 0; y < KS; ++y) v += y < K ? c[y] * (int32_t)S[x * K + y] : 0;
+    row[x] = v;
+  }
+}
+
 // Every join's match scores M(i, j) = cntA[i] . PB[j], tile by tile over
 // the whole GPU (a join's own workgroup would compute them on one CU): a
 // workgroup takes 64 rows x 64 columns of one join, stages their counts and
@@ -457,6 +475,16 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
     prof[s].cell.assign(codes.begin() + off[s], codes.begin() + off[s + 1]);
   }
   hipStream_t st = ctx->stream;
+  // The substitution table on the device (the PB kernel reads it).
+  int8_t *d_S = (int8_t *)device_alloc((size_t)K * K);
+  if (!d_S || hipMemcpy(d_S, S, (size_t)K * K, hipMemcpyHostToDevice) != hipSuccess) {
+    device_free(d_S);
+    return TA_E_DEVICE;
+  }
+  struct FreeS {
+    int8_t *p;
+    ~FreeS() { device_free(p); }
+  } free_s{d_S};
   // The levels' count tables, kept by the calling thread between levels and
   // calls: fresh multi-MB vectors pay a page fault per 4 KB.
   static thread_local std::vector<int32_t> tl_cnt, tl_pb;
@@ -497,34 +525,31 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       J.ops_off = ops_total;
       ops_total += PA.len + PB.len;
     }
+    tr.mark("level sizes");
     // each join's tables are cleared by the thread that fills them
     if (hcnt.size() < (size_t)cnt_total) hcnt.resize((size_t)cnt_total);
     if (hpb.size() < (size_t)pb_total) hpb.resize((size_t)pb_total);
-    parallel_for(ts.size(), [&](size_t q) {
-      const Profile &PA = prof[ja[ts[q]]], &PB = prof[jb[ts[q]]];
-      const JoinJob &J = jobs[q];
-      int32_t *ca = hcnt.data() + J.cnt_off;
-      std::fill(ca, ca + (size_t)PA.len * KS, 0);
-      std::fill(hpb.data() + J.pb_off, hpb.data() + J.pb_off + (size_t)PB.len * KS, 0);
-      for (size_t r = 0; r < PA.rows.size(); ++r)
-        for (int32_t i = 0; i < PA.len; ++i) {
-          const int c = PA.cell[r * PA.len + i];
-          if (c >= 0) ca[(size_t)i * KS + c] += 1;
-        }
-      // PB from B's column counts: K x K per column instead of K per residue
-      int32_t *pbj = hpb.data() + J.pb_off;
-      std::vector<int32_t> cb((size_t)PB.len * KS, 0);
-      for (size_t r = 0; r < PB.rows.size(); ++r)
-        for (int32_t j = 0; j < PB.len; ++j) {
-          const int c = PB.cell[r * PB.len + j];
-          if (c >= 0) cb[(size_t)j * KS + c] += 1;
-        }
-      for (int32_t j = 0; j < PB.len; ++j)
-        for (int32_t y = 0; y < K; ++y) {
-          const int32_t n = cb[(size_t)j * KS + y];
-          if (!n) continue;
-          for (int32_t x = 0; x < K; ++x) pbj[(size_t)j * KS + x] += n * S[x * K + y];
-        }
+    // Column counts of both sides on the host threads, joins dealt in turn;
+    // PB (K x K per column of B) is made from B's counts on the GPU.
+    const int nthr = (int)std::min<size_t>(16, ts.size());
+    parallel_tasks(nthr, [&](int t) {
+      for (size_t q = (size_t)t; q < ts.size(); q += (size_t)nthr) {
+        const Profile &PA = prof[ja[ts[q]]], &PB = prof[jb[ts[q]]];
+        const JoinJob &J = jobs[q];
+        int32_t *ca = hcnt.data() + J.cnt_off, *cb = hpb.data() + J.pb_off;
+        std::fill(ca, ca + (size_t)PA.len * KS, 0);
+        std::fill(cb, cb + (size_t)PB.len * KS, 0);
+        for (size_t r = 0; r < PA.rows.size(); ++r)
+          for (int32_t i = 0; i < PA.len; ++i) {
+            const int c = PA.cell[r * PA.len + i];
+            if (c >= 0) ca[(size_t)i * KS + c] += 1;
+          }
+        for (size_t r = 0; r < PB.rows.size(); ++r)
+          for (int32_t j = 0; j < PB.len; ++j) {
+            const int c = PB.cell[r * PB.len + j];
+            if (c >= 0) cb[(size_t)j * KS + c] += 1;
+          }
+      }
     });
     tr.mark("level prep");
     const size_t lds = (wide ? sizeof(int64_t) : sizeof(int32_t)) * 2 * (size_t)(max_lb + 2);
@@ -563,6 +588,10 @@ int ta_msa(ta_ctx *ctx, const ta_params *p, const char *const *seqs, const int32
       int2 *d_tiles = (int2 *)device_alloc(sizeof(int2) * tiles.size());
       if (!d_tiles || hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, st))
         rc = TA_E_DEVICE;
+      const int64_t pb_rows = pb_total / KS;
+      if (!rc && pb_rows > 0)
+        hipLaunchKernelGGL(msa_pb_kernel, dim3((unsigned)((pb_rows + 255) / 256)), dim3(256), 0, st, d_pb, pb_rows,
+                           d_S, K);
       if (!rc) hipLaunchKernelGGL(msa_scores_kernel, dim3((unsigned)tiles.size()), dim3(256), 0, st, d_jobs, d_tiles, d_cnt,
                                   d_pb, d_m, jr);
       void (*kern)(JoinArgs) =
