@@ -114,6 +114,12 @@ __device__ __forceinline__ int64_t join_m_at(int i, int j, int JR, int NST) {
 // column: PB[j][x] = sum_y S(x, y) cntB[j][y] (K x K per column, which the
 // host used to spend most of a level's preparation on).
 __global__ __launch_bounds__(256) void msa_pb_kernel(int32_t *pb, int64_t rows, const int8_t *S, int K) {
+  __shared__ int32_t st[KS * KS];  // S(x, y) at [x][y], zero beyond K
+  for (int e = threadIdx.x; e < KS * KS; e += blockDim.x) {
+    const int x = e / KS, y = e % KS;
+    st[e] = (x < K && y < K) ? (int32_t)S[x * K + y] : 0;
+  }
+  __syncthreads();
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= rows) return;
   int32_t *row = pb + j * KS;
@@ -123,7 +129,7 @@ __global__ __launch_bounds__(256) void msa_pb_kernel(int32_t *pb, int64_t rows, 
   for (int x = 0; x < K; ++x) {
     int32_t v = 0;
 #pragma unroll
-    for (int y = 0; y < KS; ++y) v += y < K ? c[y] * (int32_t)S[x * K + y] : 0;
+    for (int y = 0; y < KS; ++y) v += c[y] * st[x * KS + y];
     row[x] = v;
   }
 }
