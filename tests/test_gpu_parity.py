@@ -228,26 +228,34 @@ def test_workload_local_batch_prefix(eng):
     check(eng, pairs, "65,536-pair local batch, first 1,500", mode=ta.LOCAL, gap_open=-11, gap_extend=-1)
 
 
-def test_local_batch_in_eight_lane_units(eng, monkeypatch):
-    """A local batch with more couples than resident fill waves, planned with
+@pytest.mark.parametrize("mode,traceback", [(ta.GLOBAL, 1), (ta.GLOBAL, 0), (ta.LOCAL, 1)])
+def test_batch_in_eight_lane_units(eng, monkeypatch, mode, traceback):
+    """A batch with more couples than resident fill waves (local, global with
+    and without traceback), planned with
     8-lane units (64-row strips, 8 tasks per wave) beside 16-lane units and
     whole-wave tasks: by default (the smaller couples) and with every couple
-    that is not whole-wave in 8-lane units.  Both plans write fewer
+    that is not whole-wave in 8-lane units, walked inside the fill and (the
+    last) by the traceback kernel afterwards.  Both plans write fewer
     direction-plane bytes than the one without 8-lane units (fewer pad
     cells), and every pair equals the oracle."""
-    rng = random.Random(71)
+    rng = random.Random(71 + mode)
     pairs = []
     for _ in range(5000):
         a = rand_prot(rng, rng.randint(20, 300))
         pairs.append((a, mutate(rng, a, psub=0.25, pindel=0.04)))
-    params = ta.engine.make_params(mode=ta.LOCAL, gap_open=-11, gap_extend=-1)
+    params = ta.engine.make_params(mode=mode, gap_open=-11, gap_extend=-1, traceback=traceback)
     want = oracle_results(params, pairs)
     planes = {}
-    for label, limit in (("none", "0"), ("default", None), ("all", "1000000")):
+    for label, limit, walk_in_fill in (("none", "0", True), ("default", None, True), ("all", "1000000", True),
+                                       ("all, walks after the fill", "1000000", False)):
         if limit is None:
             monkeypatch.delenv("TWEAKALIGN_EIGHT_LANE", raising=False)
         else:
             monkeypatch.setenv("TWEAKALIGN_EIGHT_LANE", limit)
+        if walk_in_fill:
+            monkeypatch.delenv("TWEAKALIGN_WALK_IN_FILL", raising=False)
+        else:
+            monkeypatch.setenv("TWEAKALIGN_WALK_IN_FILL", "0")
         b = eng.batch(pairs, params)
         planes[label] = b.info()["tb_bytes"]
         b.run()
@@ -255,7 +263,8 @@ def test_local_batch_in_eight_lane_units(eng, monkeypatch):
         got = b.results()
         b.close()
         compare(got, want, params, pairs, "8-lane units: " + label)
-    assert planes["default"] < planes["none"] and planes["all"] < planes["default"], planes
+    if params.mode == ta.LOCAL or traceback:
+        assert planes["default"] < planes["none"] and planes["all"] < planes["default"], planes
 
 
 @pytest.mark.parametrize("mode", [ta.GLOBAL, ta.LOCAL])

@@ -72,7 +72,7 @@ struct Lane<1> {
 
 struct FillArgs {
   const Task *tasks[3];  // [0] whole-wave tasks (GG = 64), [1] wave units of NGW G-lane tasks,
-                         // [2] wave units of 8 8-lane tasks (local packed couples)
+                         // [2] wave units of 8 8-lane tasks (packed couples)
   int32_t ntasks[3];     // tasks, wave units, wave units
   int32_t *ctr;          // work-queue heads ctr[0], ctr[1]
   int32_t *ctr8;         // work-queue head of tasks[2]
@@ -964,7 +964,7 @@ __global__ __launch_bounds__(TB ? 128 * WALK_FW : 64) void fill_kernel(FillArgs 
     fill_phase<GWAVE, P, LOCAL, TB>(A, A.tasks[0] + ncoop, A.ntasks[0] - ncoop, A.ctr, tab, slot, wq, wtail);
   if (A.ntasks[1])
     fill_phase<G, P, LOCAL, TB>(A, A.tasks[1], A.ntasks[1], A.ctr + 1, tab, slot, wq, wtail);
-  if constexpr (LOCAL && P == 2) {
+  if constexpr (P == 2) {
     if (A.ntasks[2]) fill_phase<8, P, LOCAL, TB>(A, A.tasks[2], A.ntasks[2], A.ctr8, tab, slot, wq, wtail);
   }
   if constexpr (TB) {

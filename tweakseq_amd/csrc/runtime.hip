@@ -506,7 +506,7 @@ struct CodeJob {
 
 struct Plan {
   // [0] whole-wave groups, [1] G-lane groups (NGW per wave unit), [2] 8-lane
-  // groups (8 per wave unit; local packed couples only, tasks1[2] stays empty)
+  // groups (8 per wave unit; packed couples only, tasks1[2] stays empty)
   std::vector<Task> tasks2[3], tasks1[3];
   std::vector<CodeJob> cjobs;
   uint64_t rows_n = 0, cols_n = 0;        // rowcodes / colcodes entries planned
@@ -791,14 +791,15 @@ double group_cost(int mmax, int n) { return (double)ceil_div(mmax, G * R) * ceil
 // The same for an 8-lane group (64-row strips, a 7-column skew).
 double cost8(int mmax, int n) { return (double)ceil_div(mmax, 8 * R) * ceil_div(n + 8, 8); }
 
-// Local packed couples whose 8-lane sweep takes at most this fraction of the
-// mean work of a group slot go into 8-lane units (8 tasks per wave): 64-row
-// strips and a 7-column skew compute fewer pad cells than 128-row strips and
-// a 15-column skew (65,536-pair log-normal batch: 1.118x the real cells
-// instead of 1.164x), while the 16-lane units keep the tasks whose 8-lane
-// units would be long enough to end the run late.  A list-scheduling model
-// of that batch gives the shortest run between 0.4x and 0.9x; 8-lane units
-// of every task end it ~50% late.
+// Packed couples whose 8-lane sweep takes at most this fraction of the mean
+// work of a group slot go into 8-lane units (8 tasks per wave; global ones of
+// equal column counts): 64-row strips and a 7-column skew compute fewer pad
+// cells than 128-row strips and a 15-column skew (65,536-pair log-normal
+// local batch: 1.11x the real cells instead of 1.164x; the 130,816-pair
+// all-vs-all fill 4.07 -> 3.85 ms), while the 16-lane units keep the tasks
+// whose 8-lane units would be long enough to end the run late.  A
+// list-scheduling model of the local batch gives the shortest run between
+// 0.4x and 0.9x; 8-lane units of every task end it ~50% late.
 double eight_lane_limit() {
   const char *e = getenv("TWEAKALIGN_EIGHT_LANE");  // fraction; "0" turns 8-lane units off
   return e ? atof(e) : 0.5;
@@ -1357,7 +1358,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   std::vector<Unit> u2, u1, u8;
   pl.tasks2[0].reserve(couples.size());
   pl.tasks1[0].reserve(i32.size());
-  const double lim8 = local && !few ? eight_lane_limit() * mean_slot : -1.0;
+  const double lim8 = !few ? eight_lane_limit() * mean_slot : -1.0;
   for (auto &c : couples) {
     const bool single = c.second < 0;
     const int hi = single ? c.first : c.second;
