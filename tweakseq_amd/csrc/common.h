@@ -5,7 +5,7 @@
 namespace ta {
 
 // A DP group is 16 lanes (one DPP row; a wave64 runs 4 groups), 8 lanes (two
-// per row, interleaved; a wave runs 8: the smaller local couples of a large
+// per row, interleaved; a wave runs 8: the smaller packed couples of a large
 // batch, whose 64-row strips waste fewer pad rows) or the whole 64-lane wave
 // (long tasks).  A group owns one task (a pair, or a couple of pairs packed
 // into int16 halves) and sweeps it strip by strip.
