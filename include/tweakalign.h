@@ -174,7 +174,8 @@ typedef struct {
   ta_seq a, b;
 } ta_seq_pair;
 int ta_seq_add(ta_ctx *ctx, int32_t alphabet, const char *letters, int32_t len, ta_seq *out);
-                                      /* TA_E_ALPHABET: a letter outside the alphabet */
+                                      /* TA_E_ALPHABET: a letter outside the alphabet;
+                                         TA_E_INVALID: len outside [0, 1,000,000] */
 int ta_seq_reset(ta_ctx *ctx);        /* forgets every resident sequence */
 /* ta_align_batch on resident sequences: same results and return codes;
  * TA_E_INVALID for an unknown handle or one of another alphabet than

@@ -106,7 +106,13 @@ def test_resident_store_grows_and_resets(eng):
 
 def test_resident_errors(eng):
     eng.seq_reset()
+    prot0 = ta.engine.make_params(mode=ta.LOCAL, gap_open=-11, gap_extend=-1)
+    # no pairs: nothing to do, with or without sequences in the store (as
+    # ta_align_batch with no pairs)
+    assert eng.align([], prot0) == []
+    assert eng.align_resident([], prot0) == []
     p = eng.seq_add("ACDEFGHIK")
+    assert eng.align_resident([], prot0) == []
     d = eng.seq_add("ACGTACGT", alphabet=ta.DNA)
     with pytest.raises(RuntimeError, match="outside the alphabet|alphabet"):
         eng.seq_add("AC#DE")

@@ -997,7 +997,7 @@ int batch_create_on(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32
   if (!ctx || !p || !out || n < 0 || (n > 0 && !pairs && !res)) return TA_E_INVALID;
   *out = nullptr;
   if (params_check(p)) return TA_E_INVALID;
-  if (res && ((int64_t)res->la.size() != n || !res->arena)) return TA_E_INVALID;
+  if (res && ((int64_t)res->la.size() != n || (n > 0 && !res->arena))) return TA_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
   PhaseTrace tr("create");
 
@@ -2040,9 +2040,9 @@ int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_
 
 // ---------------------------------------------------------------------------
 // Resident sequences: codes encoded on the host once, appended to the
-// context's device arena (grown by copying into a larger one: no batch holds
-// the arena between calls, since ta_align_resident returns only after its
-// batch is freed), aligned by handle.
+// context's device arena (grown by copying into a larger one; batches and
+// served requests that still read the old arena hold it until they are
+// freed), aligned by handle.
 int ta_seq_add(ta_ctx *ctx, int32_t alphabet, const char *letters, int32_t len, ta_seq *out) {
   if (!ctx || !out || len < 0 || len > 1000000 || (len > 0 && !letters)) return TA_E_INVALID;
   if (alphabet != TA_PROTEIN && alphabet != TA_DNA) return TA_E_INVALID;
