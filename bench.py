@@ -402,6 +402,7 @@ def main():
         out["align_resident_e2e"] = {
             "ms": round(min(resd) * 1e3, 1), "gcups": round(cells / min(resd) / 1e9, 1),
             "sequences": 2 * w3.npairs, "seq_add_s": round(add_s, 2),
+            "first_call_ms": round(resd[0] * 1e3, 1),  # includes the one upload of the added codes
             "cigar_text_equals_one_call": ru.value == want_used and rbuf.raw[:want_used] == buf.raw[:want_used],
             "form": "sequences added once (ta_seq_add), one ta_align_resident call: handles in, CIGAR text out"}
         log("resident sequences (handles -> host CIGARs): %.1f ms, %.1f GCUPS (adding %d sequences: %.2f s)"

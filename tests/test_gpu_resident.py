@@ -89,13 +89,15 @@ def test_resident_store_grows_and_resets(eng):
     rng = random.Random(5)
     eng.seq_reset()
     small = ["".join(rng.choice(AA) for _ in range(rng.randint(50, 200))) for _ in range(6)]
+    params = ta.engine.make_params(mode=ta.GLOBAL, gap_open=-10, gap_extend=-1)
     before = [eng.seq_add(s) for s in small[:3]]
+    # the first call uploads the codes added so far
+    same(eng.align_resident([(before[0], before[1])], params), oracle(params, [(small[0], small[1])]))
     big = "".join(rng.choice(AA) for _ in range(1000000))
-    for _ in range(20):  # 20 MB: past the first arena
-        eng.seq_add(big)
+    for _ in range(70):  # 70 MB: past the first arena (its uploaded part copied over) and
+        eng.seq_add(big)  # past the 64 MB of codes held back before an upload
     after = [eng.seq_add(s) for s in small[3:]]
     hs = before + after
-    params = ta.engine.make_params(mode=ta.GLOBAL, gap_open=-10, gap_extend=-1)
     idx = [(i, j) for i in range(6) for j in range(6)]
     got = eng.align_resident([(hs[i], hs[j]) for i, j in idx], params)
     same(got, oracle(params, [(small[i], small[j]) for i, j in idx]))

@@ -2042,7 +2042,23 @@ int ta_align_batch(ta_ctx *ctx, const ta_params *p, const ta_pair *pairs, int32_
 // Resident sequences: codes encoded on the host once, appended to the
 // context's device arena (grown by copying into a larger one; batches and
 // served requests that still read the old arena hold it until they are
-// freed), aligned by handle.
+// freed), aligned by handle.  Codes added since the last upload wait in host
+// memory and go up in one copy (seq_flush) when a call resolves handles, or
+// before an add once 64 MB are pending.
+
+// Uploads the pending codes (the caller holds seq_mu).
+static int seq_flush(ta_ctx *ctx) {
+  if (ctx->seq_pending.empty()) return TA_OK;
+  if (hipMemcpy(ctx->seq_arena.get() + ctx->seq_flushed, ctx->seq_pending.data(), ctx->seq_pending.size(),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    return TA_E_DEVICE;
+  }
+  ctx->seq_flushed = ctx->seq_used;  // the 32 zero bytes after it are on the device too
+  ctx->seq_pending.clear();
+  return TA_OK;
+}
+
 int ta_seq_add(ta_ctx *ctx, int32_t alphabet, const char *letters, int32_t len, ta_seq *out) {
   if (!ctx || !out || len < 0 || len > 1000000 || (len > 0 && !letters)) return TA_E_INVALID;
   if (alphabet != TA_PROTEIN && alphabet != TA_DNA) return TA_E_INVALID;
@@ -2050,19 +2066,18 @@ int ta_seq_add(ta_ctx *ctx, int32_t alphabet, const char *letters, int32_t len, 
   const Alphabet A = make_alphabet(alphabet, 1, -1);
   // the codes, code 0 up to the next 16-byte start, then the walkers' slack
   const size_t span = ((size_t)len + 15) & ~(size_t)15;
-  std::vector<uint8_t> codes(span + 32, 0);
-  std::vector<int32_t> cnt(32, 0);
+  int32_t cnt[32] = {0};
   for (int32_t i = 0; i < len; ++i) {
     const int c = A.map[(unsigned char)letters[i]];
     if (c < 0) return TA_E_ALPHABET;
-    codes[i] = (uint8_t)c;
     ++cnt[c];
   }
   std::lock_guard<std::mutex> g(ctx->seq_mu);
+  if (ctx->seq_pending.size() >= ((size_t)64 << 20) && seq_flush(ctx)) return TA_E_DEVICE;
   const size_t off = ctx->seq_used, need = off + span + 32;
   if (need > ctx->seq_cap) {
-    // a larger arena with the old contents; the old one is freed once no
-    // batch holds it
+    // a larger arena with the uploaded contents (the pending ones follow at
+    // the next flush); the old one is freed once no batch holds it
     const size_t cap = std::max(need, std::max(2 * ctx->seq_cap, (size_t)16 << 20));
     uint8_t *d = nullptr;
     if (hipMalloc((void **)&d, cap) != hipSuccess) {
@@ -2070,14 +2085,24 @@ int ta_seq_add(ta_ctx *ctx, int32_t alphabet, const char *letters, int32_t len, 
       return TA_E_NOMEM;
     }
     std::shared_ptr<uint8_t> arena(d, [](uint8_t *q) { (void)hipFree(q); });
-    if (off && hipMemcpy(d, ctx->seq_arena.get(), off, hipMemcpyDeviceToDevice) != hipSuccess) return TA_E_DEVICE;
+    if (ctx->seq_flushed &&
+        hipMemcpy(d, ctx->seq_arena.get(), ctx->seq_flushed, hipMemcpyDeviceToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      return TA_E_DEVICE;
+    }
     ctx->seq_arena = std::move(arena);
     ctx->seq_cap = cap;
   }
-  HIPCHK(hipMemcpy(ctx->seq_arena.get() + off, codes.data(), span + 32, hipMemcpyHostToDevice));
+  // pending bytes [seq_flushed, off + span + 32): this sequence's codes
+  // replace the previous one's trailing slack
+  std::vector<uint8_t> &pend = ctx->seq_pending;
+  const size_t at = off - ctx->seq_flushed;
+  pend.resize(at + span + 32);
+  std::fill(pend.begin() + at, pend.end(), (uint8_t)0);
+  for (int32_t i = 0; i < len; ++i) pend[at + i] = (uint8_t)A.map[(unsigned char)letters[i]];
   ctx->seq_used = off + span;
   ctx->seqs.push_back(ta_ctx::SeqRec{off, len, alphabet});
-  ctx->seq_counts.insert(ctx->seq_counts.end(), cnt.begin(), cnt.end());
+  ctx->seq_counts.insert(ctx->seq_counts.end(), cnt, cnt + 32);
   *out = (ta_seq)(ctx->seqs.size() - 1);
   return TA_OK;
 }
@@ -2086,7 +2111,9 @@ int ta_seq_reset(ta_ctx *ctx) {
   if (!ctx) return TA_E_INVALID;
   std::lock_guard<std::mutex> g(ctx->seq_mu);
   ctx->seq_arena.reset();  // freed now, or when the last batch that holds it is
-  ctx->seq_cap = ctx->seq_used = 0;
+  ctx->seq_cap = ctx->seq_used = ctx->seq_flushed = 0;
+  ctx->seq_pending.clear();
+  ctx->seq_pending.shrink_to_fit();
   ctx->seqs.clear();
   ctx->seq_counts.clear();
   return TA_OK;
@@ -2119,8 +2146,9 @@ int resident_resolve(ta_ctx *ctx, const ta_params *p, const ta_seq_pair *pairs, 
     out.la[k] = ctx->seqs[a].len; out.lb[k] = ctx->seqs[b].len;
     out.a_bound[k] = bound(a); out.b_bound[k] = bound(b);
   }
+  if (n > 0 && !ctx->seq_arena) return TA_E_INVALID;
+  if (n > 0 && seq_flush(ctx)) return TA_E_DEVICE;
   out.arena = ctx->seq_arena;
-  if (n > 0 && !out.arena) return TA_E_INVALID;
   return TA_OK;
 }
 

@@ -39,15 +39,19 @@ struct ta_ctx {
   // alphabet and the count of each residue code (the int16 saturation bound
   // of a batch depends on its scores).  Batches hold the arena they read
   // (shared), so growing or resetting the store never frees an arena a
-  // served request still reads.  seq_mu guards the store: serving threads
-  // read it while the caller adds.
+  // served request still reads.  Added codes wait in host memory
+  // (seq_pending: arena bytes [seq_flushed, seq_used + 32)) and go up in one
+  // copy when a call resolves handles, or before an add once 64 MB are
+  // pending.
+  // seq_mu guards the store: serving threads read it while the caller adds.
   struct SeqRec {
     uint64_t off;
     int32_t len, alphabet;
   };
   std::mutex seq_mu;
   std::shared_ptr<uint8_t> seq_arena;
-  size_t seq_cap = 0, seq_used = 0;
+  size_t seq_cap = 0, seq_used = 0, seq_flushed = 0;
+  std::vector<uint8_t> seq_pending;
   std::vector<SeqRec> seqs;
   std::vector<int32_t> seq_counts;  // 32 per sequence
 };
