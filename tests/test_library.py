@@ -121,3 +121,15 @@ def test_tune_bucket_by_median_length():
         ps = _Pairs([("A" * n, "C" * max(1, n // 2)) for n in lens])
         assert L.ta_tune_bucket(ps.arr, ps.n) == want, lens
     assert L.ta_tune_bucket(None, 0) == 0
+    # element n // 2 of the sorted max(la, lb), odd and even counts, lengths
+    # at and around the bucket limits
+    import random
+    rng = random.Random(11)
+    edges = [1, 255, 256, 257, 1023, 1024, 1025, 4095, 4096, 4097, 6000]
+    for _ in range(200):
+        n = rng.randint(1, 12)
+        prs = [(rng.choice(edges), rng.choice(edges)) for _ in range(n)]
+        med = sorted(max(a, b) for a, b in prs)[n // 2]
+        want = 0 if med < 256 else (1 if med < 1024 else (2 if med < 4096 else 3))
+        ps = _Pairs([("A" * a, "C" * b) for a, b in prs])
+        assert L.ta_tune_bucket(ps.arr, ps.n) == want, prs

@@ -2294,11 +2294,17 @@ int ta_batch_run_plain(ta_ctx *ctx, ta_batch *B, int32_t first, int32_t count) {
 // kept per kind of batch on the context.
 int ta_tune_bucket(const ta_pair *pairs, int32_t n) {
   if (!pairs || n <= 0) return 0;
-  std::vector<int32_t> len(n);
-  for (int32_t k = 0; k < n; ++k) len[k] = std::max(pairs[k].la, pairs[k].lb);
-  std::nth_element(len.begin(), len.begin() + n / 2, len.end());
-  const int32_t med = len[n / 2];
-  return med < 256 ? 0 : (med < 1024 ? 1 : (med < 4096 ? 2 : 3));
+  // The median (element n / 2 in sorted order) is below a limit exactly when
+  // more than n / 2 lengths are: counts against the three limits, no sort.
+  int32_t c[3] = {0, 0, 0};
+  for (int32_t k = 0; k < n; ++k) {
+    const int32_t v = std::max(pairs[k].la, pairs[k].lb);
+    c[0] += v < 256;
+    c[1] += v < 1024;
+    c[2] += v < 4096;
+  }
+  const int32_t h = n / 2;
+  return c[0] > h ? 0 : (c[1] > h ? 1 : (c[2] > h ? 2 : 3));
 }
 
 int ta_tune_set(ta_ctx *ctx, const ta_params *p, int32_t bucket, const ta_tuning *t) {
