@@ -209,8 +209,9 @@ def main():
                     help="requests of the served run (120,000 = the 60 s Poisson schedule at 2,000/s)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
-              "(pairs are independent; sharding them is a host-side split, see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the library runs on a single MI355X and has no multi-GPU path "
+              "(pairs are independent: sharding them is a host-side split of the pair list, one context "
+              "per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import tweakseq_amd as ta
