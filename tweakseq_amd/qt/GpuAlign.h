@@ -7,8 +7,9 @@
 //  * in-process path (alignPair): a global alignment of two residue strings
 //    on the GPU through ta_align_pair, rendered as two gapped rows -- what the
 //    editor's realign-selection splices back (tweakseq/Core/Project.cpp:964-1026).
-// Built into tweakseq next to Muscle.cpp; see INTEGRATION.md for the
-// registration sites.
+// Built into tweakseq next to Muscle.cpp and registered by name where the
+// tools are listed (tweakseq/Core/Application.cpp:196-204,
+// tweakseq/Core/Project.cpp:1142-1152, tweakseq/UI/SetupWizard.cpp:207-209).
 #ifndef GPU_ALIGN_H
 #define GPU_ALIGN_H
 
