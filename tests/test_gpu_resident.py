@@ -160,6 +160,7 @@ def test_served_resident_requests_equal_oracle(eng):
             for _ in range(18):
                 eng.seq_add(big)
     eng.seq_reset()
+    tickets.append(eng.submit_resident([], local) + (local, []))  # no pairs, empty store
     for t, n, params, idx in tickets:
         deadline = time.time() + 60
         got = eng.poll(t, n, params.traceback)
