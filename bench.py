@@ -172,8 +172,8 @@ def cpu_long_dna(workload, npairs=4):
     from oracle import pyoracle as po
     codes, a_off, la, b_off, lb, cells = oracle_inputs(workload, range(npairs))
     t0 = time.perf_counter()
-    po.align_batch(oracle_params(workload, traceback=0), codes, a_off, la, b_off, lb, nthreads=npairs,
-                   with_cigar=False)
+    res, _ = po.align_batch(oracle_params(workload, traceback=0), codes, a_off, la, b_off, lb, nthreads=npairs,
+                            with_cigar=False)
     dt = time.perf_counter() - t0
     total = workload.cells()
     return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": npairs, "kind": "port",
@@ -181,7 +181,22 @@ def cpu_long_dna(workload, npairs=4):
                       % (npairs, workload.npairs, cells, dt),
             "extrapolated_s_all_pairs": round(dt * total / cells / npairs, 1),
             "note": "extrapolated: all %d pairs' cells at this sample's per-thread rate, on %d threads"
-                    % (workload.npairs, npairs)}
+                    % (workload.npairs, npairs)}, res
+
+
+def cigar_consumes(results, workload):
+    """Pairs whose CIGAR does not consume both sequences exactly (M and I
+    take query residues, M and D target residues), or that carry the
+    walk-failure flag."""
+    import re
+    bad = []
+    for k, g in enumerate(results):
+        ops = re.findall(r"(\d+)([MID])", g.cigar)
+        qa = sum(int(n) for n, o in ops if o != "D")
+        qb = sum(int(n) for n, o in ops if o != "I")
+        if qa != workload.arr[k].la or qb != workload.arr[k].lb or g.flags & 0x100:
+            bad.append(k)
+    return bad
 
 
 def fill_traffic():
@@ -282,6 +297,8 @@ def main():
     if tr:
         out["roofline"]["hbm_gbs"] = round(tr["bytes_per_launch"] / (fill_avg / 1e3) / 1e9, 1)
 
+    bad_parity = False  # set by any check of GPU results against the oracle or their own invariants
+    res5 = None  # the long DNA pairs' results (extras only)
     if not args.no_extra:
         # The same batch with the walks in their own kernel after the fill:
         # the fill kernel alone against the roofline.
@@ -463,7 +480,12 @@ def main():
         c5 = b5.info()["cells"]
         wall5, f5, _ = timed_steps(b5, 2, 1)
         t5 = b5.timings()
+        res5 = b5.results()
         b5.close()
+        bad5 = cigar_consumes(res5, w5)
+        out["long_dna_cigars"] = {"pairs": w5.npairs, "bad": len(bad5), "first_bad": bad5[:5],
+                                  "checked": "every CIGAR consumes both sequences exactly, no walk-failure flag"}
+        bad_parity = bad_parity or len(bad5) > 0
         out["long_dna_gcups"] = round(c5 / (wall5 / 2) / 1e9, 1)
         out["long_dna_ms"] = round(wall5 / 2 * 1e3, 1)
         out["long_dna_fill_gcups"] = round(c5 / (sum(f5) / len(f5) / 1e3) / 1e9, 1)
@@ -572,7 +594,6 @@ def main():
 
     if tuned:
         out["autotune"] = tuned
-    bad_parity = False
     if not args.no_cpu:
         hc = host_cpu()
         out["cpu_host"] = hc
@@ -593,13 +614,21 @@ def main():
             log("cpu all-vs-all:", b2base)
             log("parity, all-vs-all:", par2)
             out["cpu_single_pair"] = cpu_single_pair(ta.Workload(1))
-            out["cpu_long_dna"] = cpu_long_dna(ta.Workload(5))
+            out["cpu_long_dna"], resl = cpu_long_dna(ta.Workload(5))
+            if res5 is not None:
+                bad = parity(res5[:len(resl)], resl, None, False)
+                out["parity_full"]["long_dna_first4"] = {
+                    "pairs": len(resl), "mismatches": len(bad), "first_bad": bad,
+                    "checked": "score, coordinates against oracle/gotoh.c (the CPU sample's pairs, 10k-100k bases)",
+                    "workload": "the first 4 of the 256 long DNA pairs"}
+                bad_parity = bad_parity or len(bad) > 0
             log("cpu single pair:", out["cpu_single_pair"])
             log("cpu long dna:", out["cpu_long_dna"])
     eng.close()
     print(json.dumps(out), flush=True)
     if bad_parity:
-        log("PARITY FAILURE: GPU results differ from the oracle (parity_full)")
+        log("PARITY FAILURE: GPU results differ from the oracle (parity_full) or fail their invariants "
+            "(long_dna_cigars)")
         sys.exit(3)
 
 
