@@ -574,6 +574,7 @@ def main():
                 # (half global, half local)
                 out["served"]["frac_of_roofline"] = round(out["served"]["served_gcups"] * 9.5 / VALU_PEAK_GOPS, 4)
                 out["served"]["admission"] = "device queue"
+                bad_parity = bad_parity or out["served"].get("bad_cigars", 0) > 0
                 log("served: p50 %.1f ms, p99 %.1f ms, %.1f GCUPS served of %.1f offered"
                     % (out["served"]["p50_ms"], out["served"]["p99_ms"], out["served"]["served_gcups"],
                        out["served"]["offered_gcups"]))
@@ -583,6 +584,7 @@ def main():
             if r.returncode == 0:
                 out["served_batches"] = json.loads(r.stdout.strip().splitlines()[-1])
                 out["served_batches"]["admission"] = "batches (TWEAKALIGN_DEVICE_QUEUE=0)"
+                bad_parity = bad_parity or out["served_batches"].get("bad_cigars", 0) > 0
                 log("served through batches (%d requests): p50 %.1f ms, p99 %.1f ms"
                     % (nb, out["served_batches"]["p50_ms"], out["served_batches"]["p99_ms"]))
         # Measured VALU throughput (instruction-kind microbenchmark) beside the theoretical peak.
@@ -628,7 +630,7 @@ def main():
     print(json.dumps(out), flush=True)
     if bad_parity:
         log("PARITY FAILURE: GPU results differ from the oracle (parity_full) or fail their invariants "
-            "(long_dna_cigars)")
+            "(long_dna_cigars, served bad_cigars)")
         sys.exit(3)
 
 

@@ -59,6 +59,26 @@ int main(int argc, char **argv) {
       std::this_thread::sleep_for(std::chrono::microseconds(100));
   }
 
+  // Every delivered CIGAR must consume exactly the aligned span of both
+  // sequences (query: M and I; target: M and D), and an empty local result
+  // has no CIGAR: a check of every served pair at full size.
+  auto cigar_ok = [&](const ta_result &r, const ta_pair &q, const char *text) {
+    int64_t qa = 0, qb = 0, run = 0;
+    for (int64_t c = 0; c < r.cigar_len; ++c) {
+      const char ch = text[r.cigar_off + c];
+      if (ch >= '0' && ch <= '9') { run = 10 * run + (ch - '0'); continue; }
+      if (ch != 'M' && ch != 'I' && ch != 'D') return false;
+      if (ch != 'D') qa += run;
+      if (ch != 'I') qb += run;
+      run = 0;
+    }
+    if (run) return false;
+    if (r.flags & TA_F_EMPTY) return r.cigar_len == 0;
+    return r.a_begin >= 0 && r.a_end < q.la && r.b_begin >= 0 && r.b_end < q.lb &&
+           qa == r.a_end - r.a_begin + 1 && qb == r.b_end - r.b_begin + 1;
+  };
+  std::atomic<int64_t> bad_cigars{0};
+
   std::atomic<int> submitted{0};
   std::atomic<bool> failed{false};
   const Clock::time_point t0 = Clock::now();
@@ -81,6 +101,8 @@ int main(int argc, char **argv) {
         rq[q].done = now_s();
         ++finished;
         any = true;
+        for (int k = 0; k < rq[q].count; ++k)
+          if (!cigar_ok(res[k], pairs[rq[q].first + k], cig.data())) ++bad_cigars;
       }
       if (!any) std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
@@ -125,11 +147,11 @@ int main(int argc, char **argv) {
   const double span = rq[R - 1].at;
   printf("{\"requests\": %d, \"pairs\": %d, \"cells\": %lld, \"arrival_span_s\": %.3f, \"makespan_s\": %.3f, "
          "\"offered_gcups\": %.2f, \"served_gcups\": %.2f, \"p50_ms\": %.2f, \"p99_ms\": %.2f, \"max_ms\": %.2f, "
-         "\"budget_gb\": %.1f, \"inflight\": %d, \"max_len\": %d, \"p90_ms\": %.2f, "
+         "\"budget_gb\": %.1f, \"inflight\": %d, \"max_len\": %d, \"p90_ms\": %.2f, \"bad_cigars\": %lld, "
          "\"slowest\": [%s], \"slowest_fields\": \"request, arrival_s, submit_lag_ms, latency_ms, pairs, mode, longest\"}\n",
          R, np, (long long)total_cells, span, last, total_cells / span / 1e9, total_cells / last / 1e9,
          pct(0.5) * 1e3, pct(0.99) * 1e3, lat[R - 1] * 1e3, budget_gb, inflight, max_len ? max_len : 5000,
-         pct(0.9) * 1e3, slow.c_str());
+         pct(0.9) * 1e3, (long long)bad_cigars.load(), slow.c_str());
   (void)end;
   ta_workload_destroy(w);
   return 0;
