@@ -534,7 +534,7 @@ def main():
         mt = []
         for _ in range(3):
             t0 = time.perf_counter()
-            rows_m, _ = eng.msa(seqs)
+            rows_m, joins_m = eng.msa(seqs)
             mt.append(time.perf_counter() - t0)
         out["msa_512_ms"] = round(min(mt) * 1e3, 1)
         out["msa_512_columns"] = len(rows_m[0])
@@ -551,6 +551,12 @@ def main():
             alen, cols, wj = po.msa([po.encode(x) for x in sub])
             out["msa_48_cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
             out["msa_48_matches_oracle"] = rows_s == po.msa_rows(sub, alen, cols) and joins_s == wj
+            # the whole 512-sequence alignment against the oracle (rows and joins)
+            t0 = time.perf_counter()
+            alen5, cols5, wj5 = po.msa([po.encode(x) for x in seqs])
+            out["msa_512_cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+            out["msa_512_matches_oracle"] = rows_m == po.msa_rows(seqs, alen5, cols5) and joins_m == wj5
+            bad_parity = bad_parity or not (out["msa_48_matches_oracle"] and out["msa_512_matches_oracle"])
         log("msa 512 sequences: %.1f ms on the GPU (end to end, %d columns)"
             % (out["msa_512_ms"], out["msa_512_columns"]))
         # Served form: open-loop Poisson arrivals (2,000 requests/s, 1-256
@@ -630,7 +636,7 @@ def main():
     print(json.dumps(out), flush=True)
     if bad_parity:
         log("PARITY FAILURE: GPU results differ from the oracle (parity_full) or fail their invariants "
-            "(long_dna_cigars, served bad_cigars)")
+            "(long_dna_cigars, served bad_cigars, msa_*_matches_oracle)")
         sys.exit(3)
 
 
